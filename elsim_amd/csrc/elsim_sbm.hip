@@ -1,0 +1,812 @@
+// elsim_sbm.hip -- device-resident SBM electrochemical step
+// (SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90:143-840 on one GPU).
+//
+// The whole domain lives in HBM: ghosted FP64 fields (psP, psE, CnP, CnE,
+// phP, phE and the cell-centred coefficients) plus interior work arrays.  The
+// MPI halo exchange BcMpiYZ (SBM:1031-1090) becomes in-kernel boundary rules:
+// on one rank its outer boundary is a Neumann mirror, and the d2 faces carry
+// the per-field Dirichlet / flux values (SBM:557-560, 647, 717-719, 798-800,
+// 833-835).  Every kernel restates the reference expression in its
+// evaluation order (compiled without FMA contraction), so the arithmetic
+// kernels are bit-identical to the FP64 CPU restatement; only reductions
+// (tree order) and exp/tanh (device libm) differ at the rounding level.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+
+#include "elsim_solver.h"
+
+namespace elsim {
+
+struct SbmScal {
+  double tpp, tpe, xrf, infx, bve, scrnt, dv, avcne, cvgp, cvgl, red;
+};
+
+// Generic reduction ops finished by the last block.
+enum RedOp { R_TPP = 0, R_TPE, R_XRF, R_INFX, R_AVCNE, R_CVGP, R_CVGL, R_SCRNT, R_RAW };
+
+struct BcArgs {
+  int lo, hi;          // 0 mirror, 1 value, 2 CnE flux (low face only)
+  double lo_v, hi_v;   // values (kind 1) unless lo_vp / hi_vp given
+  const double* lo_vp; // device scalar (e.g. BvE)
+  const double* dab01; // CnE flux: Dab on planes j=0 and j=1, (n1+2)(n3+2) each
+  const double* infx;  // CnE flux: device scalar
+  double tm, dx;       // CnE flux: t_minus, dx
+};
+
+namespace {
+
+__device__ __forceinline__ double table(double c, const double* __restrict__ cord, const double* __restrict__ vals,
+                                        int sz) {
+  // Tbl_Chk_Pln_3DA (SBM:998-1021)
+  const double ditv = cord[1] - cord[0];
+  if (c < 0.0) c = 1.0e-6;
+  if (c > 1.0) c = 1.0;
+  int lb = (int)floor(c / ditv);
+  if (lb == 0)
+    lb = 1;
+  else if (lb == sz)
+    lb = sz - 1;
+  // a well-formed table (uniform grid over [0,1]) never needs this clamp;
+  // it keeps malformed tables and NaN inside the arrays
+  if (lb < 1 || lb > sz - 1) lb = lb < 1 ? 1 : sz - 1;
+  return vals[lb - 1] + (c - cord[lb - 1]) / ditv * (vals[lb] - vals[lb - 1]);
+}
+
+struct Col {
+  int i, j, k0, k1;
+  bool active;
+};
+__device__ __forceinline__ Col colx(const Dims& d) {
+  Col c;
+  c.i = blockIdx.x * TX + threadIdx.x + 1;
+  c.j = blockIdx.y * TY + threadIdx.y + 1;
+  c.k0 = blockIdx.z * KC + 1;
+  c.k1 = min(c.k0 + KC - 1, d.n3);
+  c.active = c.i <= d.n1 && c.j <= d.n2;
+  return c;
+}
+template <class F>
+__device__ __forceinline__ void each(const Dims& d, const Col& ci, F&& f) {
+  if (!ci.active) return;
+  const int64_t sz = d.g12(), nz = (int64_t)d.n1 * d.n2;
+  int64_t c = d.gi(ci.i, ci.j, ci.k0), n = d.ni(ci.i, ci.j, ci.k0);
+  for (int k = ci.k0; k <= ci.k1; k++, c += sz, n += nz) f(c, n, k);
+}
+
+__device__ __forceinline__ void finish_red(int op, double tot, SbmScal* s, const elsim_sbm_params& P, const Dims& d,
+                                           double* raw) {
+  switch (op) {
+    case R_TPP: s->tpp = tot; break;
+    case R_TPE: s->tpe = tot; break;
+    case R_XRF: s->xrf = tot / s->tpp; break;
+    case R_INFX: {
+      const double lsm = tot * (P.dx * P.dx * P.dx);
+      s->infx = lsm / ((double)(d.n1 * d.n3) * (P.dx * P.dx));
+      break;
+    }
+    case R_AVCNE: s->avcne = tot / s->tpe; break;
+    case R_CVGP: s->cvgp = sqrt(tot / s->tpp); break;
+    case R_CVGL: s->cvgl = sqrt(tot / s->tpe); break;
+    case R_SCRNT: {
+      s->scrnt = tot;
+      const double dv = (P.trgi - tot) >= 0 ? fabs(P.vsr) : -fabs(P.vsr);
+      s->dv = dv;
+      s->bve = s->bve + dv * P.dt;
+      break;
+    }
+    default: s->red = tot; break;
+  }
+  if (raw) *raw = tot;
+}
+
+// sum over the interior of a(G) * w(G) (w may be null)
+__global__ __launch_bounds__(256) void k_red_gw(Dims d, const double* __restrict__ a, const double* __restrict__ w,
+                                                int op, SbmScal* s, elsim_sbm_params P, double* partials,
+                                                unsigned* counter) {
+  const Col ci = colx(d);
+  double acc[1] = {0.0};
+  each(d, ci, [&](int64_t c, int64_t, int) { acc[0] += w ? a[c] * w[c] : a[c]; });
+  double tot[1];
+  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(op, tot[0], s, P, d, nullptr);
+}
+
+// sum over the interior of (x(G) - x0(N))^2 w(G)
+__global__ __launch_bounds__(256) void k_red_cvg(Dims d, const double* __restrict__ x, const double* __restrict__ x0,
+                                                 const double* __restrict__ w, int op, SbmScal* s, elsim_sbm_params P,
+                                                 double* partials, unsigned* counter) {
+  const Col ci = colx(d);
+  double acc[1] = {0.0};
+  each(d, ci, [&](int64_t c, int64_t n, int) {
+    const double t = x[c] - x0[n];
+    acc[0] += t * t * w[c];
+  });
+  double tot[1];
+  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(op, tot[0], s, P, d, nullptr);
+}
+
+// sum over the interior of rxn(N) * (avp(N) / dx)
+__global__ __launch_bounds__(256) void k_red_rxn(Dims d, const double* __restrict__ rxn, const double* __restrict__ avp,
+                                                 int op, SbmScal* s, elsim_sbm_params P, double* partials,
+                                                 unsigned* counter) {
+  const Col ci = colx(d);
+  double acc[1] = {0.0};
+  each(d, ci, [&](int64_t, int64_t n, int) { acc[0] += rxn[n] * (avp[n] / P.dx); });
+  double tot[1];
+  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(op, tot[0], s, P, d, nullptr);
+}
+
+// psi = clamp(0.5(1+tanh(d/zeta))) + off on the interior (SBM:154-161, 247/258)
+__global__ __launch_bounds__(256) void k_psi(Dims d, const float* __restrict__ sdf, double zeta, double off,
+                                             double* __restrict__ psi) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t c, int64_t n, int) {
+    double g = (double)sdf[n];
+    g = 0.5 * (1.0 + tanh(g / zeta));
+    if (g < 0.0) g = 0.0;
+    if (g > 1.0) g = 1.0;
+    psi[c] = g + off;
+  });
+}
+
+// Ghost cells of a ghosted array.  Every ghost takes the value of its clamped
+// interior neighbour (the one-rank BcMpiYZ outer boundary plus the d2 mirror,
+// SBM:163-176), except the d2 planes whose rule is a value or the CnE flux.
+__global__ void k_bc(Dims d, double* __restrict__ f, BcArgs b) {
+  const int64_t A = 2 * d.g12();                           // d3 planes, all i, j
+  const int64_t B = 2 * (int64_t)(d.n1 + 2) * d.n3;        // d2 planes, all i, k in 1..n3
+  const int64_t C = 2 * (int64_t)d.n2 * d.n3;              // d1 lines, j, k interior
+  const int64_t tot = A + B + C;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    if (q < A) {
+      const int64_t r = q % d.g12();
+      k = q < d.g12() ? 0 : d.n3 + 1;
+      i = (int)(r % d.g1());
+      j = (int)(r / d.g1());
+    } else if (q < A + B) {
+      const int64_t r0 = q - A, half = B / 2, r = r0 % half;
+      j = r0 < half ? 0 : d.n2 + 1;
+      i = (int)(r % (d.n1 + 2));
+      k = (int)(r / (d.n1 + 2)) + 1;
+    } else {
+      const int64_t r0 = q - A - B, half = C / 2, r = r0 % half;
+      i = r0 < half ? 0 : d.n1 + 1;
+      j = (int)(r % d.n2) + 1;
+      k = (int)(r / d.n2) + 1;
+    }
+    const int ic = min(max(i, 1), d.n1), jc = min(max(j, 1), d.n2), kc = min(max(k, 1), d.n3);
+    double v = f[d.gi(ic, jc, kc)];
+    if (j == 0 && b.lo == 1) {
+      v = b.lo_vp ? *b.lo_vp : b.lo_v;
+    } else if (j == 0 && b.lo == 2) {
+      // SBM:558: infx*t_minus*dx/(0.5*(Dab(:,0,:)+Dab(:,1,:))) + CnE(:,1,:)
+      const int64_t pl = (int64_t)(d.n1 + 2) * (d.n3 + 2);
+      const int64_t o = i + (int64_t)(d.n1 + 2) * k;
+      v = *b.infx * b.tm * b.dx / (0.5 * (b.dab01[o] + b.dab01[pl + o])) + v;
+    } else if (j == d.n2 + 1 && b.hi == 1) {
+      v = b.hi_v;
+    }
+    f[d.gi(i, j, k)] = v;
+  }
+}
+
+// AvP (SBM:274-289), interior
+__global__ __launch_bounds__(256) void k_avp(Dims d, const double* __restrict__ psp, const double* __restrict__ pse,
+                                             double amin, double* __restrict__ avp) {
+  const Col ci = colx(d);
+  const int64_t sx = d.g1(), sz = d.g12();
+  each(d, ci, [&](int64_t c, int64_t n, int) {
+    double a = (psp[c + 1] - psp[c - 1]) / 2;
+    double b = (psp[c + sx] - psp[c - sx]) / 2;
+    double e = (psp[c + sz] - psp[c - sz]) / 2;
+    const double vp = sqrt(a * a + b * b + e * e);
+    a = (pse[c + 1] - pse[c - 1]) / 2;
+    b = (pse[c + sx] - pse[c - sx]) / 2;
+    e = (pse[c + sz] - pse[c - sz]) / 2;
+    const double ve = sqrt(a * a + b * b + e * e);
+    double v = sqrt(vp * ve);
+    if (v < amin) v = 0.0;
+    avp[n] = v;
+  });
+}
+
+__global__ void k_scale_g(int64_t n, const double* __restrict__ a, double s, double* __restrict__ b) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    b[q] = s * a[q];
+}
+
+// CH part 1 (SBM:398-419): mu = T_mu(c) - eps Lap c, pm = psP T_M(c), interior.
+__global__ __launch_bounds__(256) void k_ch_mu(Dims d, const double* __restrict__ c, const double* __restrict__ psp,
+                                               const double* __restrict__ tab, int sz, double eps, double dx,
+                                               double* __restrict__ mu, double* __restrict__ pm) {
+  const Col ci = colx(d);
+  const int64_t sx = d.g1(), sz3 = d.g12();
+  each(d, ci, [&](int64_t q, int64_t n, int) {
+    const double cc = c[q];
+    double v = c[q - sx] - 2 * cc + c[q + sx];
+    v = v + c[q - 1] - 2 * cc + c[q + 1];
+    v = v + c[q - sz3] - 2 * cc + c[q + sz3];
+    const double lap = v / (dx * dx);
+    mu[n] = table(cc, tab, tab + sz, sz) - eps * lap;
+    pm[n] = psp[q] * table(cc, tab + 6 * sz, tab + 7 * sz, sz);
+  });
+}
+
+// CH part 2 (SBM:422-443): DvC with mirrored mu/pm, update where psP > min,
+// Xrf partial.
+__global__ __launch_bounds__(256) void k_ch_update(Dims d, double* __restrict__ c, const double* __restrict__ mu,
+                                                   const double* __restrict__ pm, const double* __restrict__ psp,
+                                                   const double* __restrict__ rxn, const double* __restrict__ avp,
+                                                   elsim_sbm_params P, SbmScal* s, double* partials,
+                                                   unsigned* counter) {
+  const Col ci = colx(d);
+  const int64_t nx = d.n1, nz = (int64_t)d.n1 * d.n2;
+  double acc[1] = {0.0};
+  each(d, ci, [&](int64_t q, int64_t n, int k) {
+    const int i = ci.i, j = ci.j;
+    const int64_t yp = i < d.n1 ? 1 : 0, ym = i > 1 ? 1 : 0;
+    const int64_t xp = j < d.n2 ? nx : 0, xm = j > 1 ? nx : 0;
+    const int64_t zp = k < d.n3 ? nz : 0, zm = k > 1 ? nz : 0;
+    const double m0 = mu[n], p0 = pm[n];
+    double v = 0.5 * (pm[n + yp] + p0) * (mu[n + yp] - m0) - 0.5 * (p0 + pm[n - ym]) * (m0 - mu[n - ym]);
+    v = v + 1.0 * 0.5 * (pm[n + xp] + p0) * (mu[n + xp] - m0);
+    v = v - 1.0 * 0.5 * (p0 + pm[n - xm]) * (m0 - mu[n - xm]);
+    v = v + 0.5 * (pm[n + zp] + p0) * (mu[n + zp] - m0);
+    v = v - 0.5 * (p0 + pm[n - zm]) * (m0 - mu[n - zm]);
+    const double dvc = v / (P.dx * P.dx);
+    double cn = c[q];
+    const double ps = psp[q];
+    if (ps > P.ch_psi_min) {
+      cn = cn + P.dt * (dvc + (rxn[n] / P.rho) * (avp[n] / P.dx)) / ps;
+      c[q] = cn;
+    }
+    acc[0] += cn * ps;
+  });
+  double tot[1];
+  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(R_XRF, tot[0], s, P, d, nullptr);
+}
+
+// CnE right-hand side (SBM:461-462) and the influx partial (SBM:465-467).
+__global__ __launch_bounds__(256) void k_cne_rhs(Dims d, const double* __restrict__ cne, const double* __restrict__ pse,
+                                                 const double* __restrict__ rxn, const double* __restrict__ avp,
+                                                 elsim_sbm_params P, double* __restrict__ rhs, SbmScal* s,
+                                                 double* partials, unsigned* counter) {
+  const Col ci = colx(d);
+  double acc[1] = {0.0};
+  each(d, ci, [&](int64_t q, int64_t n, int) {
+    const double ax = avp[n] / P.dx;
+    rhs[n] = pse[q] * cne[q] - P.dt * (rxn[n]) * ax * P.t_minus;
+    acc[0] += rxn[n] * ax;
+  });
+  double tot[1];
+  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(R_INFX, tot[0], s, P, d, nullptr);
+}
+
+// Cell coefficient over the ghosted box: out = psE * [c *] (scale*D0)*exp(c0 - c1 c + c2 (c3 c)^2)
+// (SBM:474+478, 579+583, 601+604).  dab01 (optional): D on planes j = 0, 1.
+__global__ void k_coef(Dims d, const double* __restrict__ cne, const double* __restrict__ pse, double sd0,
+                       elsim_sbm_params P, int times_c, double* __restrict__ out, double* __restrict__ dab01) {
+  const int64_t G = d.G(), g1 = d.g1(), g12 = d.g12();
+  const int64_t pl = (int64_t)(d.n1 + 2) * (d.n3 + 2);
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < G; q += (int64_t)gridDim.x * blockDim.x) {
+    const double c = cne[q];
+    const double e = P.dab_c3 * c;
+    const double dab = sd0 * exp(P.dab_c0 - P.dab_c1 * c + P.dab_c2 * (e * e));
+    out[q] = times_c ? pse[q] * c * dab : pse[q] * dab;
+    if (dab01) {
+      const int64_t j = (q / g1) % (d.n2 + 2);
+      if (j <= 1) {
+        const int64_t i = q % g1, k = q / g12;
+        dab01[j * pl + i + (int64_t)(d.n1 + 2) * k] = dab;
+      }
+    }
+  }
+}
+
+// interior += delta (device scalar expression)
+__global__ __launch_bounds__(256) void k_shift(Dims d, double* __restrict__ f, const SbmScal* s, int which,
+                                               double target, double dt) {
+  const Col ci = colx(d);
+  // which 0: CnE mass correction (SBM:567); 1: phE current control (SBM:831)
+  const double del = which == 0 ? (s->avcne - target) : s->dv * dt;
+  each(d, ci, [&](int64_t q, int64_t, int) { f[q] = which == 0 ? f[q] - del : f[q] + del; });
+}
+
+// Kfw, Kbw (SBM:614-622)
+__global__ __launch_bounds__(256) void k_kfw(Dims d, const double* __restrict__ cnp, const double* __restrict__ tab,
+                                             int sz, elsim_sbm_params P, double* __restrict__ kfw,
+                                             double* __restrict__ kbw) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t q, int64_t n, int) {
+    const double cp = cnp[q];
+    const double ioc = table(cp, tab + 2 * sz, tab + 3 * sz, sz) * P.i0_scale;
+    const double ocv = table(cp, tab + 4 * sz, tab + 5 * sz, sz);
+    kfw[n] = ioc / (P.frd * P.kfw_cref) * exp(0.5 * P.cst1 * ocv);
+    kbw[n] = ioc / (P.frd * cp) * exp(-(0.5 * P.cst1 * ocv));
+  });
+}
+
+__global__ void k_table_n(int64_t n, const double* __restrict__ cord, const double* __restrict__ vals, int sz,
+                          const double* __restrict__ c, double* __restrict__ f) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    f[q] = table(c[q], cord, vals, sz);
+}
+
+// Butler-Volmer (SBM:631-641) and both potential right-hand sides
+// (SBM:645, 727).
+__global__ __launch_bounds__(256) void k_bv(Dims d, const double* __restrict__ php, const double* __restrict__ phe,
+                                            const double* __restrict__ cne, const double* __restrict__ cnp,
+                                            const double* __restrict__ kfw, const double* __restrict__ kbw,
+                                            const double* __restrict__ avp, const double* __restrict__ rhe,
+                                            elsim_sbm_params P, double* __restrict__ rxn, double* __restrict__ rxc,
+                                            double* __restrict__ rhsp, double* __restrict__ rhse) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t q, int64_t n, int) {
+    const double a = avp[n];
+    if (a > P.avp_min) {
+      const double dph = php[q] - phe[q];
+      const double rx = kfw[n] * cne[q] * exp(-(0.5 * P.cst1 * dph)) - kbw[n] * cnp[q] * exp(0.5 * P.cst1 * dph);
+      rxn[n] = rx;
+      rxc[n] = rx * P.frd;
+    }
+    const double ax = a / P.dx;
+    rhsp[n] = -(rxc[n] * ax);
+    rhse[n] = (rxn[n]) * ax + rhe[n];
+  });
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ state --
+
+struct SbmState {
+  Dims d{0, 0, 0};
+  elsim_sbm_params P{};
+  int sz = 0;
+  int iter = 1;
+  DevBuf tab, psp, pse, cnp, cne, php, phe, kapp, pce, pcl;  // ghosted (tab: 8 sz)
+  DevBuf avp, rxn, rxc, rhsp, rhse, rhsc, rhe, kfw, kbw, mu, pm;  // interior
+  DevBuf dab01, scal;
+  CgWork cg;
+  SbmScal* hs = nullptr;  // pinned mirror
+  ~SbmState() {
+    DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
+                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal};
+    for (DevBuf* b : bs) dev_free(*b);
+    cg.release();
+    if (hs) (void)hipHostFree(hs);
+  }
+  double* g(DevBuf& b) { return (double*)b.p; }
+  SbmScal* S() { return (SbmScal*)scal.p; }
+};
+
+namespace {
+std::mutex g_sbm_mu;
+std::map<int64_t, std::unique_ptr<SbmState>> g_sbm;
+int64_t g_next = 1;
+
+SbmState* lookup(const int64_t* h) {
+  if (!h) return nullptr;
+  std::lock_guard<std::mutex> lk(g_sbm_mu);
+  auto it = g_sbm.find(*h);
+  return it == g_sbm.end() ? nullptr : it->second.get();
+}
+
+int bc(SbmState& s, double* f, const BcArgs& b) {
+  const Dims& d = s.d;
+  const int64_t tot = 2 * d.g12() + 2 * (int64_t)(d.n1 + 2) * d.n3 + 2 * (int64_t)d.n2 * d.n3;
+  k_bc<<<flat_blocks(tot), FLAT_T, 0, stream()>>>(d, f, b);
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+BcArgs mirror_bc() { return BcArgs{0, 0, 0.0, 0.0, nullptr, nullptr, nullptr, 0.0, 0.0}; }
+
+int sync_scal(SbmState& s) {
+  ELSIM_HIP(hipMemcpyAsync(s.hs, s.scal.p, sizeof(SbmScal), hipMemcpyDeviceToHost, stream()));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  return ELSIM_OK;
+}
+
+int field_info(SbmState& s, int f, double** p, size_t* n) {
+  const size_t G = (size_t)s.d.G(), N = (size_t)s.d.N();
+  switch (f) {
+    case ELSIM_F_PSP: *p = s.g(s.psp); *n = G; return 0;
+    case ELSIM_F_PSE: *p = s.g(s.pse); *n = G; return 0;
+    case ELSIM_F_AVP: *p = s.g(s.avp); *n = N; return 0;
+    case ELSIM_F_CNP: *p = s.g(s.cnp); *n = G; return 0;
+    case ELSIM_F_CNE: *p = s.g(s.cne); *n = G; return 0;
+    case ELSIM_F_PHP: *p = s.g(s.php); *n = G; return 0;
+    case ELSIM_F_PHE: *p = s.g(s.phe); *n = G; return 0;
+    case ELSIM_F_RXN: *p = s.g(s.rxn); *n = N; return 0;
+    case ELSIM_F_KAPP: *p = s.g(s.kapp); *n = G; return 0;
+  }
+  set_error("unknown field id %d", f);
+  return ELSIM_ERR_ARG;
+}
+
+// One CG solve of the state's current system.
+int solve(SbmState& s, int op, int mode, SolveResult* r) {
+  const elsim_sbm_params& P = s.P;
+  SolveArgs a{};
+  a.d = s.d;
+  a.op = op;
+  a.stop_rule = P.stop_rule;
+  a.rtol = P.rtol;
+  a.delta = P.inner_rtol;
+  if (op == ELSIM_OP_CNE) {
+    a.x = s.g(s.cne);
+    a.kap = s.g(s.pce);
+    a.pse = s.g(s.pse);
+    a.rhs = s.g(s.rhsc);
+    a.scale = P.dt / (P.dx * P.dx);
+    a.crt = P.crt_e;
+    a.maxit = P.max_it_e;
+    a.w8 = s.g(s.pse);
+    a.tw = s.hs->tpe;
+  } else if (op == ELSIM_OP_PHP) {
+    a.x = s.g(s.php);
+    a.kap = s.g(s.kapp);
+    a.pse = s.g(s.pse);
+    a.rhs = s.g(s.rhsp);
+    a.scale = 1.0 / (P.dx * P.dx);
+    a.crt = P.crt_p;
+    a.maxit = P.max_it_p;
+    a.w8 = s.g(s.psp);
+    a.tw = s.hs->tpp;
+  } else {
+    a.x = s.g(s.phe);
+    a.kap = s.g(s.pcl);
+    a.pse = s.g(s.pse);
+    a.rhs = s.g(s.rhse);
+    a.scale = 1.0 / (P.dx * P.dx);
+    a.crt = P.crt_l;
+    a.maxit = P.max_it_l;
+    a.w8 = s.g(s.pse);
+    a.tw = s.hs->tpe;
+  }
+  return cg_solve(s.cg, a, mode, r);
+}
+
+int red_gw(SbmState& s, const double* a, const double* w, int op) {
+  Workspace& ws = workspace();
+  k_red_gw<<<col_grid(s.d), dim3(TX, TY, 1), 0, stream()>>>(s.d, a, w, op, s.S(), s.P, ws.partials, ws.counter);
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+
+int ch_update(SbmState& s) {
+  const Dims& d = s.d;
+  const dim3 grid = col_grid(d), blk(TX, TY, 1);
+  Workspace& ws = workspace();
+  hipStream_t st = stream();
+  k_ch_mu<<<grid, blk, 0, st>>>(d, s.g(s.cnp), s.g(s.psp), s.g(s.tab), s.sz, s.P.eps, s.P.dx, s.g(s.mu), s.g(s.pm));
+  k_ch_update<<<grid, blk, 0, st>>>(d, s.g(s.cnp), s.g(s.mu), s.g(s.pm), s.g(s.psp), s.g(s.rxn), s.g(s.avp), s.P,
+                                    s.S(), ws.partials, ws.counter);
+  ELSIM_CHECK_LAUNCH();
+  return bc(s, s.g(s.cnp), mirror_bc());
+}
+
+int coef(SbmState& s, double sd0, int times_c, double* out, double* dab01) {
+  const int64_t G = s.d.G();
+  k_coef<<<flat_blocks(G), FLAT_T, 0, stream()>>>(s.d, s.g(s.cne), s.g(s.pse), sd0, s.P, times_c, out, dab01);
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+
+BcArgs cne_bc_args(SbmState& s) {
+  BcArgs b = mirror_bc();
+  b.lo = 2;
+  b.dab01 = s.g(s.dab01);
+  b.infx = &s.S()->infx;
+  b.tm = s.P.t_minus;
+  b.dx = s.P.dx;
+  return b;
+}
+
+int step(SbmState& s, int mode, elsim_step_stats* st) {
+  const Dims& d = s.d;
+  const elsim_sbm_params& P = s.P;
+  const dim3 grid = col_grid(d), blk(TX, TY, 1);
+  Workspace& ws = workspace();
+  hipStream_t stv = stream();
+  elsim_step_stats z{};
+  int status = 0;
+  SolveResult r;
+  hipEvent_t e0, e1;
+  ELSIM_HIP(hipEventCreate(&e0));
+  ELSIM_HIP(hipEventCreate(&e1));
+  ELSIM_HIP(hipEventRecord(e0, stv));
+
+  // 1. Cahn-Hilliard (SBM:397-443)
+  ELSIM_TRY(ch_update(s));
+
+  // 2. CnE (SBM:461-572)
+  k_cne_rhs<<<grid, blk, 0, stv>>>(d, s.g(s.cne), s.g(s.pse), s.g(s.rxn), s.g(s.avp), P, s.g(s.rhsc), s.S(),
+                                   ws.partials, ws.counter);
+  ELSIM_CHECK_LAUNCH();
+  ELSIM_TRY(coef(s, P.d0, 0, s.g(s.pce), s.g(s.dab01)));
+  ELSIM_TRY(sync_scal(s));
+  if (!(P.skip_cne_first && s.iter == 1)) {
+    ELSIM_TRY(solve(s, ELSIM_OP_CNE, mode, &r));
+    z.it_cne = r.iters;
+    z.relres_cne = r.relres;
+    z.n_reliable += r.n_reliable;
+    if (r.status) status = r.status;
+  }
+  ELSIM_TRY(bc(s, s.g(s.cne), cne_bc_args(s)));
+  if (P.mass_corr_period > 0 && s.iter % P.mass_corr_period == 1) {
+    ELSIM_TRY(red_gw(s, s.g(s.pse), s.g(s.cne), R_AVCNE));
+    k_shift<<<grid, blk, 0, stv>>>(d, s.g(s.cne), s.S(), 0, P.cne_target, P.dt);
+    ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(bc(s, s.g(s.cne), cne_bc_args(s)));
+  }
+
+  // 3. potential coefficients (SBM:579-622)
+  ELSIM_TRY(coef(s, P.tc1 * P.d0, 0, s.g(s.pce), nullptr));
+  ELSIM_TRY(launch_matvec_f64(ELSIM_OP_PHP, d, s.g(s.cne), s.g(s.pce), nullptr, 1.0 / (P.dx * P.dx), s.g(s.rhe),
+                              nullptr, false));
+  ELSIM_TRY(coef(s, P.tc2 * P.d0, 1, s.g(s.pcl), nullptr));
+  k_kfw<<<grid, blk, 0, stv>>>(d, s.g(s.cnp), s.g(s.tab), s.sz, P, s.g(s.kfw), s.g(s.kbw));
+  ELSIM_CHECK_LAUNCH();
+
+  // 4. outer loop (SBM:624-812)
+  double cvgp = 1.0, cvgl = 1.0;
+  int nlp = 0;
+  while (cvgp > P.outer_tol && cvgl > P.outer_tol) {
+    if (nlp >= P.max_outer) {
+      if (!status) status = ELSIM_NOT_CONVERGED;
+      break;
+    }
+    k_bv<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.kfw), s.g(s.kbw),
+                                s.g(s.avp), s.g(s.rhe), P, s.g(s.rxn), s.g(s.rxc), s.g(s.rhsp), s.g(s.rhse));
+    ELSIM_CHECK_LAUNCH();
+    {
+      // SBM:647: phP(:, nx+1, :) = BvP only
+      BcArgs only_hi = mirror_bc();
+      only_hi.hi = 1;
+      only_hi.hi_v = P.bvp;
+      // the mirror ghosts of phP are already exact (last BC application),
+      // so re-mirroring them changes no value
+      ELSIM_TRY(bc(s, s.g(s.php), only_hi));
+    }
+    ELSIM_TRY(solve(s, ELSIM_OP_PHP, mode, &r));
+    z.it_php += r.iters;
+    z.relres_php = r.relres;
+    z.n_reliable += r.n_reliable;
+    if (r.status) status = r.status;
+    BcArgs pbc = mirror_bc();
+    pbc.hi = 1;
+    pbc.hi_v = P.bvp;
+    ELSIM_TRY(bc(s, s.g(s.php), pbc));
+    k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.psp), R_CVGP, s.S(), P,
+                                     ws.partials, ws.counter);
+    ELSIM_CHECK_LAUNCH();
+
+    ELSIM_TRY(solve(s, ELSIM_OP_PHE, mode, &r));
+    z.it_phe += r.iters;
+    z.relres_phe = r.relres;
+    z.n_reliable += r.n_reliable;
+    if (r.status) status = r.status;
+    BcArgs ebc = mirror_bc();
+    ebc.lo = 1;
+    ebc.lo_vp = &s.S()->bve;
+    ELSIM_TRY(bc(s, s.g(s.phe), ebc));
+    k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.phe), (const double*)s.cg.x0.p, s.g(s.pse), R_CVGL, s.S(), P,
+                                     ws.partials, ws.counter);
+    ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(sync_scal(s));
+    cvgp = s.hs->cvgp;
+    cvgl = s.hs->cvgl;
+    nlp++;
+  }
+  z.n_outer = nlp;
+  z.cvgp = cvgp;
+  z.cvgl = cvgl;
+
+  // 5. current control (SBM:826-835)
+  k_red_rxn<<<grid, blk, 0, stv>>>(d, s.g(s.rxn), s.g(s.avp), R_SCRNT, s.S(), P, ws.partials, ws.counter);
+  k_shift<<<grid, blk, 0, stv>>>(d, s.g(s.phe), s.S(), 1, 0.0, P.dt);
+  ELSIM_CHECK_LAUNCH();
+  BcArgs ebc = mirror_bc();
+  ebc.lo = 1;
+  ebc.lo_vp = &s.S()->bve;
+  ELSIM_TRY(bc(s, s.g(s.phe), ebc));
+  ELSIM_HIP(hipEventRecord(e1, stv));
+  ELSIM_TRY(sync_scal(s));
+  float ms = 0.f;
+  ELSIM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  z.xrf = s.hs->xrf;
+  z.scrnt = s.hs->scrnt;
+  z.status = status;
+  z.seconds = ms * 1e-3;
+  s.iter++;
+  if (st) *st = z;
+  return status;
+}
+
+}  // namespace
+}  // namespace elsim
+
+using namespace elsim;
+
+extern "C" {
+
+int elsim_sbm_create(const int* n1, const int* n2, const int* n3, const elsim_sbm_params* prm, const float* sdf_s,
+                     const float* sdf_l, const int* sz, const double* tables, int64_t* handle) {
+  if (!n1 || !n2 || !n3 || !prm || !sdf_s || !sdf_l || !sz || !tables || !handle) return ELSIM_ERR_ARG;
+  if (!dims_ok(*n1, *n2, *n3) || *sz < 3) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  auto s = std::make_unique<SbmState>();
+  s->d = Dims{*n1, *n2, *n3};
+  s->P = *prm;
+  s->sz = *sz;
+  const Dims& d = s->d;
+  if (blocks_of(col_grid(d)) > kMaxRedBlocks) {
+    set_error("grid too large for the reduction workspace");
+    return ELSIM_ERR_ARG;
+  }
+  const size_t G = (size_t)d.G() * 8, N = (size_t)d.N() * 8;
+  ELSIM_TRY(dev_alloc(s->tab, 8 * (size_t)*sz * 8));
+  for (DevBuf* b : {&s->psp, &s->pse, &s->cnp, &s->cne, &s->php, &s->phe, &s->kapp, &s->pce, &s->pcl})
+    ELSIM_TRY(dev_alloc(*b, G));
+  for (DevBuf* b : {&s->avp, &s->rxn, &s->rxc, &s->rhsp, &s->rhse, &s->rhsc, &s->rhe, &s->kfw, &s->kbw, &s->mu, &s->pm})
+    ELSIM_TRY(dev_alloc(*b, N));
+  ELSIM_TRY(dev_alloc(s->dab01, 2 * (size_t)(d.n1 + 2) * (d.n3 + 2) * 8));
+  ELSIM_TRY(dev_alloc(s->scal, sizeof(SbmScal)));
+  ELSIM_HIP(hipHostMalloc((void**)&s->hs, sizeof(SbmScal), hipHostMallocDefault));
+  hipStream_t st = stream();
+  ELSIM_HIP(hipMemcpyAsync(s->tab.p, tables, 8 * (size_t)*sz * 8, hipMemcpyHostToDevice, st));
+  ELSIM_HIP(hipMemsetAsync(s->rxn.p, 0, N, st));
+  ELSIM_HIP(hipMemsetAsync(s->rxc.p, 0, N, st));
+  ELSIM_HIP(hipMemsetAsync(s->scal.p, 0, sizeof(SbmScal), st));
+  // psi fields (SBM:149-207, 247, 258)
+  float* dsdf = (float*)scratch(0, (size_t)d.N() * 4);
+  if (!dsdf) return ELSIM_ERR_ALLOC;
+  const dim3 grid = col_grid(d), blk(TX, TY, 1);
+  ELSIM_HIP(hipMemcpyAsync(dsdf, sdf_s, (size_t)d.N() * 4, hipMemcpyHostToDevice, st));
+  k_psi<<<grid, blk, 0, st>>>(d, dsdf, prm->zeta, prm->psp_off, s->g(s->psp));
+  ELSIM_TRY(bc(*s, s->g(s->psp), mirror_bc()));
+  ELSIM_HIP(hipMemcpyAsync(dsdf, sdf_l, (size_t)d.N() * 4, hipMemcpyHostToDevice, st));
+  k_psi<<<grid, blk, 0, st>>>(d, dsdf, prm->zeta, prm->pse_off, s->g(s->pse));
+  ELSIM_TRY(bc(*s, s->g(s->pse), mirror_bc()));
+  ELSIM_CHECK_LAUNCH();
+  ELSIM_TRY(red_gw(*s, s->g(s->psp), nullptr, R_TPP));
+  ELSIM_TRY(red_gw(*s, s->g(s->pse), nullptr, R_TPE));
+  k_avp<<<grid, blk, 0, st>>>(d, s->g(s->psp), s->g(s->pse), prm->avp_min, s->g(s->avp));
+  k_scale_g<<<flat_blocks(d.G()), FLAT_T, 0, st>>>(d.G(), s->g(s->psp), prm->kap_scale, s->g(s->kapp));
+  ELSIM_CHECK_LAUNCH();
+  // initial fields (SBM:297-337)
+  std::vector<double> h((size_t)d.G());
+  auto fill = [&](DevBuf& b, double v) -> int {
+    for (auto& x : h) x = v;
+    ELSIM_HIP(hipMemcpyAsync(b.p, h.data(), G, hipMemcpyHostToDevice, st));
+    ELSIM_HIP(hipStreamSynchronize(st));
+    return ELSIM_OK;
+  };
+  ELSIM_TRY(fill(s->cnp, 2.02e-2));
+  ELSIM_TRY(fill(s->cne, prm->cne_target));
+  ELSIM_TRY(fill(s->php, prm->bvp));
+  ELSIM_TRY(fill(s->phe, prm->bve));
+  ELSIM_TRY(red_gw(*s, s->g(s->cnp), s->g(s->psp), R_XRF));
+  ELSIM_TRY(sync_scal(*s));
+  s->hs->bve = prm->bve;
+  ELSIM_HIP(hipMemcpyAsync(s->scal.p, s->hs, sizeof(SbmScal), hipMemcpyHostToDevice, st));
+  ELSIM_TRY(s->cg.alloc(d));
+  ELSIM_HIP(hipStreamSynchronize(st));
+  std::lock_guard<std::mutex> lk(g_sbm_mu);
+  int64_t id = g_next++;
+  g_sbm[id] = std::move(s);
+  *handle = id;
+  return ELSIM_OK;
+}
+
+int elsim_sbm_destroy(const int64_t* handle) {
+  if (!handle) return ELSIM_ERR_ARG;
+  (void)hipStreamSynchronize(stream());
+  std::lock_guard<std::mutex> lk(g_sbm_mu);
+  return g_sbm.erase(*handle) ? ELSIM_OK : ELSIM_ERR_ARG;
+}
+
+int elsim_sbm_get(const int64_t* handle, const int* field, double* out) {
+  SbmState* s = lookup(handle);
+  if (!s || !field || !out) return ELSIM_ERR_ARG;
+  double* p;
+  size_t n;
+  ELSIM_TRY(field_info(*s, *field, &p, &n));
+  ELSIM_HIP(hipMemcpyAsync(out, p, n * 8, hipMemcpyDeviceToHost, stream()));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  return ELSIM_OK;
+}
+
+int elsim_sbm_set(const int64_t* handle, const int* field, const double* in) {
+  SbmState* s = lookup(handle);
+  if (!s || !field || !in) return ELSIM_ERR_ARG;
+  double* p;
+  size_t n;
+  ELSIM_TRY(field_info(*s, *field, &p, &n));
+  ELSIM_HIP(hipMemcpyAsync(p, in, n * 8, hipMemcpyHostToDevice, stream()));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  if (*field == ELSIM_F_CNP) {
+    ELSIM_TRY(red_gw(*s, s->g(s->cnp), s->g(s->psp), R_XRF));
+    ELSIM_TRY(sync_scal(*s));
+  }
+  return ELSIM_OK;
+}
+
+int elsim_sbm_field_ptr(const int64_t* handle, const int* field, double** ptr) {
+  SbmState* s = lookup(handle);
+  if (!s || !field || !ptr) return ELSIM_ERR_ARG;
+  size_t n;
+  return field_info(*s, *field, ptr, &n);
+}
+
+int elsim_sbm_scalars(const int64_t* handle, double* tpp, double* tpe, double* xrf, double* bve, int* iter) {
+  SbmState* s = lookup(handle);
+  if (!s) return ELSIM_ERR_ARG;
+  ELSIM_TRY(sync_scal(*s));
+  if (tpp) *tpp = s->hs->tpp;
+  if (tpe) *tpe = s->hs->tpe;
+  if (xrf) *xrf = s->hs->xrf;
+  if (bve) *bve = s->hs->bve;
+  if (iter) *iter = s->iter;
+  return ELSIM_OK;
+}
+
+int elsim_sbm_set_params(const int64_t* handle, const elsim_sbm_params* prm) {
+  SbmState* s = lookup(handle);
+  if (!s || !prm) return ELSIM_ERR_ARG;
+  s->P = *prm;
+  return ELSIM_OK;
+}
+
+int elsim_sbm_step(const int64_t* handle, const int* mode, elsim_step_stats* stats) {
+  SbmState* s = lookup(handle);
+  if (!s || !mode || (*mode != ELSIM_MODE_FP64 && *mode != ELSIM_MODE_MIXED)) return ELSIM_ERR_ARG;
+  return step(*s, *mode, stats);
+}
+
+int elsim_sbm_ch_update(const int64_t* handle) {
+  SbmState* s = lookup(handle);
+  if (!s) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ch_update(*s));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  return ELSIM_OK;
+}
+
+int elsim_sbm_matvec(const int64_t* handle, const int* op, const double* p, double* out) {
+  SbmState* s = lookup(handle);
+  if (!s || !op || !p || !out) return ELSIM_ERR_ARG;
+  const elsim_sbm_params& P = s->P;
+  if (*op == ELSIM_OP_CNE)
+    return launch_matvec_f64(*op, s->d, p, s->g(s->pce), s->g(s->pse), P.dt / (P.dx * P.dx), out, nullptr, true);
+  const double* kap = *op == ELSIM_OP_PHP ? s->g(s->kapp) : s->g(s->pcl);
+  return launch_matvec_f64(*op, s->d, p, kap, nullptr, 1.0 / (P.dx * P.dx), out, nullptr, true);
+}
+
+int elsim_tbl_chk_pln_3da(const int* py, const int* px, const int* pz, const int* sz, const double* cord,
+                          const double* vals, const double* cn, double* fn) {
+  if (!py || !px || !pz || !sz || !cord || !vals || !cn || !fn || *sz < 2 || !dims_ok(*py, *px, *pz))
+    return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  const int64_t N = (int64_t)*py * *px * *pz;
+  double* t = (double*)scratch(2, 2 * (size_t)*sz * 8);
+  double* c = (double*)scratch(3, (size_t)N * 8);
+  double* f = (double*)scratch(4, (size_t)N * 8);
+  if (!t || !c || !f) return ELSIM_ERR_ALLOC;
+  hipStream_t s = stream();
+  ELSIM_HIP(hipMemcpyAsync(t, cord, (size_t)*sz * 8, hipMemcpyHostToDevice, s));
+  ELSIM_HIP(hipMemcpyAsync(t + *sz, vals, (size_t)*sz * 8, hipMemcpyHostToDevice, s));
+  ELSIM_HIP(hipMemcpyAsync(c, cn, (size_t)N * 8, hipMemcpyHostToDevice, s));
+  k_table_n<<<flat_blocks(N), FLAT_T, 0, s>>>(N, t, t + *sz, *sz, c, f);
+  ELSIM_CHECK_LAUNCH();
+  ELSIM_HIP(hipMemcpyAsync(fn, f, (size_t)N * 8, hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  return ELSIM_OK;
+}
+
+}  // extern "C"

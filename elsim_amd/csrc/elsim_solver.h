@@ -1,0 +1,84 @@
+// elsim_solver.h -- the conjugate-gradient solver interface shared by the SBM
+// step driver and the standalone entry points.
+#pragma once
+
+#include "elsim_common.h"
+
+namespace elsim {
+
+// Device-resident CG scalars; kernels read alpha/beta from here so that a
+// batch of iterations runs without host round trips.
+struct CgDev {
+  double rr;       // FP64 path: sum r^2 (the reference's DMY); mixed: sum r32^2
+  double rz;       // mixed: sum r dinv r
+  double pw;       // p . A p
+  double alpha, beta;
+  double bn2;      // ||b_eff||^2
+  double tol2;     // (rtol ||b_eff||)^2
+  double crt;      // reference stop tolerance
+  double tw;       // reference stop weight total (tPP or tPE)
+  double stop;     // last reference stop norm
+  double rmax2;    // mixed: ||r||^2 at the last reliable update
+  double delta2;   // mixed: reliable-update trigger (relative, squared)
+  double rr_true;  // last FP64 true residual norm^2
+  int iters;
+  int maxit;
+  int done;        // 0 running, 1 converged, 2 iteration cap, 3 not finite
+  int need_reliable;
+  int n_reliable;
+  int stop_rule;
+};
+
+// Device buffers of one grid's solves (allocated once per grid).
+struct CgWork {
+  Dims d{0, 0, 0};
+  DevBuf res, wmx, prj, x0, r0;      // FP64: N, N, G, N, N
+  DevBuf e64;                        // FP64 G (correction, ghosted for the stencil)
+  DevBuf y32, r32, w32, p32;         // FP32: N, N, N, G
+  DevBuf kap32, pse32, dinv32;       // FP32: G, G, N
+  DevBuf cg;                         // CgDev
+  CgDev* host = nullptr;             // pinned mirror
+  int alloc(const Dims& d);
+  void release();
+};
+
+struct SolveArgs {
+  Dims d;
+  int op;               // ELSIM_OP_*
+  double* x;            // ghosted FP64 unknown (ghosts = boundary data as stored)
+  const double* kap;    // ghosted cell-centred coefficient
+  const double* pse;    // ghosted psi_E (ELSIM_OP_CNE diagonal; may be null otherwise)
+  const double* rhs;    // interior right-hand side
+  double scale;         // dt/dx^2 (CNE) or 1/dx^2
+  int stop_rule;        // ELSIM_STOP_*
+  double rtol;
+  double crt;           // reference tolerance (CrtE/CrtP/CrtL)
+  int maxit;
+  const double* w8;     // ghosted weight of the reference stop norm (psE or psP)
+  double tw;            // its interior total
+  double delta;         // mixed: reliable-update trigger
+};
+
+struct SolveResult {
+  int iters = 0;
+  int status = 0;
+  int n_reliable = 0;
+  double relres = 0.0;  // FP64 true relative residual of the correction system
+};
+
+// One CG solve of the correction system A_hom e = b - A x (x's ghosts as
+// stored), x <- x + e.  On return x0 (interior copy of x at entry) is in
+// work.x0.  mode = ELSIM_MODE_FP64 or ELSIM_MODE_MIXED.
+int cg_solve(CgWork& w, const SolveArgs& a, int mode, SolveResult* res);
+
+// Single operator application (parity tests / roofline).  hom: in-kernel
+// homogeneous ghost rule; otherwise ghosts are read from memory.
+int launch_matvec_f64(int op, const Dims& d, const double* p, const double* kap, const double* pse,
+                      double scale, double* out, double* dot_dev, bool hom);
+int launch_matvec_f32(int op, const Dims& d, const float* p, const float* kap, const float* pse,
+                      float scale, float* out, double* dot_dev, bool hom);
+
+// Flat kernels shared with the SBM driver.
+int launch_copy_interior_g2n(const Dims& d, const double* g, double* n);  // n = g(interior)
+
+}  // namespace elsim

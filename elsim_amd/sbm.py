@@ -1,0 +1,144 @@
+"""Host-side mirror of the SBM solver core: synthetic problem set-up and a thin
+object over the device-resident state.
+
+Array convention (as the reference): Fortran order, d1 contiguous, ghosted
+arrays of shape (n1+2, n2+2, n3+2); numpy arrays here are Fortran-ordered
+views of the flat buffers.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (GHOSTED_FIELDS, MODE_FP64, STOP_RELRES, ElsimError, SbmParams, StepStats, check, iref, lib,
+                   ptr)
+
+T_MINUS = 4.0e-6 / (4.0e-6 + 1.25e-6)  # SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90:121
+
+
+def reference_params():
+    """The reference's own constants (SBM:88-136), REAL(4) kinds resolved."""
+    p = SbmParams()
+    lib().elsim_sbm_default_params(C.byref(p))
+    return p
+
+
+def synthetic_params(**kw):
+    """Nondimensional parameters of the synthetic electrochemical step (h = 1
+    voxel): psi = 0.5(1+tanh(d/2)), i0 = 1 (table) x 1e-3 (SBM:617), alpha = 0.5,
+    F/RT = 1, dt = 1e-3, every CG to relative residual 1e-8."""
+    tm = T_MINUS
+    p = SbmParams(
+        dx=1.0, dt=1.0e-3, zeta=2.0, psp_off=1.0e-7, pse_off=1.0e-6, avp_min=1.0e-2, eps=1.0, rho=1.0,
+        ch_psi_min=1.0e-5, frd=1.0, cst1=1.0, t_minus=tm, d0=1.0, tc1=(2 * tm - 1.0) / (2 * tm * (1.0 - tm)),
+        tc2=1.0 / (2 * tm * (1.0 - tm)), dab_c0=0.0, dab_c1=0.2, dab_c2=0.0, dab_c3=0.0, kap_scale=1.0,
+        i0_scale=1.0e-3, kfw_cref=1.0, bvp=0.0, bve=-0.5, vsr=0.1, trgi=0.0, cne_target=1.0, rtol=1.0e-8,
+        crt_e=1.0e-7, crt_p=1.0e-7, crt_l=1.0e-6, outer_tol=1.0e-8, inner_rtol=0.1, mass_corr_period=100,
+        skip_cne_first=1, stop_rule=STOP_RELRES, max_it_e=50000, max_it_p=30000, max_it_l=100000, max_outer=30)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def sphere_pack(n, seed, rmin, rmax, target=0.5, hardcore=0.85, connect=True, dcut=48.0):
+    """Interior FP32 SDF (Fortran order) of a seeded sphere pack."""
+    n1, n2, n3 = (n, n, n) if np.isscalar(n) else n
+    sdf = np.zeros(n1 * n2 * n3, np.float32)
+    ns, fr = C.c_int(), C.c_double()
+    check(lib().elsim_gen_sphere_pack(n1, n2, n3, seed, rmin, rmax, target, hardcore, int(connect), dcut,
+                                      ptr(sdf), C.byref(ns), C.byref(fr)), "gen_sphere_pack")
+    return sdf.reshape((n1, n2, n3), order="F"), ns.value, fr.value
+
+
+def ellipsoid_pack(n, seed, amin=8.0, amax=24.0, cone_deg=15.0, target=0.5, dcut=48.0):
+    n1, n2, n3 = (n, n, n) if np.isscalar(n) else n
+    sdf = np.zeros(n1 * n2 * n3, np.float32)
+    ns, fr = C.c_int(), C.c_double()
+    check(lib().elsim_gen_ellipsoid_pack(n1, n2, n3, seed, amin, amax, cone_deg, target, dcut, ptr(sdf),
+                                         C.byref(ns), C.byref(fr)), "gen_ellipsoid_pack")
+    return sdf.reshape((n1, n2, n3), order="F"), ns.value, fr.value
+
+
+def carve_tunnel(sdf, seed, radius=3.0):
+    n1, n2, n3 = sdf.shape
+    flat = np.asfortranarray(sdf).ravel(order="F").copy()
+    check(lib().elsim_gen_tunnel(n1, n2, n3, seed, radius, ptr(flat)), "gen_tunnel")
+    return flat.reshape((n1, n2, n3), order="F")
+
+
+def tables(sz=101):
+    t = np.zeros(8 * sz)
+    check(lib().elsim_gen_tables(sz, ptr(t)), "gen_tables")
+    return t
+
+
+def initial_cnp(n, seed, mean=0.5, amp=0.01):
+    """CnP over the ghosted box: mean +- amp uniform noise, mirrored ghosts."""
+    n1, n2, n3 = (n, n, n) if np.isscalar(n) else n
+    rng = np.random.default_rng(seed)
+    c = mean + rng.uniform(-amp, amp, (n1, n2, n3))
+    return np.pad(c, 1, mode="edge")
+
+
+class Sbm:
+    """Device-resident SBM state (elsim_sbm_* entry points)."""
+
+    def __init__(self, sdf_s, sdf_l, params, tab=None):
+        self.shape = sdf_s.shape
+        n1, n2, n3 = self.shape
+        self.params = params
+        tab = tables() if tab is None else np.ascontiguousarray(tab, dtype=np.float64)
+        self.sz = tab.size // 8
+        s = np.ascontiguousarray(np.asfortranarray(sdf_s, dtype=np.float32).ravel(order="F"))
+        l = np.ascontiguousarray(np.asfortranarray(sdf_l, dtype=np.float32).ravel(order="F"))
+        self.h = C.c_int64(0)
+        check(lib().elsim_sbm_create(iref(n1), iref(n2), iref(n3), C.byref(params), ptr(s), ptr(l),
+                                     iref(self.sz), ptr(tab), C.byref(self.h)), "elsim_sbm_create")
+
+    def _size(self, field):
+        n1, n2, n3 = self.shape
+        return ((n1 + 2, n2 + 2, n3 + 2) if field in GHOSTED_FIELDS else (n1, n2, n3))
+
+    def get(self, field):
+        shp = self._size(field)
+        out = np.empty(int(np.prod(shp)))
+        check(lib().elsim_sbm_get(C.byref(self.h), iref(field), ptr(out)), "elsim_sbm_get")
+        return out.reshape(shp, order="F")
+
+    def set(self, field, a):
+        shp = self._size(field)
+        a = np.asarray(a, dtype=np.float64)
+        assert a.shape == shp, (a.shape, shp)
+        flat = np.ascontiguousarray(a.ravel(order="F"))
+        check(lib().elsim_sbm_set(C.byref(self.h), iref(field), ptr(flat)), "elsim_sbm_set")
+
+    def field_ptr(self, field):
+        p = C.c_void_p()
+        check(lib().elsim_sbm_field_ptr(C.byref(self.h), iref(field), C.byref(p)), "field_ptr")
+        return p
+
+    def scalars(self):
+        v = [C.c_double() for _ in range(4)]
+        it = C.c_int()
+        check(lib().elsim_sbm_scalars(C.byref(self.h), *[C.byref(x) for x in v], C.byref(it)), "scalars")
+        return dict(tpp=v[0].value, tpe=v[1].value, xrf=v[2].value, bve=v[3].value, iter=it.value)
+
+    def set_params(self, params):
+        self.params = params
+        check(lib().elsim_sbm_set_params(C.byref(self.h), C.byref(params)), "set_params")
+
+    def step(self, mode=MODE_FP64):
+        st = StepStats()
+        rc = lib().elsim_sbm_step(C.byref(self.h), iref(mode), C.byref(st))
+        check(rc, "elsim_sbm_step")
+        return st
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().elsim_sbm_destroy(C.byref(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,275 @@
+/*
+ * elsim.h -- C-ABI of the MI355X-native SBM solver core (libelsim_hip.so).
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and is
+ * callable from Fortran through `bind(C)` interfaces with arguments passed by
+ * reference (see elsim_amd/fortran/elsim_gpu.f90).  Return value: 0 = ok,
+ * > 0 = did not converge (iteration cap hit), < 0 = error (see
+ * elsim_last_error()).
+ *
+ * Array conventions follow the reference Fortran program exactly:
+ *   - column-major, Fortran dim 1 ("y", i) contiguous, dim 2 ("x", j) is the
+ *     through-thickness axis that carries every Dirichlet/flux condition,
+ *     dim 3 ("z", k) slowest;
+ *   - "ghosted" arrays are (0:n1+1, 0:n2+1, 0:n3+1), "interior" arrays are
+ *     (1:n1, 1:n2, 1:n3);
+ *   - ghost values are boundary conditions (Dirichlet values sit at the
+ *     ghost-cell centre, SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90:647).
+ * Host-array entry points (no `_dev_` in the name) copy their inputs to the
+ * GPU, run, and copy results back, so data crosses PCIe once per call.
+ * `_dev_` entry points take device pointers (from elsim_malloc) and leave the
+ * data resident in HBM.
+ *
+ * Reference file abbreviations used in the citations below:
+ *   SBM = src/frtrn/SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90
+ *   LS  = src/frtrn/LevelSet_3D.f90
+ *   RM  = src/frtrn/RM_UCNCT_3D.f90
+ *   AD  = src/frtrn/AccDiff_3D.f90
+ */
+#ifndef ELSIM_H
+#define ELSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+#define ELSIM_OK            0
+#define ELSIM_NOT_CONVERGED 1
+#define ELSIM_ERR_ARG      -1
+#define ELSIM_ERR_HIP      -2
+#define ELSIM_ERR_NAN      -3
+#define ELSIM_ERR_ALLOC    -4
+
+/* Stop rules of the conjugate-gradient solves. */
+#define ELSIM_STOP_REFERENCE 0 /* SBM:545-547 (CnE residual RMS), SBM:699-705, 780-786 (update RMS) */
+#define ELSIM_STOP_RELRES    1 /* ||r||_2 <= rtol * ||b_eff||_2, b_eff = residual of a zero interior */
+
+/* Precision / warm-start modes of an electrochemical step. */
+#define ELSIM_MODE_FP64      0 /* reference CG (unpreconditioned) in FP64 */
+#define ELSIM_MODE_MIXED     1 /* FP32 Jacobi-PCG inner iterations, FP64 reliable updates */
+
+/* Which linear system (operator + homogeneous ghost rule of the search
+ * direction) a CG solve works on. */
+#define ELSIM_OP_CNE 0 /* A_c p = psE p - dt/dx^2 sum_f peN_f (p_nb - p); SBM:515-521; PRJ ghosts SBM:511-513 */
+#define ELSIM_OP_PHP 1 /* L p = dx^-2 sum_f pkN_f (p_nb - p); SBM:670-676; PRJ ghosts SBM:666-668 */
+#define ELSIM_OP_PHE 2 /* L p = dx^-2 sum_f pcN_f (p_nb - p); SBM:751-757; PRJ ghosts SBM:747-749 */
+
+/* Parameters of one SBM run.  Defaults (elsim_sbm_default_params) are the
+ * reference's literal values with their REAL(4)/REAL(8) kinds resolved to the
+ * double the reference computes with.  Layout is mirrored by
+ * elsim_amd/sbm.py (ctypes) and elsim_amd/fortran/elsim_gpu.f90 (bind(C)). */
+typedef struct elsim_sbm_params {
+  double dx;          /* SBM:90 */
+  double dt;          /* SBM:91 (fixed; the adaptive dt of SBM:374-383 is host time-loop control) */
+  double zeta;        /* SBM:94  psi = 0.5(1+tanh(d/zeta)) */
+  double psp_off;     /* SBM:247 */
+  double pse_off;     /* SBM:258 */
+  double avp_min;     /* SBM:287 */
+  double eps;         /* SBM:136 gradient-energy coefficient */
+  double rho;         /* SBM:100 */
+  double ch_psi_min;  /* SBM:430 */
+  double frd;         /* SBM:102 Faraday constant */
+  double cst1;        /* SBM:101 F/RT */
+  double t_minus;     /* SBM:121 */
+  double d0;          /* SBM:124 */
+  double tc1;         /* SBM:125 */
+  double tc2;         /* SBM:126 */
+  double dab_c0;      /* SBM:474  D = D0*exp(c0 - c1*c + c2*(c3*c)**2) */
+  double dab_c1;
+  double dab_c2;
+  double dab_c3;
+  double kap_scale;   /* SBM:324 */
+  double i0_scale;    /* SBM:617 */
+  double kfw_cref;    /* SBM:621 */
+  double bvp;         /* SBM:106 */
+  double bve;         /* SBM:107 (evolves: SBM:830) */
+  double vsr;         /* SBM:830 voltage scan rate (fixed) */
+  double trgi;        /* SBM:341 target current */
+  double cne_target;  /* SBM:567 */
+  double rtol;        /* relative-residual tolerance (ELSIM_STOP_RELRES) */
+  double crt_e;       /* SBM:96 */
+  double crt_p;       /* SBM:97 */
+  double crt_l;       /* SBM:98 */
+  double outer_tol;   /* SBM:627 */
+  double inner_rtol;  /* mixed mode: FP32 residual reduction between FP64 reliable updates */
+  int32_t mass_corr_period; /* SBM:562 */
+  int32_t skip_cne_first;   /* SBM:505 */
+  int32_t stop_rule;        /* ELSIM_STOP_* */
+  int32_t max_it_e;         /* SBM:551 */
+  int32_t max_it_p;         /* SBM:710 */
+  int32_t max_it_l;         /* SBM:791 */
+  int32_t max_outer;        /* cap on SBM:627 (the reference has none) */
+  int32_t pad_;
+} elsim_sbm_params;
+
+/* Per-step statistics returned by the step entry points. */
+typedef struct elsim_step_stats {
+  int32_t it_cne;      /* CG iterations of the CnE solve */
+  int32_t it_php;      /* CG iterations summed over outer iterations */
+  int32_t it_phe;
+  int32_t n_outer;     /* outer (BV/phP/phE) iterations */
+  int32_t n_reliable;  /* FP64 reliable updates (mixed mode) */
+  int32_t status;      /* worst solve status */
+  double  xrf;         /* SBM:443 */
+  double  scrnt;       /* SBM:828 */
+  double  cvgp, cvgl;  /* SBM:723, 804 at exit */
+  double  relres_cne, relres_php, relres_phe; /* true FP64 relative residual at exit of the last solve */
+  double  seconds;     /* device time of the step (hipEvent) */
+} elsim_step_stats;
+
+void elsim_sbm_default_params(elsim_sbm_params* p);
+
+/* ------------------------------------------------------------------------ */
+/* Runtime                                                                  */
+/* ------------------------------------------------------------------------ */
+int         elsim_init(int device);
+const char* elsim_last_error(void);
+int         elsim_device_count(void);
+int         elsim_synchronize(void);
+int         elsim_malloc(void** ptr, size_t bytes);
+int         elsim_free(void* ptr);
+int         elsim_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int         elsim_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int         elsim_memset(void* dst, int value, size_t bytes);
+/* Event timer on the library's stream: start, stop -> milliseconds. */
+int         elsim_timer_start(void);
+int         elsim_timer_stop(float* ms);
+
+/* ------------------------------------------------------------------------ */
+/* Level-set family (LS:164-322, RM:180-337)                                */
+/* ------------------------------------------------------------------------ */
+/* Replaces NoGrdBc3D(py,px,pz,f) (LS:164-190): Neumann ghost copy, in place. */
+int elsim_nogrdbc3d(const int* py, const int* px, const int* pz, double* f);
+/* Replaces AbsGrd3DUW_XYZ(py,px,pz,sg,f,gr) (LS:265-322): Godunov upwind
+ * |grad f|; sg, f ghosted; gr interior. */
+int elsim_absgrd3duw_xyz(const int* py, const int* px, const int* pz,
+                         const double* sg, const double* f, double* gr);
+/* Replaces ReDist3D_XYZ(lit,py,px,pz,sg,f) (LS:194-261; RM:209-276 with
+ * dt = REAL(4) 0.1): lit sweeps of f <- f + sg(1-|grad f|)dt with the
+ * sign-flip guard, over the ghosted box.  dt is 0.1d0 for LS, and
+ * 0.10000000149011612 for RM. */
+int elsim_redist3d_xyz(const int* lit, const int* py, const int* px, const int* pz,
+                       const double* sg, double* f, const double* dt);
+/* Same sweeps in FP32 on device: sg as int8 (+1/0/-1), f ghosted FP32. */
+int elsim_dev_redist3d_f32(int lit, int py, int px, int pz, const int8_t* sg,
+                           float* f, float dt);
+int elsim_dev_redist3d_f64(int lit, int py, int px, int pz, const double* sg,
+                           double* f, double dt);
+
+/* ------------------------------------------------------------------------ */
+/* Connectivity (AD:68-245, RM:79-82,128-130)                               */
+/* ------------------------------------------------------------------------ */
+/* One phase of AccDiff_3D (AD:103-169 solid, AD:174-243 electrolyte):
+ * rdin = signed distance (interior, FP64); phase 0 = solid (Dirichlet at the
+ * high d2 face), 1 = electrolyte (psi -> 1-psi, Dirichlet at the low d2
+ * face).  term = last iteration index (reference 1501).  cn = interior FP64
+ * connectivity field written by AD:166 / AD:240. */
+int elsim_accdiff_phase(const int* py, const int* px, const int* pz, const double* rdin,
+                        const int* phase, const int* term, double* cn);
+int elsim_dev_accdiff_phase(int py, int px, int pz, const double* rdin, int phase,
+                            int term, double* cn);
+/* Connected-component labelling (6-connectivity) of {psi >= thr} with
+ * psi = 0.5(1+tanh(sign*rdin/width)), and of which components touch the
+ * Dirichlet face of `phase` (as in AccDiff_3D).  labels: int32 interior,
+ * -1 outside the phase, otherwise the smallest 0-based interior linear index
+ * in the component.  connected: 1 where the component touches the face.
+ * This is the exact, iteration-free form of the AD mask `Cn >= 1e-4`
+ * (RM:80,128) once the AD front has covered every reachable voxel. */
+int elsim_label_connected(const int* py, const int* px, const int* pz, const double* rdin,
+                          const int* phase, int32_t* labels, int32_t* connected);
+int elsim_dev_label_connected(int py, int px, int pz, const double* rdin, int phase,
+                              int32_t* labels, int32_t* connected);
+/* RM:79-82 / RM:128-130: psi_out = (cn < 1e-4f) ? 0 : psi_in (interior). */
+int elsim_rm_threshold(const int* py, const int* px, const int* pz, const double* cn,
+                       const double* psi_in, double* psi_out);
+
+/* ------------------------------------------------------------------------ */
+/* SBM solver core (SBM:143-838)                                             */
+/* ------------------------------------------------------------------------ */
+/* Replaces Tbl_Chk_Pln_3DA(py,px,pz,sz,Cord,Vals,Cn,fn) (SBM:987-1028). */
+int elsim_tbl_chk_pln_3da(const int* py, const int* px, const int* pz, const int* sz,
+                          const double* cord, const double* vals, const double* cn, double* fn);
+
+/* The SBM state lives on the device; the handle is an opaque integer.
+ * tables: 4 tables x 2 rows x sz, row 0 = concentration grid, row 1 = values,
+ * in the order mu (C_Li_M6), i0 (C_Li_J2), OCV (C_Li_O3), mobility (C_Li_Mb5
+ * after the caller's SBM:242 scaling).
+ * sdf_s, sdf_l: interior FP32 signed distances of the particle and
+ * electrolyte phases (SBM:149-152, 181-184). */
+int elsim_sbm_create(const int* n1, const int* n2, const int* n3, const elsim_sbm_params* prm,
+                     const float* sdf_s, const float* sdf_l, const int* sz,
+                     const double* tables, int64_t* handle);
+int elsim_sbm_destroy(const int64_t* handle);
+/* Field ids for get/set: interior or ghosted FP64 arrays. */
+#define ELSIM_F_PSP   0  /* ghosted */
+#define ELSIM_F_PSE   1  /* ghosted */
+#define ELSIM_F_AVP   2  /* interior */
+#define ELSIM_F_CNP   3  /* ghosted */
+#define ELSIM_F_CNE   4  /* ghosted */
+#define ELSIM_F_PHP   5  /* ghosted */
+#define ELSIM_F_PHE   6  /* ghosted */
+#define ELSIM_F_RXN   7  /* interior */
+#define ELSIM_F_KAPP  8  /* ghosted, cell-centred kappa of the phP operator */
+#define ELSIM_F_COUNT 9
+int elsim_sbm_get(const int64_t* handle, const int* field, double* out);
+int elsim_sbm_set(const int64_t* handle, const int* field, const double* in);
+/* Device pointer of a state field (for device-resident callers). */
+int elsim_sbm_field_ptr(const int64_t* handle, const int* field, double** ptr);
+/* Scalars: tPP, tPE, Xrf, BvE, iteration counter. */
+int elsim_sbm_scalars(const int64_t* handle, double* tpp, double* tpe, double* xrf,
+                      double* bve, int* iter);
+int elsim_sbm_set_params(const int64_t* handle, const elsim_sbm_params* prm);
+/* One electrochemical step SBM:397-840 (CH -> CnE solve -> outer
+ * {BV -> phP -> phE} -> current control).  mode = ELSIM_MODE_*. */
+int elsim_sbm_step(const int64_t* handle, const int* mode, elsim_step_stats* stats);
+/* The Cahn-Hilliard stage alone (SBM:397-443). */
+int elsim_sbm_ch_update(const int64_t* handle);
+/* Homogeneous operator of the state's current coefficients applied to a
+ * device ghosted field (interior output on device). */
+int elsim_sbm_matvec(const int64_t* handle, const int* op, const double* p_dev_ghosted,
+                     double* out_dev_interior);
+
+/* Standalone operator application on device arrays (one launch), used for
+ * parity tests and roofline measurement.  p: ghosted input whose ghost rule
+ * is the homogeneous one of `op` (applied in-kernel, ghost memory not read
+ * except the ghost coefficients).  kap: ghosted cell-centred coefficient.
+ * pse: ghosted psi_E (only for ELSIM_OP_CNE).  out: interior.  scale:
+ * dt/dx^2 (CNE) or 1/dx^2 (potentials).  dot (device, 1 double): p.out. */
+int elsim_dev_matvec_f64(int op, int n1, int n2, int n3, const double* p, const double* kap,
+                         const double* pse, double scale, double* out, double* dot);
+int elsim_dev_matvec_f32(int op, int n1, int n2, int n3, const float* p, const float* kap,
+                         const float* pse, float scale, float* out, double* dot);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic inputs (host only; deterministic for a seed)                   */
+/* ------------------------------------------------------------------------ */
+/* Random spheres, radius ~ U[rmin,rmax], centres ~ U[0,n)^3, with a hard
+ * core: centres at least hardcore*(r1+r2) apart (1 = non-overlapping), added
+ * until the voxel solid fraction reaches `target` or attempts run out.
+ * sdf (interior FP32) = max_s (r_s - |x - c_s|), clamped below at -dcut.
+ * connect != 0 drops particles not 6-connected to the high d2 face and fills
+ * electrolyte pockets not connected to the low d2 face. */
+int elsim_gen_sphere_pack(int n1, int n2, int n3, uint64_t seed, double rmin, double rmax,
+                          double target, double hardcore, int connect, double dcut, float* sdf,
+                          int* nspheres, double* frac);
+/* Random ellipsoids, semi-axes ~ U[amin,amax], major axis within `cone_deg`
+ * of the d2 axis; voxel-level non-overlap; approximate SDF. */
+int elsim_gen_ellipsoid_pack(int n1, int n2, int n3, uint64_t seed, double amin, double amax,
+                             double cone_deg, double target, double dcut, float* sdf,
+                             int* nshapes, double* frac);
+/* Carve a random-walk tunnel of radius `radius` through the solid from the
+ * low to the high d2 face: sdf <- min(sdf, dist(path) - radius). */
+int elsim_gen_tunnel(int n1, int n2, int n3, uint64_t seed, double radius, float* sdf);
+/* Synthetic 101-point property tables (mu, i0, OCV, mobility) on [0,1]. */
+int elsim_gen_tables(int sz, double* tables);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ELSIM_H */

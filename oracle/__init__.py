@@ -1,0 +1,222 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so) and of the
+reference's own level-set subroutines (oracle/_ref/libref_{ls,rm}.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; the product never uses it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LS = os.path.join(HERE, "_ref", "libref_ls.so")
+REF_RM = os.path.join(HERE, "_ref", "libref_rm.so")
+
+_o = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _o
+    if _o is None:
+        if not os.path.exists(LIB):
+            build()
+        _o = C.CDLL(LIB)
+        _o.or_sbm_create.restype = C.c_void_p
+        _o.or_table.restype = C.c_double
+        _o.or_table.argtypes = [C.c_double, C.c_void_p, C.c_void_p, C.c_int]
+        for f in ("or_sbm_step", "or_sbm_get", "or_sbm_set", "or_ch_update", "or_bv", "or_sbm_destroy",
+                  "or_sbm_scalars", "or_sbm_set_params"):
+            getattr(_o, f).argtypes = None
+    return _o
+
+
+def p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def flat(a, dtype=np.float64):
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
+
+
+def unflat(a, shape):
+    return a.reshape(shape, order="F")
+
+
+# ------------------------------------------------------------ level set --
+def nogrdbc3d(f):
+    n1, n2, n3 = (s - 2 for s in f.shape)
+    x = flat(f)
+    lib().or_nogrdbc3d(n1, n2, n3, p(x))
+    return unflat(x, f.shape)
+
+
+def absgrd3duw(sg, f):
+    n1, n2, n3 = (s - 2 for s in f.shape)
+    g = np.zeros(n1 * n2 * n3)
+    lib().or_absgrd3duw(n1, n2, n3, p(flat(sg)), p(flat(f)), p(g))
+    return unflat(g, (n1, n2, n3))
+
+
+def redist3d(lit, sg, f, dt=0.1):
+    n1, n2, n3 = (s - 2 for s in f.shape)
+    x = flat(f)
+    lib().or_redist3d(lit, n1, n2, n3, p(flat(sg)), p(x), C.c_double(dt))
+    return unflat(x, f.shape)
+
+
+def redist3d_f32(lit, sg8, f, dt=np.float32(0.1)):
+    n1, n2, n3 = (s - 2 for s in f.shape)
+    x = flat(f, np.float32)
+    lib().or_redist3d_f32(lit, n1, n2, n3, p(flat(sg8, np.int8)), p(x), C.c_float(dt))
+    return unflat(x, f.shape)
+
+
+def _ref(path):
+    if not os.path.exists(path):
+        return None
+    return C.CDLL(path)
+
+
+def ref_available():
+    return os.path.exists(REF_LS) and os.path.exists(REF_RM)
+
+
+def ref_redist3d(lit, sg, f, which="ls"):
+    """The reference's ReDist3D_XYZ (LevelSet_3D.f90 or RM_UCNCT_3D.f90)."""
+    L = _ref(REF_LS if which == "ls" else REF_RM)
+    n1, n2, n3 = (s - 2 for s in f.shape)
+    x = flat(f)
+    ii = [C.c_int(v) for v in (lit, n1, n2, n3)]
+    L.redist3d_xyz_(*[C.byref(v) for v in ii], p(flat(sg)), p(x))
+    return unflat(x, f.shape)
+
+
+def ref_absgrd3duw(sg, f, which="ls"):
+    L = _ref(REF_LS if which == "ls" else REF_RM)
+    n1, n2, n3 = (s - 2 for s in f.shape)
+    g = np.zeros(n1 * n2 * n3)
+    ii = [C.c_int(v) for v in (n1, n2, n3)]
+    L.absgrd3duw_xyz_(*[C.byref(v) for v in ii], p(flat(sg)), p(flat(f)), p(g))
+    return unflat(g, (n1, n2, n3))
+
+
+def ref_nogrdbc3d(f, which="ls"):
+    L = _ref(REF_LS if which == "ls" else REF_RM)
+    n1, n2, n3 = (s - 2 for s in f.shape)
+    x = flat(f)
+    ii = [C.c_int(v) for v in (n1, n2, n3)]
+    L.nogrdbc3d_(*[C.byref(v) for v in ii], p(x))
+    return unflat(x, f.shape)
+
+
+# --------------------------------------------------------- connectivity --
+def accdiff_phase(rdin, phase, term=1501):
+    n1, n2, n3 = rdin.shape
+    cn = np.zeros(n1 * n2 * n3)
+    lib().or_accdiff_phase(n1, n2, n3, p(flat(rdin)), phase, term, p(cn))
+    return unflat(cn, rdin.shape)
+
+
+def label_connected(rdin, phase):
+    n1, n2, n3 = rdin.shape
+    lab = np.zeros(n1 * n2 * n3, np.int32)
+    con = np.zeros(n1 * n2 * n3, np.int32)
+    lib().or_label_connected(n1, n2, n3, p(flat(rdin)), phase, p(lab), p(con))
+    return unflat(lab, rdin.shape), unflat(con, rdin.shape)
+
+
+# ------------------------------------------------------------------ SBM --
+def table(c, cord, vals):
+    return lib().or_table(float(c), p(cord), p(vals), len(cord))
+
+
+def apply_op(op, kap, pse, x, dt, dx):
+    n1, n2, n3 = (s - 2 for s in x.shape)
+    out = np.zeros(n1 * n2 * n3)
+    pse_f = flat(pse) if pse is not None else flat(np.zeros_like(kap))
+    lib().or_apply_op(op, n1, n2, n3, p(flat(kap)), p(pse_f), p(flat(x)), C.c_double(dt), C.c_double(dx), p(out))
+    return unflat(out, (n1, n2, n3))
+
+
+def prj_bc(op, x):
+    n1, n2, n3 = (s - 2 for s in x.shape)
+    f = flat(x)
+    lib().or_prj_bc(op, n1, n2, n3, p(f))
+    return unflat(f, x.shape)
+
+
+def ghost_mirror(x):
+    n1, n2, n3 = (s - 2 for s in x.shape)
+    f = flat(x)
+    lib().or_ghost_mirror(n1, n2, n3, p(f))
+    return unflat(f, x.shape)
+
+
+def psi_from_sdf(sdf, zeta):
+    n1, n2, n3 = sdf.shape
+    psi = np.zeros((n1 + 2) * (n2 + 2) * (n3 + 2))
+    lib().or_psi_from_sdf(n1, n2, n3, p(flat(sdf, np.float32)), C.c_double(zeta), p(psi))
+    return unflat(psi, (n1 + 2, n2 + 2, n3 + 2))
+
+
+class OracleSbm:
+    """The oracle's SBM state (same construction as elsim_sbm_create)."""
+
+    GHOSTED = {0, 1, 3, 4, 5, 6, 8}
+
+    def __init__(self, sdf_s, sdf_l, params, tab):
+        self.shape = sdf_s.shape
+        n1, n2, n3 = self.shape
+        self.params = params
+        self._s = flat(sdf_s, np.float32)
+        self._l = flat(sdf_l, np.float32)
+        self._t = np.ascontiguousarray(tab, dtype=np.float64)
+        self.h = C.c_void_p(lib().or_sbm_create(n1, n2, n3, C.byref(params), p(self._s), p(self._l),
+                                                 self._t.size // 8, p(self._t)))
+
+    def _shape(self, f):
+        n1, n2, n3 = self.shape
+        return (n1 + 2, n2 + 2, n3 + 2) if f in self.GHOSTED else (n1, n2, n3)
+
+    def get(self, f):
+        shp = self._shape(f)
+        out = np.zeros(int(np.prod(shp)))
+        lib().or_sbm_get(self.h, f, p(out))
+        return unflat(out, shp)
+
+    def set(self, f, a):
+        x = flat(a)
+        lib().or_sbm_set(self.h, f, p(x))
+
+    def step(self):
+        from elsim_amd import StepStats
+        st = StepStats()
+        lib().or_sbm_step(self.h, C.byref(st))
+        return st
+
+    def ch_update(self):
+        lib().or_ch_update(self.h)
+
+    def scalars(self):
+        v = [C.c_double() for _ in range(4)]
+        it = C.c_int()
+        lib().or_sbm_scalars(self.h, *[C.byref(x) for x in v], C.byref(it))
+        return dict(tpp=v[0].value, tpe=v[1].value, xrf=v[2].value, bve=v[3].value, iter=it.value)
+
+    def close(self):
+        if self.h:
+            lib().or_sbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

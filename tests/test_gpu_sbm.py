@@ -1,0 +1,111 @@
+"""GPU parity of the SBM electrochemical step against the FP64 CPU oracle."""
+import numpy as np
+import pytest
+
+import elsim_amd as E
+from elsim_amd._lib import F_AVP, F_CNE, F_CNP, F_PHE, F_PHP, F_PSE, F_PSP, F_RXN
+
+pytestmark = pytest.mark.gpu
+
+
+def case(n, seed=1, **kw):
+    sdf, ns, fr = E.sphere_pack(n, seed, 4.0, 8.0, target=0.5, hardcore=0.85, connect=True)
+    prm = E.synthetic_params(**kw)
+    return sdf, -sdf, prm
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def interior(a):
+    return a[1:-1, 1:-1, 1:-1]
+
+
+def test_setup_matches_oracle(gpu, oracle_lib):
+    sdf, sdl, prm = case(24)
+    tab = E.tables()
+    g = E.Sbm(sdf, sdl, prm, tab)
+    o = oracle_lib.OracleSbm(sdf, sdl, prm, tab)
+    for f in (F_PSP, F_PSE):
+        a, b = g.get(f), o.get(f)
+        # device tanh vs libm tanh: a few ulp
+        assert np.abs(a - b).max() <= 4e-16
+    assert np.abs(g.get(F_AVP) - o.get(F_AVP)).max() <= 1e-13
+    sg, so = g.scalars(), o.scalars()
+    assert sg["tpp"] == pytest.approx(so["tpp"], rel=1e-13)
+    assert sg["tpe"] == pytest.approx(so["tpe"], rel=1e-13)
+
+
+def test_ch_update_bit_exact(gpu, oracle_lib):
+    sdf, sdl, prm = case(20, seed=3)
+    tab = E.tables()
+    g = E.Sbm(sdf, sdl, prm, tab)
+    o = oracle_lib.OracleSbm(sdf, sdl, prm, tab)
+    cnp = E.initial_cnp(20, 9)
+    rng = np.random.default_rng(1)
+    rxn = rng.uniform(-1e-3, 1e-3, (20, 20, 20))
+    for f, v in ((F_PSP, o.get(F_PSP)), (F_AVP, o.get(F_AVP)), (F_CNP, cnp), (F_RXN, rxn)):
+        g.set(f, v)
+        o.set(f, v)
+    E.check(gpu.elsim_sbm_ch_update(__import__("ctypes").byref(g.h)))
+    o.ch_update()
+    assert np.array_equal(g.get(F_CNP), o.get(F_CNP))
+
+
+@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_MIXED])
+def test_step_matches_oracle(gpu, oracle_lib, mode):
+    n = 24
+    sdf, sdl, prm = case(n, seed=2)
+    tab = E.tables()
+    g = E.Sbm(sdf, sdl, prm, tab)
+    o = oracle_lib.OracleSbm(sdf, sdl, prm, tab)
+    cnp = E.initial_cnp(n, 4)
+    g.set(F_CNP, cnp)
+    o.set(F_CNP, cnp)
+    for step in range(3):
+        sg = g.step(mode)
+        so = o.step()
+        assert sg.status == 0 and so.status == 0
+        for r in (sg.relres_php, sg.relres_phe):
+            assert r <= 1e-8
+        if step > 0:
+            assert sg.relres_cne <= 1e-8
+    for f in (F_CNP, F_CNE, F_PHP, F_PHE):
+        e = rel(interior(g.get(f)), interior(o.get(f)))
+        assert e <= 1e-6, (f, e)
+    assert g.scalars()["bve"] == pytest.approx(o.scalars()["bve"], rel=1e-12)
+
+
+def test_reference_stop_rule(gpu, oracle_lib):
+    """FP64 path with the reference's absolute stop rules (SBM:545-547, 699-705, 780-786)."""
+    n = 16
+    sdf, sdl, prm = case(n, seed=5, stop_rule=E.STOP_REFERENCE)
+    tab = E.tables()
+    g = E.Sbm(sdf, sdl, prm, tab)
+    o = oracle_lib.OracleSbm(sdf, sdl, prm, tab)
+    cnp = E.initial_cnp(n, 4)
+    g.set(F_CNP, cnp)
+    o.set(F_CNP, cnp)
+    for _ in range(2):
+        sg, so = g.step(E.MODE_FP64), o.step()
+        # same algorithm; reductions differ only in summation order
+        assert abs(sg.it_php - so.it_php) <= max(2, so.it_php // 50)
+        assert abs(sg.it_phe - so.it_phe) <= max(2, so.it_phe // 50)
+    for f in (F_PHP, F_PHE, F_CNE):
+        assert rel(interior(g.get(f)), interior(o.get(f))) <= 1e-6
+
+
+def test_step_deterministic(gpu):
+    n = 20
+    sdf, sdl, prm = case(n, seed=6)
+    outs = []
+    for _ in range(2):
+        g = E.Sbm(sdf, sdl, prm)
+        g.set(F_CNP, E.initial_cnp(n, 1))
+        for _ in range(2):
+            g.step(E.MODE_MIXED)
+        outs.append([g.get(f) for f in (F_CNP, F_CNE, F_PHP, F_PHE)])
+        g.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
