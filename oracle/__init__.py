@@ -42,7 +42,8 @@ def p(a):
 
 
 def flat(a, dtype=np.float64):
-    return np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
+    """Fortran-order flat copy (never a view: callees write into it)."""
+    return np.array(np.asarray(a, dtype=dtype).ravel(order="F"), dtype=dtype, copy=True)
 
 
 def unflat(a, shape):

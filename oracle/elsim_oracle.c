@@ -993,3 +993,54 @@ int or_sbm_step(void* h, elsim_step_stats* st) {
   s->iter++;
   return status;
 }
+
+/* ======================================================================= */
+/* Standalone pieces for the oracle's own pins                             */
+/* ======================================================================= */
+
+/* One CG solve (same code path as the step) on caller-supplied arrays:
+ * kap, pse, w8, x ghosted; rhs interior.  Returns the status. */
+int or_solve_system(int op, int n1, int n2, int n3, const double* kap, const double* pse, double* x,
+                    const double* rhs, double dt, double dx, int stop_rule, double rtol, double crt, int maxit,
+                    const double* w8, double tw, int* iters, double* relres) {
+  OrSbm s;
+  memset(&s, 0, sizeof(s));
+  s.n1 = n1;
+  s.n2 = n2;
+  s.n3 = n3;
+  s.p.dt = dt;
+  s.p.dx = dx;
+  s.p.stop_rule = stop_rule;
+  s.p.rtol = rtol;
+  s.p.crt_e = s.p.crt_p = s.p.crt_l = crt;
+  s.p.max_it_e = s.p.max_it_p = s.p.max_it_l = maxit;
+  /* or_cg weights the reference stop norm with psp (phP) or pse (CnE, phE);
+   * for phE the weight array is w8 */
+  s.psp = (double*)w8;
+  s.pse = (double*)(op == ELSIM_OP_PHE ? w8 : pse);
+  s.tpp = s.tpe = tw;
+  s.res = nalloc(&s);
+  s.wmx = nalloc(&s);
+  s.tmp = nalloc(&s);
+  s.rhs = nalloc(&s);
+  s.prj = galloc(&s);
+  memcpy(s.rhs, rhs, sizeof(double) * (size_t)n1 * n2 * n3);
+  int st = 0;
+  *iters = or_cg(&s, op, x, kap, &st, relres);
+  free(s.res);
+  free(s.wmx);
+  free(s.tmp);
+  free(s.rhs);
+  free(s.prj);
+  return st;
+}
+
+/* Kfw, Kbw (SBM:614-622) and the Butler-Volmer rate (SBM:636-637) of one
+ * voxel, with the step's code. */
+double or_bv_voxel(const elsim_sbm_params* P, const double* tab, int sz, double cp, double ce, double dph) {
+  double ioc = or_table(cp, tab + 2 * sz, tab + 3 * sz, sz) * P->i0_scale;
+  double ocv = or_table(cp, tab + 4 * sz, tab + 5 * sz, sz);
+  double kfw = ioc / (P->frd * P->kfw_cref) * exp(0.5 * P->cst1 * ocv);
+  double kbw = ioc / (P->frd * cp) * exp(-(0.5 * P->cst1 * ocv));
+  return kfw * ce * exp(-(0.5 * P->cst1 * dph)) - kbw * cp * exp(0.5 * P->cst1 * dph);
+}
