@@ -1,0 +1,182 @@
+"""Benchmark: seconds per SBM electrochemical step at 256^3 on one MI355X.
+
+One step = CH update -> CnE solve -> outer {Butler-Volmer -> phP solve -> phE
+solve} -> current control (SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90:397-840),
+every CG to relative residual 1e-8.  Two paths run on the same synthetic
+microstructure from the same initial state:
+  fp64  - the reference CG (unpreconditioned, FP64);
+  fast  - FP32 Jacobi-PCG with FP64 reliable updates, plus the learned warm
+          start when a trained network is available.
+`value` is the fast path's seconds per step (lower is better).  Inputs are
+resident in HBM before the timed region; W warm-up steps run first, then K
+timed steps bracketed by device synchronisation.
+
+Also reported: the live roofline of the dominant kernel (the FP32 7-point
+operator, timed with HIP events on the library stream) and the FP64 CPU
+oracle timed on a bounded sample on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_case(n, seed):
+    import elsim_amd as E
+    rng = np.random.default_rng(seed)
+    target = float(rng.uniform(0.40, 0.60))
+    sdf, ns, frac = E.sphere_pack(n, seed, 6.0, 16.0, target=target, hardcore=0.85, connect=True)
+    return sdf, ns, frac, target
+
+
+def matvec_roofline(n, reps=50):
+    """Average duration of the FP32 potential operator at n^3 (HIP events on
+    the library stream) and its algorithmic bandwidth: read p and kappa, write
+    Ap = 12 B per voxel."""
+    import elsim_amd as E
+    L = E.lib()
+    G = (n + 2) ** 3
+    rng = np.random.default_rng(0)
+    p = E.DeviceArray.from_host(rng.standard_normal(G).astype(np.float32))
+    k = E.DeviceArray.from_host(rng.uniform(1e-7, 1.0, G).astype(np.float32))
+    out = E.DeviceArray(n ** 3, np.float32)
+    for _ in range(5):
+        E.check(L.elsim_dev_matvec_f32(E.OP_PHP, n, n, n, p.ptr, k.ptr, None, C.c_float(1.0), out.ptr, None))
+    E.check(L.elsim_synchronize())
+    E.check(L.elsim_timer_start())
+    for _ in range(reps):
+        L.elsim_dev_matvec_f32(E.OP_PHP, n, n, n, p.ptr, k.ptr, None, C.c_float(1.0), out.ptr, None)
+    ms = C.c_float()
+    E.check(L.elsim_timer_stop(C.byref(ms)))
+    t = ms.value * 1e-3 / reps
+    bytes_alg = 12.0 * n ** 3
+    return t, bytes_alg / t / 1e9
+
+
+def cpu_baseline(sdf, prm, tab, iters_per_step, budget_iters=12):
+    """FP64 oracle, one thread: time `budget_iters` CG iterations of the phP
+    system at full size and scale by the FP64 path's iterations per step."""
+    import oracle as O
+    from elsim_amd._lib import F_KAPP, F_PHP, F_PSP
+    o = O.OracleSbm(sdf, -sdf, prm, tab)
+    kap, php, psp = o.get(F_KAPP), o.get(F_PHP), o.get(F_PSP)
+    n1, n2, n3 = sdf.shape
+    rhs = np.random.default_rng(1).standard_normal(n1 * n2 * n3) * 1e-3
+    it, rr = C.c_int(), C.c_double()
+    xf = O.flat(php)
+    t0 = time.perf_counter()
+    O.lib().or_solve_system(1, n1, n2, n3, O.p(O.flat(kap)), O.p(O.flat(psp)), O.p(xf), O.p(rhs),
+                            C.c_double(prm.dt), C.c_double(prm.dx), 1, C.c_double(1e-30), C.c_double(0.0),
+                            budget_iters - 1, O.p(O.flat(psp)), C.c_double(1.0), C.byref(it), C.byref(rr))
+    t = time.perf_counter() - t0
+    o.close()
+    per_it = t / max(it.value, 1)
+    return per_it * iters_per_step, t, it.value
+
+
+def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0):
+    from elsim_amd._lib import F_CNP
+    s = E.Sbm(sdf, -sdf, prm, tab)
+    s.set(F_CNP, cnp0)
+    stats = []
+    for w in range(warmup):
+        st = s.step(mode)
+        log(f"  warmup {w}: {st.as_dict()}")
+    E.check(E.lib().elsim_synchronize())
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = s.step(mode)
+        stats.append(st.as_dict())
+        log(f"  step {k}: {st.as_dict()}")
+    E.check(E.lib().elsim_synchronize())
+    t = time.perf_counter() - t0
+    s.close()
+    return t / steps, stats
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path has no multi-GPU mode.",
+              file=sys.stderr)
+        sys.exit(2)
+    import elsim_amd as E
+    L = E.lib()
+    E.check(L.elsim_init(0), "elsim_init")
+    n = a.n
+    t0 = time.perf_counter()
+    sdf, ns, frac, target = make_case(n, a.seed)
+    log(f"case: n={n} spheres={ns} solid={frac:.3f} (target {target:.3f}) gen {time.perf_counter() - t0:.1f}s")
+    prm = E.synthetic_params()
+    tab = E.tables()
+    cnp0 = E.initial_cnp(n, a.seed)
+
+    log("fp64 path")
+    t64, st64 = run_path(E, sdf, prm, tab, E.MODE_FP64, a.warmup, a.steps, cnp0)
+    log("fast path (mixed precision)")
+    tfast, stf = run_path(E, sdf, prm, tab, E.MODE_MIXED, a.warmup, a.steps, cnp0)
+
+    t_mv, gbs = matvec_roofline(n)
+    log(f"matvec f32 {n}^3: {t_mv * 1e6:.1f} us, {gbs:.0f} GB/s")
+
+    iters64 = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in st64]))
+    itersf = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in stf]))
+    cpu = None
+    if not a.no_cpu:
+        v, t, k = cpu_baseline(sdf, prm, tab, iters64)
+        cpu = {"value": v, "unit": "s/step", "cores": 1, "kind": "port",
+               "sample": f"{k} FP64 CG iterations of the phP system at {n}^3 on the oracle ({t:.1f} s, one thread), "
+                         f"scaled by the FP64 path's {iters64:.0f} CG iterations per step"}
+    N = n ** 3
+    out = {
+        "metric": "s/electrochem step @256³: FP64 CG vs FP32-refined+NN warm start, rel res 1e-8",
+        "value": tfast,
+        "unit": "s/step",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": tfast * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"sbm_electrochem_step_{n}^3", "grid": [n, n, n], "spheres": ns,
+                   "solid_fraction": round(frac, 4), "sphere_radius": [6, 16], "dt": prm.dt, "rtol": prm.rtol,
+                   "fast_path": "fp32 jacobi-pcg + fp64 reliable updates", "learned_warm_start": False},
+        "fp64_s_per_step": t64,
+        "speedup_vs_fp64": t64 / tfast,
+        "cg_iters_per_step": {"fp64": iters64, "fast": itersf},
+        "voxel_updates_per_s": {"fp64": N * iters64 / t64, "fast": N * itersf / tfast},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_matvec<float, phP> (7-point operator, FP32)",
+                     "kernel_us": t_mv * 1e6},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

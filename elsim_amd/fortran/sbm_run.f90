@@ -1,0 +1,110 @@
+!! sbm_run -- Fortran host driver of the GPU SBM step (the time loop of
+!! SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90:357-877 with a fixed dt and
+!! step count, on one GPU instead of 192 MPI ranks).
+!!
+!! Usage: sbm_run <namelist file>
+!!   &run n1=, n2=, n3=, nsteps=, mode=, sdf_s='', sdf_l='', cnp_file='',
+!!        tables_file='', outdir='.', outpf='GBQ105' /
+!!   &params prm%dx=..., prm%dt=..., ... /   (defaults: the reference values)
+!! Inputs are Fortran unformatted records as the reference reads them:
+!! FP32 SDFs (SBM:149-152, 181-184), the FP64 ghosted CnP (optional) and the
+!! tables as one FP64 (101,8) record.  Outputs: one FP32 record per field
+!! named as OutPut3D_A does (SBM:892-919) and the TmFx text file of ReC_TmFX
+!! (SBM:922-943).
+PROGRAM sbm_run
+   USE elsim_gpu
+   IMPLICIT NONE
+   INTEGER :: n1, n2, n3, nsteps, mode, sz, k, u, ios
+   CHARACTER(LEN=512) :: sdf_s, sdf_l, cnp_file, tables_file, outdir, nml
+   CHARACTER(LEN=6) :: outpf
+   TYPE(elsim_sbm_params) :: prm
+   TYPE(elsim_step_stats) :: st
+   REAL(KIND=4), ALLOCATABLE :: DmyS(:,:,:), DmyL(:,:,:)
+   REAL(KIND=8), ALLOCATABLE :: Tbls(:,:), Fld(:,:,:)
+   INTEGER(KIND=8) :: h
+   REAL(KIND=8) :: tpp, tpe, xrf, bve, tm
+   INTEGER(c_int) :: it
+   NAMELIST /run/ n1, n2, n3, nsteps, mode, sdf_s, sdf_l, cnp_file, tables_file, outdir, outpf
+   NAMELIST /params/ prm
+
+   CALL elsim_sbm_default_params(prm)
+   n1 = 0; n2 = 0; n3 = 0; nsteps = 1; mode = ELSIM_MODE_FP64
+   sdf_s = ''; sdf_l = ''; cnp_file = ''; tables_file = ''; outdir = '.'; outpf = 'GBQ105'
+   IF (COMMAND_ARGUMENT_COUNT() < 1) THEN
+      WRITE (0, *) 'usage: sbm_run <namelist file>'
+      ERROR STOP 1
+   END IF
+   CALL GET_COMMAND_ARGUMENT(1, nml)
+   OPEN (NEWUNIT=u, FILE=TRIM(nml), STATUS='OLD', ACTION='READ')
+   READ (u, NML=run)
+   READ (u, NML=params, IOSTAT=ios)
+   CLOSE (u)
+
+   ALLOCATE (DmyS(n1,n2,n3), DmyL(n1,n2,n3), Fld(0:n1+1,0:n2+1,0:n3+1))
+   OPEN (NEWUNIT=u, FILE=TRIM(sdf_s), FORM='UNFORMATTED', STATUS='OLD', ACTION='READ')
+   READ (u) DmyS
+   CLOSE (u)
+   OPEN (NEWUNIT=u, FILE=TRIM(sdf_l), FORM='UNFORMATTED', STATUS='OLD', ACTION='READ')
+   READ (u) DmyL
+   CLOSE (u)
+   sz = 101
+   ALLOCATE (Tbls(sz,8))
+   OPEN (NEWUNIT=u, FILE=TRIM(tables_file), FORM='UNFORMATTED', STATUS='OLD', ACTION='READ')
+   READ (u) Tbls
+   CLOSE (u)
+
+   CALL SBM_Init(n1, n2, n3, prm, DmyS, DmyL, sz, Tbls, h)
+   IF (LEN_TRIM(cnp_file) > 0) THEN
+      OPEN (NEWUNIT=u, FILE=TRIM(cnp_file), FORM='UNFORMATTED', STATUS='OLD', ACTION='READ')
+      READ (u) Fld
+      CLOSE (u)
+      CALL SBM_Set(h, ELSIM_F_CNP, Fld)
+   END IF
+
+   tm = 0.0d0
+   DO k = 1, nsteps
+      CALL SBM_Step(h, mode, st)
+      tm = tm + prm%dt
+      WRITE (*, '(A,I6,A,I6,A,I7,A,I7,A,I4,A,ES12.5,A,ES10.3,A,ES10.3)') 'step', k, ' it_cne', st%it_cne, &
+         ' it_php', st%it_php, ' it_phe', st%it_phe, ' outer', st%n_outer, ' xrf', st%xrf, &
+         ' relres_php', st%relres_php, ' s', st%seconds
+   END DO
+   CALL elsim_check(elsim_sbm_scalars(h, tpp, tpe, xrf, bve, it), 'scalars')
+
+   CALL write_field(ELSIM_F_CNP, 'CP')
+   CALL write_field(ELSIM_F_CNE, 'CE')
+   CALL write_field(ELSIM_F_PHP, 'PP')
+   CALL write_field(ELSIM_F_PHE, 'PE')
+   CALL ReC_TmFX(1, outpf, nsteps, tm, xrf, st%scrnt, prm%trgi, bve)
+   CALL SBM_Free(h)
+
+CONTAINS
+
+   !! OutPut3D_A(fr=1, ..., prfn=FF//'1000', AB=2): SNGL record named
+   !! FF//I4(1000+rank)//I4(1000+frame)//'.dat'
+   SUBROUTINE write_field(fid, ff)
+      INTEGER, INTENT(IN) :: fid
+      CHARACTER(LEN=2), INTENT(IN) :: ff
+      CHARACTER(LEN=4) :: rk, fs
+      INTEGER :: v
+      CALL SBM_Get(h, fid, Fld)
+      WRITE (rk, "(I4)") 1000
+      WRITE (fs, "(I4)") 1000 + 1
+      OPEN (NEWUNIT=v, FILE=TRIM(outdir)//'/'//ff//rk//fs//'.dat', FORM='UNFORMATTED', STATUS='REPLACE', &
+         ACTION='WRITE')
+      WRITE (v) SNGL(Fld(1:n1,1:n2,1:n3))
+      CLOSE (v)
+   END SUBROUTINE write_field
+
+   !! ReC_TmFX (SBM:922-943)
+   SUBROUTINE ReC_TmFX(fr, prfn, iter, tmv, frx, crnt, Tg, BvEv)
+      INTEGER, INTENT(IN) :: fr, iter
+      REAL(KIND=8), INTENT(IN) :: tmv, frx, crnt, Tg, BvEv
+      CHARACTER(LEN=6), INTENT(IN) :: prfn
+      INTEGER :: v
+      OPEN (NEWUNIT=v, FILE=TRIM(outdir)//'/'//prfn//'TmFx.txt', STATUS='REPLACE', ACTION='WRITE')
+      WRITE (v, *) fr, iter, tmv, frx, crnt, Tg, BvEv
+      CLOSE (v)
+   END SUBROUTINE ReC_TmFX
+
+END PROGRAM sbm_run

@@ -72,7 +72,7 @@ def test_step_matches_oracle(gpu, oracle_lib, mode):
         if step > 0:
             assert sg.relres_cne <= 1e-8
     for f in (F_CNP, F_CNE, F_PHP, F_PHE):
-        e = rel(interior(g.get(f)), interior(o.get(f)))
+        e = float(rel(interior(g.get(f)), interior(o.get(f))))
         assert e <= 1e-6, (f, e)
     assert g.scalars()["bve"] == pytest.approx(o.scalars()["bve"], rel=1e-12)
 
@@ -92,8 +92,11 @@ def test_reference_stop_rule(gpu, oracle_lib):
         # same algorithm; reductions differ only in summation order
         assert abs(sg.it_php - so.it_php) <= max(2, so.it_php // 50)
         assert abs(sg.it_phe - so.it_phe) <= max(2, so.it_phe // 50)
+    # the reference's absolute update-RMS rule (CrtP = 1e-7 on |phi| ~ 0.02)
+    # leaves an error of order 1e-5 relative, so both runs agree to that
     for f in (F_PHP, F_PHE, F_CNE):
-        assert rel(interior(g.get(f)), interior(o.get(f))) <= 1e-6
+        e = float(rel(interior(g.get(f)), interior(o.get(f))))
+        assert e <= 1e-4, (f, e)
 
 
 def test_step_deterministic(gpu):
