@@ -89,6 +89,27 @@ def cpu_baseline(sdf, prm, tab, iters_per_step, budget_iters=12):
     return per_it * iters_per_step, t, it.value
 
 
+def reference_cg_cost(E, sdf, prm, tab, cnp0, cap=400):
+    """Seconds per iteration of the reference CG (unpreconditioned FP64) on the
+    same state after one step: its solves do not reach 1e-8 within the
+    reference's caps at this size, so only the per-iteration cost is timed."""
+    from elsim_amd._lib import F_CNP
+    s = E.Sbm(sdf, -sdf, prm, tab)
+    s.set(F_CNP, cnp0)
+    s.step(E.MODE_MIXED)
+    s.set_params(prm.copy(max_it_p=cap, max_it_l=cap, max_it_e=cap, max_outer=1))
+    E.check(E.lib().elsim_synchronize())
+    t0 = time.perf_counter()
+    st = s.step(E.MODE_FP64)
+    E.check(E.lib().elsim_synchronize())
+    t = time.perf_counter() - t0
+    s.close()
+    its = st.it_cne + st.it_php + st.it_phe
+    log(f"  reference CG: {its} iterations in {t:.2f} s, relres phP {st.relres_php:.2e} phE {st.relres_phe:.2e}")
+    return {"s_per_iter": t / max(its, 1), "iters": its, "relres_php_after_cap": st.relres_php,
+            "relres_phe_after_cap": st.relres_phe, "cap_per_solve": cap}
+
+
 def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0):
     from elsim_amd._lib import F_CNP
     s = E.Sbm(sdf, -sdf, prm, tab)
@@ -112,11 +133,13 @@ def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--paths", default="fast,fp64,ref", help="comma list of fast, fp64, ref")
+    ap.add_argument("--inner-rtol", type=float, default=None)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path has no multi-GPU mode.",
@@ -133,18 +156,28 @@ def main():
     tab = E.tables()
     cnp0 = E.initial_cnp(n, a.seed)
 
-    log("fp64 path")
-    t64, st64 = run_path(E, sdf, prm, tab, E.MODE_FP64, a.warmup, a.steps, cnp0)
-    log("fast path (mixed precision)")
-    tfast, stf = run_path(E, sdf, prm, tab, E.MODE_MIXED, a.warmup, a.steps, cnp0)
+    if a.inner_rtol is not None:
+        prm.inner_rtol = a.inner_rtol
+    paths = a.paths.split(",")
+    nan = float("nan")
+    tfast, stf, t64, st64, ref_it = nan, [], nan, [], None
+    if "fast" in paths:
+        log("fast path (mixed precision)")
+        tfast, stf = run_path(E, sdf, prm, tab, E.MODE_MIXED, a.warmup, a.steps, cnp0)
+    if "fp64" in paths:
+        log("fp64 path (Jacobi-PCG, all FP64)")
+        t64, st64 = run_path(E, sdf, prm, tab, E.MODE_FP64_PCG, a.warmup, a.steps, cnp0)
+    if "ref" in paths:
+        log("reference CG (unpreconditioned FP64): cost per iteration on capped solves")
+        ref_it = reference_cg_cost(E, sdf, prm, tab, cnp0)
 
     t_mv, gbs = matvec_roofline(n)
     log(f"matvec f32 {n}^3: {t_mv * 1e6:.1f} us, {gbs:.0f} GB/s")
 
-    iters64 = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in st64]))
-    itersf = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in stf]))
+    iters64 = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in st64])) if st64 else nan
+    itersf = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in stf])) if stf else nan
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and st64:
         v, t, k = cpu_baseline(sdf, prm, tab, iters64)
         cpu = {"value": v, "unit": "s/step", "cores": 1, "kind": "port",
                "sample": f"{k} FP64 CG iterations of the phP system at {n}^3 on the oracle ({t:.1f} s, one thread), "
@@ -167,7 +200,9 @@ def main():
                    "solid_fraction": round(frac, 4), "sphere_radius": [6, 16], "dt": prm.dt, "rtol": prm.rtol,
                    "fast_path": "fp32 jacobi-pcg + fp64 reliable updates", "learned_warm_start": False},
         "fp64_s_per_step": t64,
+        "fp64_path": "fp64 jacobi-pcg (same preconditioner, all FP64)",
         "speedup_vs_fp64": t64 / tfast,
+        "reference_cg": ref_it,
         "cg_iters_per_step": {"fp64": iters64, "fast": itersf},
         "voxel_updates_per_s": {"fp64": N * iters64 / t64, "fast": N * itersf / tfast},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libelsim_hip.so")
 OK, NOT_CONVERGED = 0, 1
 ERR_ARG, ERR_HIP, ERR_NAN, ERR_ALLOC = -1, -2, -3, -4
 STOP_REFERENCE, STOP_RELRES = 0, 1
-MODE_FP64, MODE_MIXED = 0, 1
+MODE_FP64, MODE_MIXED, MODE_FP64_PCG = 0, 1, 2
 OP_CNE, OP_PHP, OP_PHE = 0, 1, 2
 F_PSP, F_PSE, F_AVP, F_CNP, F_CNE, F_PHP, F_PHE, F_RXN, F_KAPP = range(9)
 GHOSTED_FIELDS = {F_PSP, F_PSE, F_CNP, F_CNE, F_PHP, F_PHE, F_KAPP}

@@ -44,7 +44,7 @@ int ensure_init();
 
 // Device scratch owned by the library: reduction partials and counters.
 // Kernels of one stream run in order, so one partials buffer is shared.
-constexpr int kMaxRedBlocks = 65536;
+constexpr int kMaxRedBlocks = 262144;
 constexpr int kMaxRedVals = 4;
 struct Workspace {
   double* partials = nullptr;   // kMaxRedVals * kMaxRedBlocks
@@ -72,19 +72,53 @@ inline bool dims_ok(int n1, int n2, int n3) {
          (int64_t)n1 * n2 * n3 < ((int64_t)1 << 31);
 }
 
-// Column-marching launch: a block covers TX x TY columns of (d1,d2) and a
-// chunk of KC planes of d3.
-constexpr int TX = 64, TY = 4, KC = 16;
+// Column-marching launch: a block of TX x TY threads owns TX x TY columns of
+// (d1,d2) and marches KC planes of d3.  256 consecutive d1 voxels per block
+// row give fully coalesced 1 KiB (FP32) / 2 KiB (FP64) row loads.  The grid is
+// one-dimensional; blocks are renumbered so that each of the 8 XCDs (which
+// receive blocks round-robin) owns a contiguous run of tiles, ordered d2
+// fastest, and neighbouring rows' halo loads hit the same L2.
+constexpr int TX = 256, TY = 1, KC = 16;
+constexpr int KCS = 64;  // planes per block of the stencil kernels
+struct TileGrid {
+  int gx, gy, gz;
+};
+__host__ __device__ inline TileGrid tile_grid(const Dims& d, int kc = KC) {
+  return TileGrid{(d.n1 + TX - 1) / TX, (d.n2 + TY - 1) / TY, (d.n3 + kc - 1) / kc};
+}
 inline dim3 col_grid(const Dims& d, int kc = KC) {
-  return dim3((d.n1 + TX - 1) / TX, (d.n2 + TY - 1) / TY, (d.n3 + kc - 1) / kc);
+  const TileGrid t = tile_grid(d, kc);
+  return dim3((unsigned)t.gx * t.gy * t.gz);
 }
 inline int blocks_of(const dim3& g) { return (int)(g.x * g.y * g.z); }
 
-// Flat launch over n items.
-constexpr int FLAT_T = 256;
-inline int flat_blocks(int64_t n, int cap = 8192) {
-  int64_t b = (n + FLAT_T - 1) / FLAT_T;
-  return (int)(b < cap ? (b < 1 ? 1 : b) : cap);
+struct ColIdx {
+  int i, j, k0, k1;
+  bool active;
+};
+__device__ inline ColIdx col_idx(const Dims& d, int kc = KC) {
+  const TileGrid t = tile_grid(d, kc);
+  const int nb = t.gx * t.gy * t.gz, per = nb / 8;
+  int b = blockIdx.x;
+  if (b < per * 8) b = (b % 8) * per + b / 8;
+  const int tj = b % t.gy, ti = (b / t.gy) % t.gx, tk = b / (t.gx * t.gy);
+  ColIdx c;
+  c.i = ti * TX + threadIdx.x + 1;
+  c.j = tj * TY + threadIdx.y + 1;
+  c.k0 = tk * kc + 1;
+  c.k1 = min(c.k0 + kc - 1, d.n3);
+  c.active = c.i <= d.n1 && c.j <= d.n2;
+  return c;
+}
+
+// Flat launch over n items: each block owns one contiguous tile of
+// FLAT_T * FLAT_U items (one trip, no grid-stride loop: on MI355X this keeps
+// the most bytes in flight, ~6.3 TB/s for a copy against ~4.5 TB/s with a
+// capped grid-stride loop).
+constexpr int FLAT_T = 256, FLAT_U = 4;
+inline int flat_blocks(int64_t n) {
+  int64_t b = (n + (int64_t)FLAT_T * FLAT_U - 1) / ((int64_t)FLAT_T * FLAT_U);
+  return (int)(b < 1 ? 1 : b);
 }
 
 // ------------------------------------------------------------ reductions --
@@ -120,41 +154,40 @@ __device__ inline void block_sum(double (&v)[NV]) {
   __syncthreads();
 }
 
-// Store this block's partial sums; the block that arrives last sums all
-// partials in a fixed order (so the result does not depend on scheduling),
-// leaves the totals in `tot` (valid in thread 0) and returns true in every
-// thread of that block.
+// Store this block's partial sums (plain stores: the kernel boundary makes
+// them visible to the finalising kernel; no fences or atomics, which on a
+// multi-XCD device would force L2 write-backs in every block).
 template <int NV>
-__device__ inline bool reduce_last_block(double (&v)[NV], double* partials, unsigned* counter, double (&tot)[NV]) {
-  __shared__ bool last;
-  const int tid = linear_tid(), bid = linear_bid(), nb = nblocks();
+__device__ inline void block_partials(double (&v)[NV], double* partials) {
   block_sum<NV>(v);
-  if (tid == 0) {
+  if (linear_tid() == 0)
 #pragma unroll
-    for (int q = 0; q < NV; q++) partials[q * kMaxRedBlocks + bid] = v[q];
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == (unsigned)nb - 1u);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+    for (int q = 0; q < NV; q++) partials[q * kMaxRedBlocks + linear_bid()] = v[q];
+}
+
+// One-block finalisation: the sum of nb partials in a fixed order (the result
+// does not depend on scheduling); valid in thread 0.
+template <int NV>
+__device__ inline void sum_partials(const double* __restrict__ partials, int nb, double (&tot)[NV]) {
+  const int tid = linear_tid(), nt = nthreads();
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; q++) {
-    double s = 0.0;
-    for (int b = tid; b < nb; b += nthreads())
-      s += __hip_atomic_load(&partials[q * kMaxRedBlocks + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    acc[q] = s;
+    const double* p = partials + (size_t)q * kMaxRedBlocks;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int b = tid; b < nb; b += 4 * nt) {
+      s0 += p[b];
+      if (b + nt < nb) s1 += p[b + nt];
+      if (b + 2 * nt < nb) s2 += p[b + 2 * nt];
+      if (b + 3 * nt < nb) s3 += p[b + 3 * nt];
+    }
+    acc[q] = (s0 + s1) + (s2 + s3);
   }
   block_sum<NV>(acc);
-  if (tid == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; q++) tot[q] = acc[q];
-    atomicExch(counter, 0u);
-  }
-  return true;
+  for (int q = 0; q < NV; q++) tot[q] = acc[q];
 }
+constexpr int FIN_T = 1024;  // threads of a finalising kernel
 
 // -------------------------------------------------------- device buffers --
 struct DevBuf {

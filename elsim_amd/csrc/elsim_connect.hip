@@ -197,7 +197,7 @@ int label(const Dims& d, const double* rdin, int phase, int32_t* L, int32_t* con
   const int64_t N = d.N();
   k_ad_psi<<<flat_blocks(d.G()), FLAT_T, 0, s>>>(d, rdin, psi);
   ELSIM_HIP(hipMemsetAsync(flag, 0, (size_t)N * 4, s));
-  const int nb = flat_blocks(N, 65536);
+  const int nb = flat_blocks(N);
   if (phase == 0) {
     k_lab_init<0><<<nb, FLAT_T, 0, s>>>(d, psi, L);
     k_lab_merge<0><<<nb, FLAT_T, 0, s>>>(d, psi, L);

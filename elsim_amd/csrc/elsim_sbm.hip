@@ -55,19 +55,8 @@ __device__ __forceinline__ double table(double c, const double* __restrict__ cor
   return vals[lb - 1] + (c - cord[lb - 1]) / ditv * (vals[lb] - vals[lb - 1]);
 }
 
-struct Col {
-  int i, j, k0, k1;
-  bool active;
-};
-__device__ __forceinline__ Col colx(const Dims& d) {
-  Col c;
-  c.i = blockIdx.x * TX + threadIdx.x + 1;
-  c.j = blockIdx.y * TY + threadIdx.y + 1;
-  c.k0 = blockIdx.z * KC + 1;
-  c.k1 = min(c.k0 + KC - 1, d.n3);
-  c.active = c.i <= d.n1 && c.j <= d.n2;
-  return c;
-}
+using Col = ColIdx;
+__device__ __forceinline__ Col colx(const Dims& d) { return col_idx(d); }
 template <class F>
 __device__ __forceinline__ void each(const Dims& d, const Col& ci, F&& f) {
   if (!ci.active) return;
@@ -102,40 +91,43 @@ __device__ __forceinline__ void finish_red(int op, double tot, SbmScal* s, const
   if (raw) *raw = tot;
 }
 
+// one-block finalisation of an SBM reduction
+__global__ __launch_bounds__(FIN_T) void k_fin_sbm(int op, int nb, const double* __restrict__ partials, SbmScal* s,
+                                                   elsim_sbm_params P, Dims d) {
+  double tot[1];
+  sum_partials<1>(partials, nb, tot);
+  if (linear_tid() == 0) finish_red(op, tot[0], s, P, d, nullptr);
+}
+
 // sum over the interior of a(G) * w(G) (w may be null)
 __global__ __launch_bounds__(256) void k_red_gw(Dims d, const double* __restrict__ a, const double* __restrict__ w,
-                                                int op, SbmScal* s, elsim_sbm_params P, double* partials,
-                                                unsigned* counter) {
+                                                int op, SbmScal* s, elsim_sbm_params P, double* partials) {
   const Col ci = colx(d);
   double acc[1] = {0.0};
   each(d, ci, [&](int64_t c, int64_t, int) { acc[0] += w ? a[c] * w[c] : a[c]; });
-  double tot[1];
-  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(op, tot[0], s, P, d, nullptr);
+  block_partials<1>(acc, partials);
 }
 
-// sum over the interior of (x(G) - x0(N))^2 w(G)
+// sum over the interior of (x - x0)^2 w (all ghosted)
 __global__ __launch_bounds__(256) void k_red_cvg(Dims d, const double* __restrict__ x, const double* __restrict__ x0,
                                                  const double* __restrict__ w, int op, SbmScal* s, elsim_sbm_params P,
-                                                 double* partials, unsigned* counter) {
+                                                 double* partials) {
   const Col ci = colx(d);
   double acc[1] = {0.0};
-  each(d, ci, [&](int64_t c, int64_t n, int) {
-    const double t = x[c] - x0[n];
+  each(d, ci, [&](int64_t c, int64_t, int) {
+    const double t = x[c] - x0[c];
     acc[0] += t * t * w[c];
   });
-  double tot[1];
-  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(op, tot[0], s, P, d, nullptr);
+  block_partials<1>(acc, partials);
 }
 
 // sum over the interior of rxn(N) * (avp(N) / dx)
 __global__ __launch_bounds__(256) void k_red_rxn(Dims d, const double* __restrict__ rxn, const double* __restrict__ avp,
-                                                 int op, SbmScal* s, elsim_sbm_params P, double* partials,
-                                                 unsigned* counter) {
+                                                 int op, SbmScal* s, elsim_sbm_params P, double* partials) {
   const Col ci = colx(d);
   double acc[1] = {0.0};
   each(d, ci, [&](int64_t, int64_t n, int) { acc[0] += rxn[n] * (avp[n] / P.dx); });
-  double tot[1];
-  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(op, tot[0], s, P, d, nullptr);
+  block_partials<1>(acc, partials);
 }
 
 // psi = clamp(0.5(1+tanh(d/zeta))) + off on the interior (SBM:154-161, 247/258)
@@ -240,8 +232,7 @@ __global__ __launch_bounds__(256) void k_ch_mu(Dims d, const double* __restrict_
 __global__ __launch_bounds__(256) void k_ch_update(Dims d, double* __restrict__ c, const double* __restrict__ mu,
                                                    const double* __restrict__ pm, const double* __restrict__ psp,
                                                    const double* __restrict__ rxn, const double* __restrict__ avp,
-                                                   elsim_sbm_params P, SbmScal* s, double* partials,
-                                                   unsigned* counter) {
+                                                   elsim_sbm_params P, SbmScal* s, double* partials) {
   const Col ci = colx(d);
   const int64_t nx = d.n1, nz = (int64_t)d.n1 * d.n2;
   double acc[1] = {0.0};
@@ -265,15 +256,14 @@ __global__ __launch_bounds__(256) void k_ch_update(Dims d, double* __restrict__ 
     }
     acc[0] += cn * ps;
   });
-  double tot[1];
-  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(R_XRF, tot[0], s, P, d, nullptr);
+  block_partials<1>(acc, partials);
 }
 
 // CnE right-hand side (SBM:461-462) and the influx partial (SBM:465-467).
 __global__ __launch_bounds__(256) void k_cne_rhs(Dims d, const double* __restrict__ cne, const double* __restrict__ pse,
                                                  const double* __restrict__ rxn, const double* __restrict__ avp,
                                                  elsim_sbm_params P, double* __restrict__ rhs, SbmScal* s,
-                                                 double* partials, unsigned* counter) {
+                                                 double* partials) {
   const Col ci = colx(d);
   double acc[1] = {0.0};
   each(d, ci, [&](int64_t q, int64_t n, int) {
@@ -281,8 +271,7 @@ __global__ __launch_bounds__(256) void k_cne_rhs(Dims d, const double* __restric
     rhs[n] = pse[q] * cne[q] - P.dt * (rxn[n]) * ax * P.t_minus;
     acc[0] += rxn[n] * ax;
   });
-  double tot[1];
-  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) finish_red(R_INFX, tot[0], s, P, d, nullptr);
+  block_partials<1>(acc, partials);
 }
 
 // Cell coefficient over the ghosted box: out = psE * [c *] (scale*D0)*exp(c0 - c1 c + c2 (c3 c)^2)
@@ -470,11 +459,17 @@ int solve(SbmState& s, int op, int mode, SolveResult* r) {
   return cg_solve(s.cg, a, mode, r);
 }
 
-int red_gw(SbmState& s, const double* a, const double* w, int op) {
-  Workspace& ws = workspace();
-  k_red_gw<<<col_grid(s.d), dim3(TX, TY, 1), 0, stream()>>>(s.d, a, w, op, s.S(), s.P, ws.partials, ws.counter);
+int fin_sbm(SbmState& s, int op) {
+  k_fin_sbm<<<1, FIN_T, 0, stream()>>>(op, blocks_of(col_grid(s.d)), workspace().partials, s.S(), s.P, s.d);
   ELSIM_CHECK_LAUNCH();
   return ELSIM_OK;
+}
+
+int red_gw(SbmState& s, const double* a, const double* w, int op) {
+  Workspace& ws = workspace();
+  k_red_gw<<<col_grid(s.d), dim3(TX, TY, 1), 0, stream()>>>(s.d, a, w, op, s.S(), s.P, ws.partials);
+  ELSIM_CHECK_LAUNCH();
+  return fin_sbm(s, op);
 }
 
 int ch_update(SbmState& s) {
@@ -484,8 +479,9 @@ int ch_update(SbmState& s) {
   hipStream_t st = stream();
   k_ch_mu<<<grid, blk, 0, st>>>(d, s.g(s.cnp), s.g(s.psp), s.g(s.tab), s.sz, s.P.eps, s.P.dx, s.g(s.mu), s.g(s.pm));
   k_ch_update<<<grid, blk, 0, st>>>(d, s.g(s.cnp), s.g(s.mu), s.g(s.pm), s.g(s.psp), s.g(s.rxn), s.g(s.avp), s.P,
-                                    s.S(), ws.partials, ws.counter);
+                                    s.S(), ws.partials);
   ELSIM_CHECK_LAUNCH();
+  ELSIM_TRY(fin_sbm(s, R_XRF));
   return bc(s, s.g(s.cnp), mirror_bc());
 }
 
@@ -525,8 +521,9 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
 
   // 2. CnE (SBM:461-572)
   k_cne_rhs<<<grid, blk, 0, stv>>>(d, s.g(s.cne), s.g(s.pse), s.g(s.rxn), s.g(s.avp), P, s.g(s.rhsc), s.S(),
-                                   ws.partials, ws.counter);
+                                   ws.partials);
   ELSIM_CHECK_LAUNCH();
+  ELSIM_TRY(fin_sbm(s, R_INFX));
   ELSIM_TRY(coef(s, P.d0, 0, s.g(s.pce), s.g(s.dab01)));
   ELSIM_TRY(sync_scal(s));
   if (!(P.skip_cne_first && s.iter == 1)) {
@@ -582,8 +579,9 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
     pbc.hi_v = P.bvp;
     ELSIM_TRY(bc(s, s.g(s.php), pbc));
     k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.psp), R_CVGP, s.S(), P,
-                                     ws.partials, ws.counter);
+                                     ws.partials);
     ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(fin_sbm(s, R_CVGP));
 
     ELSIM_TRY(solve(s, ELSIM_OP_PHE, mode, &r));
     z.it_phe += r.iters;
@@ -595,8 +593,9 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
     ebc.lo_vp = &s.S()->bve;
     ELSIM_TRY(bc(s, s.g(s.phe), ebc));
     k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.phe), (const double*)s.cg.x0.p, s.g(s.pse), R_CVGL, s.S(), P,
-                                     ws.partials, ws.counter);
+                                     ws.partials);
     ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(fin_sbm(s, R_CVGL));
     ELSIM_TRY(sync_scal(s));
     cvgp = s.hs->cvgp;
     cvgl = s.hs->cvgl;
@@ -607,7 +606,8 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   z.cvgl = cvgl;
 
   // 5. current control (SBM:826-835)
-  k_red_rxn<<<grid, blk, 0, stv>>>(d, s.g(s.rxn), s.g(s.avp), R_SCRNT, s.S(), P, ws.partials, ws.counter);
+  k_red_rxn<<<grid, blk, 0, stv>>>(d, s.g(s.rxn), s.g(s.avp), R_SCRNT, s.S(), P, ws.partials);
+  ELSIM_TRY(fin_sbm(s, R_SCRNT));
   k_shift<<<grid, blk, 0, stv>>>(d, s.g(s.phe), s.S(), 1, 0.0, P.dt);
   ELSIM_CHECK_LAUNCH();
   BcArgs ebc = mirror_bc();
@@ -766,7 +766,8 @@ int elsim_sbm_set_params(const int64_t* handle, const elsim_sbm_params* prm) {
 
 int elsim_sbm_step(const int64_t* handle, const int* mode, elsim_step_stats* stats) {
   SbmState* s = lookup(handle);
-  if (!s || !mode || (*mode != ELSIM_MODE_FP64 && *mode != ELSIM_MODE_MIXED)) return ELSIM_ERR_ARG;
+  if (!s || !mode || (*mode != ELSIM_MODE_FP64 && *mode != ELSIM_MODE_MIXED && *mode != ELSIM_MODE_FP64_PCG))
+    return ELSIM_ERR_ARG;
   return step(*s, *mode, stats);
 }
 

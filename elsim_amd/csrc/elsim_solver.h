@@ -36,6 +36,7 @@ struct CgWork {
   DevBuf e64;                        // FP64 G (correction, ghosted for the stencil)
   DevBuf y32, r32, w32, p32;         // FP32: N, N, N, G
   DevBuf kap32, pse32, dinv32;       // FP32: G, G, N
+  DevBuf dinv64;                     // FP64: N (ELSIM_MODE_FP64_PCG)
   DevBuf cg;                         // CgDev
   CgDev* host = nullptr;             // pinned mirror
   int alloc(const Dims& d);

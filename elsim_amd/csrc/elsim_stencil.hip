@@ -11,18 +11,22 @@
 // stores six face arrays, SBM:326-331, 479-484, 605-610; the average is the
 // same IEEE operation, so results are bit-identical).
 //
-// Layout: ghosted (n1+2)(n2+2)(n3+2) arrays, d1 contiguous.  A block of
-// 64 x 4 threads owns 64 x 4 (d1,d2) columns and marches KC planes along d3,
-// keeping p and k of planes k-1, k, k+1 in registers; the d1/d2 neighbours
-// come from L1/L2 (each 64-wide row load is one coalesced 512 B / 256 B
-// transaction).  The search direction's homogeneous ghost rule
-// (SBM:511-513, 666-668, 747-749) is applied in registers, so ghost memory
-// of p is never relied on.
+// Layout: ghosted (n1+2)(n2+2)(n3+2) arrays, d1 contiguous.  Stencil kernels
+// run on 256-wide d1 rows, each thread marching KCS planes along d3 with the
+// planes k-1, k, k+1 of p and kappa held in registers; d1/d2 neighbours come
+// from L1/L2.  The search direction's homogeneous ghost rule (SBM:511-513,
+// 666-668, 747-749) is applied in registers, so ghost memory of p is never
+// relied on.
 //
-// Reductions are deterministic: per-block partial sums, then the last block
-// to finish adds the partials in index order.  All CG scalars (alpha, beta,
-// stop tests) are computed on the device, so the host only polls a done flag
-// every few iterations.
+// Every CG work vector lives in the same ghosted layout with its ghost cells
+// held at zero, so the pointwise kernels are flat over the whole box, one
+// contiguous tile per block: fully coalesced, and adding alpha*p (zero
+// ghosts) to the unknown leaves its boundary ghosts untouched.
+//
+// Reductions are deterministic: every block stores its partial sums, then a
+// one-block finalising kernel adds them in index order and updates the CG
+// scalars (alpha, beta, stop tests) on the device, so the host only polls a
+// done flag every few iterations.
 #include <cmath>
 
 #include "elsim_solver.h"
@@ -43,20 +47,6 @@ template <>
 struct Rule<ELSIM_OP_PHE> {
   static constexpr bool lo_zero = true, hi_zero = false, diag = false;
 };
-
-struct ColIdx {
-  int i, j, k0, k1;
-  bool active;
-};
-__device__ __forceinline__ ColIdx col_idx(const Dims& d) {
-  ColIdx c;
-  c.i = blockIdx.x * TX + threadIdx.x + 1;
-  c.j = blockIdx.y * TY + threadIdx.y + 1;
-  c.k0 = blockIdx.z * KC + 1;
-  c.k1 = min(c.k0 + KC - 1, d.n3);
-  c.active = c.i <= d.n1 && c.j <= d.n2;
-  return c;
-}
 
 // A p along one column; epi(c_ghosted, n_interior, p_centre, value).
 template <typename T, int OP, bool HOM, class Epi>
@@ -104,42 +94,49 @@ __device__ __forceinline__ void column(const Dims& d, const ColIdx& ci, const T*
   }
 }
 
-// Pointwise walk over a block's columns.
+// Pointwise walk over a block's columns (interior only).
 template <class F>
 __device__ __forceinline__ void for_col(const Dims& d, const ColIdx& ci, F&& f) {
   if (!ci.active) return;
   const int64_t sz = d.g12(), nz = (int64_t)d.n1 * d.n2;
   int64_t c = d.gi(ci.i, ci.j, ci.k0), n = d.ni(ci.i, ci.j, ci.k0);
-  for (int k = ci.k0; k <= ci.k1; k++, c += sz, n += nz) f(c, n);
+  for (int k = ci.k0; k <= ci.k1; k++, c += sz, n += nz) f(c, n, k);
 }
+
+// Flat loop over [0, n): block b owns the contiguous tile
+// [b*FLAT_T*FLAT_U, (b+1)*FLAT_T*FLAT_U); a load phase then a compute/store
+// phase, so the FLAT_U loads of a thread are in flight together.
+#define FLAT_LOOP(n, LOADS, BODY)                                                        \
+  {                                                                                      \
+    const int64_t base_ = (int64_t)blockIdx.x * (FLAT_T * FLAT_U) + threadIdx.x;        \
+    _Pragma("unroll") for (int u = 0; u < FLAT_U; u++) {                                 \
+      const int64_t q = base_ + u * FLAT_T;                                              \
+      if (q < (n)) { LOADS }                                                             \
+    }                                                                                    \
+    _Pragma("unroll") for (int u = 0; u < FLAT_U; u++) {                                 \
+      const int64_t q = base_ + u * FLAT_T;                                              \
+      if (q < (n)) { BODY }                                                              \
+    }                                                                                    \
+  }
 
 // ------------------------------------------------------------- kernels --
 
-// out = A p (hom or stored ghosts); optional dot = p.out into *dot or cg->pw
-// with alpha = (num_is_rz ? rz : rr) / pw.
-template <typename T, int OP, bool HOM>
+// out = A p (hom or stored ghosts); out in the ghosted layout (OUTG) or
+// interior layout; optional dot = p.out into *dot or cg->pw with
+// alpha = (num_is_rz ? rz : rr) / pw.
+template <typename T, int OP, bool HOM, bool OUTG>
 __global__ __launch_bounds__(256) void k_matvec(Dims d, const T* __restrict__ p, const T* __restrict__ kap,
                                                 const T* __restrict__ pse, T scale, T* __restrict__ out,
-                                                double* dot, CgDev* cg, int num_is_rz, double* partials,
-                                                unsigned* counter) {
+                                                double* dot, CgDev* cg, int num_is_rz, double* partials) {
   if (cg && (cg->done || cg->need_reliable)) return;
-  const ColIdx ci = col_idx(d);
+  const ColIdx ci = col_idx(d, KCS);
   double acc[1] = {0.0};
   if (ci.active)
-    column<T, OP, HOM>(d, ci, p, kap, pse, scale, [&](int64_t, int64_t n, T pc, T v) {
-      out[n] = v;
+    column<T, OP, HOM>(d, ci, p, kap, pse, scale, [&](int64_t c, int64_t n, T pc, T v) {
+      out[OUTG ? c : n] = v;
       acc[0] += (double)pc * (double)v;
     });
-  if (dot || cg) {
-    double tot[1];
-    if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) {
-      if (dot) *dot = tot[0];
-      if (cg) {
-        cg->pw = tot[0];
-        cg->alpha = (num_is_rz ? cg->rz : cg->rr) / tot[0];
-      }
-    }
-  }
+  if (dot || cg) block_partials<1>(acc, partials);
 }
 
 // Initial residual with x's ghosts as stored (SBM:492-499, 649-655, 729-735):
@@ -149,24 +146,19 @@ __global__ __launch_bounds__(256) void k_resid64(Dims d, const double* __restric
                                                  const double* __restrict__ pse, double scale,
                                                  const double* __restrict__ rhs, double* __restrict__ res,
                                                  double* __restrict__ r0, double* __restrict__ prj,
-                                                 double* __restrict__ x0, CgDev* cg, double* partials,
-                                                 unsigned* counter) {
-  const ColIdx ci = col_idx(d);
+                                                 double* __restrict__ x0, CgDev* cg, double* partials) {
+  const ColIdx ci = col_idx(d, KCS);
   double acc[1] = {0.0};
   if (ci.active)
     column<double, OP, false>(d, ci, x, kap, pse, scale, [&](int64_t c, int64_t n, double xc, double v) {
       const double r = rhs[n] - v;
-      res[n] = r;
-      r0[n] = r;
+      res[c] = r;
+      r0[c] = r;
       if (prj) prj[c] = r;
-      x0[n] = xc;
+      x0[c] = xc;
       acc[0] += r * r;
     });
-  double tot[1];
-  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) {
-    cg->rr = tot[0];
-    cg->rmax2 = tot[0];
-  }
+  block_partials<1>(acc, partials);
 }
 
 // ||b_eff||^2 with b_eff = r0 + A_hom x (the residual of a zero interior with
@@ -176,81 +168,56 @@ template <int OP>
 __global__ __launch_bounds__(256) void k_beff(Dims d, const double* __restrict__ x, const double* __restrict__ kap,
                                               const double* __restrict__ pse, double scale,
                                               const double* __restrict__ r0, double rtol, CgDev* cg,
-                                              double* partials, unsigned* counter) {
-  const ColIdx ci = col_idx(d);
+                                              double* partials) {
+  const ColIdx ci = col_idx(d, KCS);
   double acc[1] = {0.0};
   if (ci.active)
-    column<double, OP, true>(d, ci, x, kap, pse, scale, [&](int64_t, int64_t n, double, double v) {
-      const double b = r0[n] + v;
+    column<double, OP, true>(d, ci, x, kap, pse, scale, [&](int64_t c, int64_t, double, double v) {
+      const double b = r0[c] + v;
       acc[0] += b * b;
     });
-  double tot[1];
-  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) {
-    cg->bn2 = tot[0];
-    cg->tol2 = rtol * rtol * tot[0];
-    cg->rr_true = cg->rr;
-    if (cg->stop_rule == ELSIM_STOP_RELRES && !(cg->rr > cg->tol2)) cg->done = 1;
-  }
+  block_partials<1>(acc, partials);
 }
 
-// FP64 reference CG update (SBM:533-547, 688-705, 769-786).
+// FP64 reference CG update (SBM:533-547, 688-705, 769-786), flat over the box.
 template <int OP>
-__global__ __launch_bounds__(256) void k_cg64_update(Dims d, double* __restrict__ x, const double* __restrict__ prj,
+__global__ __launch_bounds__(256) void k_cg64_update(int64_t G, double* __restrict__ x, const double* __restrict__ prj,
                                                      double* __restrict__ res, const double* __restrict__ wmx,
-                                                     const double* __restrict__ w8, CgDev* cg, double* partials,
-                                                     unsigned* counter) {
+                                                     const double* __restrict__ w8, CgDev* cg, double* partials) {
   if (cg->done) return;
   const double alp = cg->alpha;
-  const ColIdx ci = col_idx(d);
   double acc[2] = {0.0, 0.0};
-  for_col(d, ci, [&](int64_t c, int64_t n) {
-    const double xo = x[c];
-    const double xn = xo + alp * prj[c];
-    x[c] = xn;
-    const double r = res[n] - alp * wmx[n];
-    res[n] = r;
+  double xo[FLAT_U], pv[FLAT_U], rv[FLAT_U], wv[FLAT_U], ww[FLAT_U];
+  FLAT_LOOP(G, { xo[u] = x[q]; pv[u] = prj[q]; rv[u] = res[q]; wv[u] = wmx[q]; ww[u] = w8[q]; }, {
+    const double xn = xo[u] + alp * pv[u];
+    x[q] = xn;
+    const double r = rv[u] - alp * wv[u];
+    res[q] = r;
     acc[0] += r * r;
     if (OP == ELSIM_OP_CNE) {
-      acc[1] += r * r * w8[c];
+      acc[1] += r * r * ww[u];
     } else {
-      const double t = xn - xo;
-      acc[1] += t * t * w8[c];
+      const double t = xn - xo[u];
+      acc[1] += t * t * ww[u];
     }
-  });
-  double tot[2];
-  if (reduce_last_block<2>(acc, partials, counter, tot) && linear_tid() == 0) {
-    const double dmy = cg->rr, rrn = tot[0];
-    const double bet = rrn / dmy;
-    cg->beta = bet;
-    cg->rr = rrn;
-    cg->iters += 1;
-    if (!isfinite(alp) || !isfinite(bet)) {
-      cg->done = 3;
-    } else if (cg->stop_rule == ELSIM_STOP_REFERENCE) {
-      const double st = sqrt(tot[1] / cg->tw);
-      cg->stop = st;
-      if (!(st > cg->crt)) cg->done = 1;
-    } else if (!(rrn > cg->tol2)) {
-      cg->done = 1;
-    }
-    if (!cg->done && cg->iters > cg->maxit) cg->done = 2;
-  }
+  })
+  block_partials<2>(acc, partials);
 }
 
 // PRJ = RES + bet*PRJ (SBM:542, 697, 778).
-__global__ __launch_bounds__(256) void k_cg64_dir(Dims d, double* __restrict__ prj, const double* __restrict__ res,
+__global__ __launch_bounds__(256) void k_cg64_dir(int64_t G, double* __restrict__ prj, const double* __restrict__ res,
                                                   const CgDev* cg) {
   if (cg->done) return;
   const double bet = cg->beta;
-  const ColIdx ci = col_idx(d);
-  for_col(d, ci, [&](int64_t c, int64_t n) { prj[c] = res[n] + bet * prj[c]; });
+  double rv[FLAT_U], pv[FLAT_U];
+  FLAT_LOOP(G, { rv[u] = res[q]; pv[u] = prj[q]; }, { prj[q] = rv[u] + bet * pv[u]; })
 }
 
 // e = x - x0 (interior of a ghosted array).
 __global__ __launch_bounds__(256) void k_diff(Dims d, const double* __restrict__ x, const double* __restrict__ x0,
                                               double* __restrict__ e) {
   const ColIdx ci = col_idx(d);
-  for_col(d, ci, [&](int64_t c, int64_t n) { e[c] = x[c] - x0[n]; });
+  for_col(d, ci, [&](int64_t c, int64_t, int) { e[c] = x[c] - x0[c]; });
 }
 
 // FP64 true residual of the correction system: r = r0 - A_hom e.
@@ -260,25 +227,236 @@ template <int OP, bool FINAL>
 __global__ __launch_bounds__(256) void k_true_resid(Dims d, const double* __restrict__ e, const double* __restrict__ kap,
                                                     const double* __restrict__ pse, double scale,
                                                     const double* __restrict__ r0, float* __restrict__ r32,
-                                                    const float* __restrict__ dinv, CgDev* cg, double* partials,
-                                                    unsigned* counter) {
+                                                    const float* __restrict__ dinv, CgDev* cg, double* partials) {
   if (!FINAL && (cg->done || !cg->need_reliable)) return;
-  const ColIdx ci = col_idx(d);
+  const ColIdx ci = col_idx(d, KCS);
   double acc[2] = {0.0, 0.0};
   if (ci.active)
-    column<double, OP, true>(d, ci, e, kap, pse, scale, [&](int64_t, int64_t n, double, double v) {
-      const double r = r0[n] - v;
+    column<double, OP, true>(d, ci, e, kap, pse, scale, [&](int64_t c, int64_t, double, double v) {
+      const double r = r0[c] - v;
       acc[0] += r * r;
       if (!FINAL) {
         const float rf = (float)r;
-        r32[n] = rf;
-        acc[1] += (double)rf * (double)(dinv[n] * rf);
+        r32[c] = rf;
+        acc[1] += (double)rf * (double)(dinv[c] * rf);
       }
     });
-  double tot[2];
-  if (reduce_last_block<2>(acc, partials, counter, tot) && linear_tid() == 0) {
-    cg->rr_true = tot[0];
-    if (!FINAL) {
+  block_partials<2>(acc, partials);
+}
+
+// ------------------------------------------------------------- Jacobi --
+
+// dinv = 1 / diag(A_hom) (FP64), stored as T.  Mirror faces do not
+// contribute; zero-Dirichlet faces do.  Interior of a ghosted array.
+template <int OP, typename T>
+__global__ __launch_bounds__(256) void k_dinv(Dims d, const double* __restrict__ kap, const double* __restrict__ pse,
+                                              double scale, T* __restrict__ dinv) {
+  const ColIdx ci = col_idx(d);
+  const int64_t sx = d.g1(), sz = d.g12();
+  for_col(d, ci, [&](int64_t c, int64_t, int k) {
+    const double kc = kap[c];
+    double s = 0.0;
+    if (ci.i > 1) s += (kap[c - 1] + kc) * 0.5;
+    if (ci.i < d.n1) s += (kap[c + 1] + kc) * 0.5;
+    if (ci.j > 1 || Rule<OP>::lo_zero) s += (kap[c - sx] + kc) * 0.5;
+    if (ci.j < d.n2 || Rule<OP>::hi_zero) s += (kap[c + sx] + kc) * 0.5;
+    if (k > 1) s += (kap[c - sz] + kc) * 0.5;
+    if (k < d.n3) s += (kap[c + sz] + kc) * 0.5;
+    const double dg = Rule<OP>::diag ? pse[c] + scale * s : -scale * s;
+    dinv[c] = (T)(1.0 / dg);
+  });
+}
+
+// ------------------------------------------------------ FP64 Jacobi-PCG --
+
+// p = dinv r0, rz = r.z
+__global__ __launch_bounds__(256) void k_pcg64_start(int64_t G, const double* __restrict__ res,
+                                                     const double* __restrict__ dinv, double* __restrict__ prj,
+                                                     CgDev* cg, double* partials) {
+  double acc[1] = {0.0};
+  double rv[FLAT_U], dv[FLAT_U];
+  FLAT_LOOP(G, { rv[u] = res[q]; dv[u] = dinv[q]; }, {
+    const double z = dv[u] * rv[u];
+    prj[q] = z;
+    acc[0] += rv[u] * z;
+  })
+  block_partials<1>(acc, partials);
+}
+
+// x += alpha p; r -= alpha w; rz' = r.(dinv r), rr = r.r
+__global__ __launch_bounds__(256) void k_pcg64_update(int64_t G, double* __restrict__ x, const double* __restrict__ prj,
+                                                      double* __restrict__ res, const double* __restrict__ wmx,
+                                                      const double* __restrict__ dinv, CgDev* cg, double* partials) {
+  if (cg->done) return;
+  const double alp = cg->alpha;
+  double acc[2] = {0.0, 0.0};
+  double xv[FLAT_U], pv[FLAT_U], rv[FLAT_U], wv[FLAT_U], dv[FLAT_U];
+  FLAT_LOOP(G, { xv[u] = x[q]; pv[u] = prj[q]; rv[u] = res[q]; wv[u] = wmx[q]; dv[u] = dinv[q]; }, {
+    x[q] = xv[u] + alp * pv[u];
+    const double r = rv[u] - alp * wv[u];
+    res[q] = r;
+    acc[0] += r * (dv[u] * r);
+    acc[1] += r * r;
+  })
+  block_partials<2>(acc, partials);
+}
+
+// p = dinv r + beta p
+__global__ __launch_bounds__(256) void k_pcg64_dir(int64_t G, double* __restrict__ prj, const double* __restrict__ res,
+                                                   const double* __restrict__ dinv, const CgDev* cg) {
+  if (cg->done) return;
+  const double bet = cg->beta;
+  double rv[FLAT_U], pv[FLAT_U], dv[FLAT_U];
+  FLAT_LOOP(G, { rv[u] = res[q]; pv[u] = prj[q]; dv[u] = dinv[q]; }, { prj[q] = dv[u] * rv[u] + bet * pv[u]; })
+}
+
+// ------------------------------------------------ mixed-precision kernels --
+
+// FP32 copies of the coefficient (and psE) over the ghosted box.
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  double v[FLAT_U];
+  FLAT_LOOP(n, { v[u] = a[q]; }, { b[q] = (float)v[u]; })
+}
+
+// Mixed start: r32 = r0, p32 = dinv r32, y32 = 0, e64 = 0; rz = r.z.
+__global__ __launch_bounds__(256) void k_mx_start(int64_t G, const double* __restrict__ r0, const float* __restrict__ dinv,
+                                                  float* __restrict__ r32, float* __restrict__ p32,
+                                                  float* __restrict__ y32, double* __restrict__ e64, CgDev* cg,
+                                                  double* partials) {
+  double acc[1] = {0.0};
+  double rv[FLAT_U];
+  float dv[FLAT_U];
+  FLAT_LOOP(G, { rv[u] = r0[q]; dv[u] = dinv[q]; }, {
+    const float r = (float)rv[u];
+    const float z = dv[u] * r;
+    r32[q] = r;
+    p32[q] = z;
+    y32[q] = 0.0f;
+    e64[q] = 0.0;
+    acc[0] += (double)r * (double)z;
+  })
+  block_partials<1>(acc, partials);
+}
+
+// y += alpha p; r -= alpha w; rz' = r.(dinv r); decide on a reliable update.
+__global__ __launch_bounds__(256) void k_mx_update(int64_t G, float* __restrict__ y32, const float* __restrict__ p32,
+                                                   float* __restrict__ r32, const float* __restrict__ w32,
+                                                   const float* __restrict__ dinv, CgDev* cg, double* partials) {
+  if (cg->done || cg->need_reliable) return;
+  const float alp = (float)cg->alpha;
+  double acc[2] = {0.0, 0.0};
+  float yv[FLAT_U], pv[FLAT_U], rv[FLAT_U], wv[FLAT_U], dv[FLAT_U];
+  FLAT_LOOP(G, { yv[u] = y32[q]; pv[u] = p32[q]; rv[u] = r32[q]; wv[u] = w32[q]; dv[u] = dinv[q]; }, {
+    y32[q] = yv[u] + alp * pv[u];
+    const float r = rv[u] - alp * wv[u];
+    r32[q] = r;
+    acc[0] += (double)r * (double)(dv[u] * r);
+    acc[1] += (double)r * (double)r;
+  })
+  block_partials<2>(acc, partials);
+}
+
+// Reliable update, part 1: e64 += y32, y32 = 0.
+__global__ __launch_bounds__(256) void k_mx_fold(int64_t G, double* __restrict__ e64, float* __restrict__ y32,
+                                                 const CgDev* cg) {
+  if (cg->done || !cg->need_reliable) return;
+  double ev[FLAT_U];
+  float yv[FLAT_U];
+  FLAT_LOOP(G, { ev[u] = e64[q]; yv[u] = y32[q]; }, {
+    e64[q] = ev[u] + (double)yv[u];
+    y32[q] = 0.0f;
+  })
+}
+
+// p = dinv r + beta p.
+__global__ __launch_bounds__(256) void k_mx_dir(int64_t G, float* __restrict__ p32, const float* __restrict__ r32,
+                                                const float* __restrict__ dinv, const CgDev* cg) {
+  if (cg->done || cg->need_reliable) return;
+  const float bet = (float)cg->beta;
+  float rv[FLAT_U], pv[FLAT_U], dv[FLAT_U];
+  FLAT_LOOP(G, { rv[u] = r32[q]; pv[u] = p32[q]; dv[u] = dinv[q]; }, { p32[q] = dv[u] * rv[u] + bet * pv[u]; })
+}
+
+// x += e (ghosts of e are zero).
+__global__ __launch_bounds__(256) void k_add_e(int64_t G, double* __restrict__ x, const double* __restrict__ e) {
+  double xv[FLAT_U], ev[FLAT_U];
+  FLAT_LOOP(G, { xv[u] = x[q]; ev[u] = e[q]; }, { x[q] = xv[u] + ev[u]; })
+}
+
+__global__ __launch_bounds__(256) void k_copy_g2n(Dims d, const double* __restrict__ g, double* __restrict__ nn) {
+  const ColIdx ci = col_idx(d);
+  for_col(d, ci, [&](int64_t c, int64_t n, int) { nn[n] = g[c]; });
+}
+
+// ---------------------------------------------------------- finalising --
+
+enum FinOp {
+  FIN_DOT = 0,        // *dot = sum
+  FIN_ALPHA_RR,       // pw = sum, alpha = rr / pw
+  FIN_ALPHA_RZ,       // pw = sum, alpha = rz / pw
+  FIN_RESID,          // rr = rmax2 = sum
+  FIN_BEFF,           // bn2, tol2, relative-rule early exit
+  FIN_CG64,           // reference CG: beta, stop rules
+  FIN_TRUE_FINAL,     // rr_true
+  FIN_TRUE_RELIABLE,  // mixed reliable update: restart from the true residual
+  FIN_RZ,             // rz = sum
+  FIN_PCG64,          // FP64 Jacobi-PCG: beta, stop
+  FIN_MX_UPDATE       // mixed: beta or request a reliable update
+};
+
+// The skip test repeats the producer's, evaluated on the same (unchanged)
+// scalars, so a skipped producer is never finalised.
+template <int NV>
+__global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __restrict__ partials, CgDev* cg,
+                                               double* dot, double rtol) {
+  bool skip = false;
+  if (cg) {
+    if (op == FIN_ALPHA_RR || op == FIN_ALPHA_RZ || op == FIN_MX_UPDATE) skip = cg->done || cg->need_reliable;
+    if (op == FIN_CG64 || op == FIN_PCG64) skip = cg->done;
+    if (op == FIN_TRUE_RELIABLE) skip = cg->done || !cg->need_reliable;
+  }
+  if (skip) return;
+  double tot[NV];
+  sum_partials<NV>(partials, nb, tot);
+  if (linear_tid() != 0) return;
+  switch (op) {
+    case FIN_DOT: *dot = tot[0]; break;
+    case FIN_ALPHA_RR:
+    case FIN_ALPHA_RZ:
+      cg->pw = tot[0];
+      cg->alpha = (op == FIN_ALPHA_RZ ? cg->rz : cg->rr) / tot[0];
+      break;
+    case FIN_RESID:
+      cg->rr = tot[0];
+      cg->rmax2 = tot[0];
+      break;
+    case FIN_BEFF:
+      cg->bn2 = tot[0];
+      cg->tol2 = rtol * rtol * tot[0];
+      cg->rr_true = cg->rr;
+      if (cg->stop_rule == ELSIM_STOP_RELRES && !(cg->rr > cg->tol2)) cg->done = 1;
+      break;
+    case FIN_CG64: {
+      const double alp = cg->alpha, dmy = cg->rr, rrn = tot[0];
+      const double bet = rrn / dmy;
+      cg->beta = bet;
+      cg->rr = rrn;
+      cg->iters += 1;
+      if (!isfinite(alp) || !isfinite(bet)) {
+        cg->done = 3;
+      } else if (cg->stop_rule == ELSIM_STOP_REFERENCE) {
+        const double st = sqrt(tot[1] / cg->tw);
+        cg->stop = st;
+        if (!(st > cg->crt)) cg->done = 1;
+      } else if (!(rrn > cg->tol2)) {
+        cg->done = 1;
+      }
+      if (!cg->done && cg->iters > cg->maxit) cg->done = 2;
+      break;
+    }
+    case FIN_TRUE_FINAL: cg->rr_true = tot[0]; break;
+    case FIN_TRUE_RELIABLE:
+      cg->rr_true = tot[0];
       cg->n_reliable += 1;
       cg->need_reliable = 0;
       if (!isfinite(tot[0]) || !isfinite(tot[1])) {
@@ -292,120 +470,36 @@ __global__ __launch_bounds__(256) void k_true_resid(Dims d, const double* __rest
         cg->rz = tot[1];
         cg->rmax2 = tot[0];
       }
-    }
-  }
-}
-
-// ------------------------------------------------ mixed-precision kernels --
-
-// FP32 copies of the coefficient (and psE) over the ghosted box.
-__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-    b[q] = (float)a[q];
-}
-
-// Jacobi: dinv = 1 / diag(A_hom) (FP64, stored FP32).  Mirror faces do not
-// contribute; zero-Dirichlet faces do.
-template <int OP>
-__global__ __launch_bounds__(256) void k_dinv(Dims d, const double* __restrict__ kap, const double* __restrict__ pse,
-                                              double scale, float* __restrict__ dinv) {
-  const ColIdx ci = col_idx(d);
-  const int64_t sx = d.g1(), sz = d.g12();
-  for_col(d, ci, [&](int64_t c, int64_t n) {
-    const int k = ci.k0 + (int)((c - d.gi(ci.i, ci.j, ci.k0)) / sz);
-    const double kc = kap[c];
-    double s = 0.0;
-    if (ci.i > 1) s += (kap[c - 1] + kc) * 0.5;
-    if (ci.i < d.n1) s += (kap[c + 1] + kc) * 0.5;
-    if (ci.j > 1 || Rule<OP>::lo_zero) s += (kap[c - sx] + kc) * 0.5;
-    if (ci.j < d.n2 || Rule<OP>::hi_zero) s += (kap[c + sx] + kc) * 0.5;
-    if (k > 1) s += (kap[c - sz] + kc) * 0.5;
-    if (k < d.n3) s += (kap[c + sz] + kc) * 0.5;
-    const double dg = Rule<OP>::diag ? pse[c] + scale * s : -scale * s;
-    dinv[n] = (float)(1.0 / dg);
-  });
-}
-
-// Mixed start: r32 = r0, p32 = dinv r32, y32 = 0, e64 = 0; rz = r.z.
-__global__ __launch_bounds__(256) void k_mx_start(Dims d, const double* __restrict__ r0, const float* __restrict__ dinv,
-                                                  float* __restrict__ r32, float* __restrict__ p32,
-                                                  float* __restrict__ y32, double* __restrict__ e64, CgDev* cg,
-                                                  double* partials, unsigned* counter) {
-  const ColIdx ci = col_idx(d);
-  double acc[1] = {0.0};
-  for_col(d, ci, [&](int64_t c, int64_t n) {
-    const float r = (float)r0[n];
-    const float z = dinv[n] * r;
-    r32[n] = r;
-    p32[c] = z;
-    y32[n] = 0.0f;
-    e64[c] = 0.0;
-    acc[0] += (double)r * (double)z;
-  });
-  double tot[1];
-  if (reduce_last_block<1>(acc, partials, counter, tot) && linear_tid() == 0) cg->rz = tot[0];
-}
-
-// y += alpha p; r -= alpha w; rz' = r.(dinv r); decide on a reliable update.
-__global__ __launch_bounds__(256) void k_mx_update(Dims d, float* __restrict__ y32, const float* __restrict__ p32,
-                                                   float* __restrict__ r32, const float* __restrict__ w32,
-                                                   const float* __restrict__ dinv, CgDev* cg, double* partials,
-                                                   unsigned* counter) {
-  if (cg->done || cg->need_reliable) return;
-  const float alp = (float)cg->alpha;
-  const ColIdx ci = col_idx(d);
-  double acc[2] = {0.0, 0.0};
-  for_col(d, ci, [&](int64_t c, int64_t n) {
-    y32[n] = y32[n] + alp * p32[c];
-    const float r = r32[n] - alp * w32[n];
-    r32[n] = r;
-    acc[0] += (double)r * (double)(dinv[n] * r);
-    acc[1] += (double)r * (double)r;
-  });
-  double tot[2];
-  if (reduce_last_block<2>(acc, partials, counter, tot) && linear_tid() == 0) {
-    cg->iters += 1;
-    cg->rr = tot[1];
-    if (!isfinite(cg->alpha) || !isfinite(tot[0])) {
-      cg->done = 3;
-    } else if (!(tot[1] > cg->delta2 * cg->rmax2) || !(tot[1] > cg->tol2) || cg->iters >= cg->maxit) {
-      cg->need_reliable = 1;
-    } else {
-      cg->beta = tot[0] / cg->rz;
+      break;
+    case FIN_RZ: cg->rz = tot[0]; break;
+    case FIN_PCG64: {
+      const double alp = cg->alpha;
+      cg->iters += 1;
+      cg->rr = tot[1];
+      const double bet = tot[0] / cg->rz;
+      cg->beta = bet;
       cg->rz = tot[0];
+      if (!isfinite(alp) || !isfinite(bet))
+        cg->done = 3;
+      else if (!(tot[1] > cg->tol2))
+        cg->done = 1;
+      else if (cg->iters >= cg->maxit)
+        cg->done = 2;
+      break;
     }
+    case FIN_MX_UPDATE:
+      cg->iters += 1;
+      cg->rr = tot[1];
+      if (!isfinite(cg->alpha) || !isfinite(tot[0])) {
+        cg->done = 3;
+      } else if (!(tot[1] > cg->delta2 * cg->rmax2) || !(tot[1] > cg->tol2) || cg->iters >= cg->maxit) {
+        cg->need_reliable = 1;
+      } else {
+        cg->beta = tot[0] / cg->rz;
+        cg->rz = tot[0];
+      }
+      break;
   }
-}
-
-// Reliable update, part 1: e64 += y32, y32 = 0.
-__global__ __launch_bounds__(256) void k_mx_fold(Dims d, double* __restrict__ e64, float* __restrict__ y32,
-                                                 const CgDev* cg) {
-  if (cg->done || !cg->need_reliable) return;
-  const ColIdx ci = col_idx(d);
-  for_col(d, ci, [&](int64_t c, int64_t n) {
-    e64[c] = e64[c] + (double)y32[n];
-    y32[n] = 0.0f;
-  });
-}
-
-// p = dinv r + beta p.
-__global__ __launch_bounds__(256) void k_mx_dir(Dims d, float* __restrict__ p32, const float* __restrict__ r32,
-                                                const float* __restrict__ dinv, const CgDev* cg) {
-  if (cg->done || cg->need_reliable) return;
-  const float bet = (float)cg->beta;
-  const ColIdx ci = col_idx(d);
-  for_col(d, ci, [&](int64_t c, int64_t n) { p32[c] = dinv[n] * r32[n] + bet * p32[c]; });
-}
-
-// x += e (interior).
-__global__ __launch_bounds__(256) void k_add_e(Dims d, double* __restrict__ x, const double* __restrict__ e) {
-  const ColIdx ci = col_idx(d);
-  for_col(d, ci, [&](int64_t c, int64_t) { x[c] = x[c] + e[c]; });
-}
-
-__global__ __launch_bounds__(256) void k_copy_g2n(Dims d, const double* __restrict__ g, double* __restrict__ nn) {
-  const ColIdx ci = col_idx(d);
-  for_col(d, ci, [&](int64_t c, int64_t n) { nn[n] = g[c]; });
 }
 
 // ------------------------------------------------------------------ host --
@@ -413,32 +507,22 @@ __global__ __launch_bounds__(256) void k_copy_g2n(Dims d, const double* __restri
 int CgWork::alloc(const Dims& dd) {
   if (d.n1 == dd.n1 && d.n2 == dd.n2 && d.n3 == dd.n3 && cg.p) return ELSIM_OK;
   d = dd;
-  const size_t N = (size_t)d.N(), G = (size_t)d.G();
-  ELSIM_TRY(dev_alloc(res, N * 8));
-  ELSIM_TRY(dev_alloc(wmx, N * 8));
-  ELSIM_TRY(dev_alloc(prj, G * 8));
-  ELSIM_TRY(dev_alloc(x0, N * 8));
-  ELSIM_TRY(dev_alloc(r0, N * 8));
-  ELSIM_TRY(dev_alloc(e64, G * 8));
-  ELSIM_TRY(dev_alloc(y32, N * 4));
-  ELSIM_TRY(dev_alloc(r32, N * 4));
-  ELSIM_TRY(dev_alloc(w32, N * 4));
-  ELSIM_TRY(dev_alloc(p32, G * 4));
-  ELSIM_TRY(dev_alloc(kap32, G * 4));
-  ELSIM_TRY(dev_alloc(pse32, G * 4));
-  ELSIM_TRY(dev_alloc(dinv32, N * 4));
+  const size_t G = (size_t)d.G();
+  DevBuf* g64[] = {&res, &wmx, &prj, &x0, &r0, &e64, &dinv64};
+  DevBuf* g32[] = {&y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32};
+  for (DevBuf* b : g64) ELSIM_TRY(dev_alloc(*b, G * 8));
+  for (DevBuf* b : g32) ELSIM_TRY(dev_alloc(*b, G * 4));
   ELSIM_TRY(dev_alloc(cg, sizeof(CgDev)));
   if (!host) ELSIM_HIP(hipHostMalloc((void**)&host, sizeof(CgDev), hipHostMallocDefault));
-  // ghost memory of the search directions is read (then replaced in
-  // registers): give it defined contents once
-  ELSIM_HIP(hipMemsetAsync(prj.p, 0, G * 8, stream()));
-  ELSIM_HIP(hipMemsetAsync(p32.p, 0, G * 4, stream()));
-  ELSIM_HIP(hipMemsetAsync(e64.p, 0, G * 8, stream()));
+  // every work vector keeps zero ghosts (the kernels only write interiors
+  // or add zero ghosts to zero ghosts)
+  for (DevBuf* b : g64) ELSIM_HIP(hipMemsetAsync(b->p, 0, G * 8, stream()));
+  for (DevBuf* b : g32) ELSIM_HIP(hipMemsetAsync(b->p, 0, G * 4, stream()));
   return ELSIM_OK;
 }
 
 void CgWork::release() {
-  DevBuf* bs[] = {&res, &wmx, &prj, &x0, &r0, &e64, &y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32, &cg};
+  DevBuf* bs[] = {&res, &wmx, &prj, &x0, &r0, &e64, &y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32, &dinv64, &cg};
   for (DevBuf* b : bs) dev_free(*b);
   if (host) (void)hipHostFree(host);
   host = nullptr;
@@ -447,13 +531,22 @@ void CgWork::release() {
 
 namespace {
 
+int poll(CgWork& w, hipStream_t s) {
+  ELSIM_HIP(hipMemcpyAsync(w.host, w.cg.p, sizeof(CgDev), hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  return ELSIM_OK;
+}
+
 template <int OP>
 int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   const Dims& d = a.d;
-  Workspace& ws = workspace();
+  double* P = workspace().partials;
   hipStream_t s = stream();
-  const dim3 grid = col_grid(d), blk(TX, TY, 1);
-  if (blocks_of(grid) > kMaxRedBlocks) {
+  const dim3 gs = col_grid(d, KCS), gc = col_grid(d), blk(TX, TY, 1);
+  const int nbs = blocks_of(gs);
+  const int64_t G = d.G();
+  const int fb = flat_blocks(G);
+  if (nbs > kMaxRedBlocks || fb > kMaxRedBlocks) {
     set_error("grid too large for the reduction workspace");
     return ELSIM_ERR_ARG;
   }
@@ -462,7 +555,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   h.crt = a.crt;
   h.tw = a.tw;
   h.maxit = a.maxit;
-  h.stop_rule = mode == ELSIM_MODE_MIXED ? ELSIM_STOP_RELRES : a.stop_rule;
+  h.stop_rule = mode == ELSIM_MODE_FP64 ? a.stop_rule : ELSIM_STOP_RELRES;
   h.delta2 = a.delta * a.delta;
   ELSIM_HIP(hipMemcpyAsync(cg, &h, sizeof(CgDev), hipMemcpyHostToDevice, s));
   double* res = (double*)w.res.p;
@@ -471,61 +564,83 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   double* x0 = (double*)w.x0.p;
   double* r0 = (double*)w.r0.p;
   double* e64 = (double*)w.e64.p;
-  float* dinv = (float*)w.dinv32.p;
-  k_resid64<OP><<<grid, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, res, r0, mode == ELSIM_MODE_FP64 ? prj : nullptr,
-                                     x0, cg, ws.partials, ws.counter);
-  k_beff<OP><<<grid, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, r0, a.rtol, cg, ws.partials, ws.counter);
+  auto fin1 = [&](int op, int nb) { k_fin<1><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
+  auto fin2 = [&](int op, int nb) { k_fin<2><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
+  k_resid64<OP><<<gs, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, res, r0, mode == ELSIM_MODE_MIXED ? nullptr : prj,
+                                   x0, cg, P);
+  fin1(FIN_RESID, nbs);
+  k_beff<OP><<<gs, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, r0, a.rtol, cg, P);
+  fin1(FIN_BEFF, nbs);
   ELSIM_CHECK_LAUNCH();
   const int batch = 8;
-  if (mode == ELSIM_MODE_FP64) {
+  if (mode == ELSIM_MODE_FP64_PCG) {
+    double* dinv64 = (double*)w.dinv64.p;
+    k_dinv<OP, double><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv64);
+    k_pcg64_start<<<fb, FLAT_T, 0, s>>>(G, res, dinv64, prj, cg, P);
+    fin1(FIN_RZ, fb);
     for (;;) {
       for (int b = 0; b < batch; b++) {
-        k_matvec<double, OP, true><<<grid, blk, 0, s>>>(d, prj, a.kap, a.pse, a.scale, wmx, nullptr, cg, 0, ws.partials,
-                                                        ws.counter);
-        k_cg64_update<OP><<<grid, blk, 0, s>>>(d, a.x, prj, res, wmx, a.w8, cg, ws.partials, ws.counter);
-        k_cg64_dir<<<grid, blk, 0, s>>>(d, prj, res, cg);
+        k_matvec<double, OP, true, true><<<gs, blk, 0, s>>>(d, prj, a.kap, a.pse, a.scale, wmx, nullptr, cg, 1, P);
+        fin1(FIN_ALPHA_RZ, nbs);
+        k_pcg64_update<<<fb, FLAT_T, 0, s>>>(G, a.x, prj, res, wmx, dinv64, cg, P);
+        fin2(FIN_PCG64, fb);
+        k_pcg64_dir<<<fb, FLAT_T, 0, s>>>(G, prj, res, dinv64, cg);
       }
       ELSIM_CHECK_LAUNCH();
-      ELSIM_HIP(hipMemcpyAsync(w.host, cg, sizeof(CgDev), hipMemcpyDeviceToHost, s));
-      ELSIM_HIP(hipStreamSynchronize(s));
+      ELSIM_TRY(poll(w, s));
+      if (w.host->done) break;
+    }
+    k_diff<<<gc, blk, 0, s>>>(d, a.x, x0, e64);
+    k_true_resid<OP, true><<<gs, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, nullptr, nullptr, cg, P);
+    fin2(FIN_TRUE_FINAL, nbs);
+  } else if (mode == ELSIM_MODE_FP64) {
+    for (;;) {
+      for (int b = 0; b < batch; b++) {
+        k_matvec<double, OP, true, true><<<gs, blk, 0, s>>>(d, prj, a.kap, a.pse, a.scale, wmx, nullptr, cg, 0, P);
+        fin1(FIN_ALPHA_RR, nbs);
+        k_cg64_update<OP><<<fb, FLAT_T, 0, s>>>(G, a.x, prj, res, wmx, a.w8, cg, P);
+        fin2(FIN_CG64, fb);
+        k_cg64_dir<<<fb, FLAT_T, 0, s>>>(G, prj, res, cg);
+      }
+      ELSIM_CHECK_LAUNCH();
+      ELSIM_TRY(poll(w, s));
       if (w.host->done) break;
     }
     // true residual of the correction system for the report
-    k_diff<<<grid, blk, 0, s>>>(d, a.x, x0, e64);
-    k_true_resid<OP, true><<<grid, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, nullptr, nullptr, cg, ws.partials,
-                                                ws.counter);
+    k_diff<<<gc, blk, 0, s>>>(d, a.x, x0, e64);
+    k_true_resid<OP, true><<<gs, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, nullptr, nullptr, cg, P);
+    fin2(FIN_TRUE_FINAL, nbs);
   } else {
-    const int64_t G = d.G();
-    k_to_f32<<<flat_blocks(G), FLAT_T, 0, s>>>(G, a.kap, (float*)w.kap32.p);
-    if (OP == ELSIM_OP_CNE) k_to_f32<<<flat_blocks(G), FLAT_T, 0, s>>>(G, a.pse, (float*)w.pse32.p);
-    k_dinv<OP><<<grid, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv);
-    k_mx_start<<<grid, blk, 0, s>>>(d, r0, dinv, (float*)w.r32.p, (float*)w.p32.p, (float*)w.y32.p, e64, cg,
-                                    ws.partials, ws.counter);
+    float* dinv = (float*)w.dinv32.p;
+    k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.kap, (float*)w.kap32.p);
+    if (OP == ELSIM_OP_CNE) k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.pse, (float*)w.pse32.p);
+    k_dinv<OP, float><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv);
+    k_mx_start<<<fb, FLAT_T, 0, s>>>(G, r0, dinv, (float*)w.r32.p, (float*)w.p32.p, (float*)w.y32.p, e64, cg, P);
+    fin1(FIN_RZ, fb);
     ELSIM_CHECK_LAUNCH();
     const float sc32 = (float)a.scale;
-    // the first iteration compares against the initial residual norm
     for (;;) {
       for (int b = 0; b < batch; b++) {
-        k_matvec<float, OP, true><<<grid, blk, 0, s>>>(d, (const float*)w.p32.p, (const float*)w.kap32.p,
-                                                       (const float*)w.pse32.p, sc32, (float*)w.w32.p, nullptr, cg, 1,
-                                                       ws.partials, ws.counter);
-        k_mx_update<<<grid, blk, 0, s>>>(d, (float*)w.y32.p, (const float*)w.p32.p, (float*)w.r32.p,
-                                         (const float*)w.w32.p, dinv, cg, ws.partials, ws.counter);
-        k_mx_fold<<<grid, blk, 0, s>>>(d, e64, (float*)w.y32.p, cg);
-        k_true_resid<OP, false><<<grid, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, (float*)w.r32.p, dinv, cg,
-                                                     ws.partials, ws.counter);
-        k_mx_dir<<<grid, blk, 0, s>>>(d, (float*)w.p32.p, (const float*)w.r32.p, dinv, cg);
+        k_matvec<float, OP, true, true><<<gs, blk, 0, s>>>(d, (const float*)w.p32.p, (const float*)w.kap32.p,
+                                                           (const float*)w.pse32.p, sc32, (float*)w.w32.p, nullptr, cg,
+                                                           1, P);
+        fin1(FIN_ALPHA_RZ, nbs);
+        k_mx_update<<<fb, FLAT_T, 0, s>>>(G, (float*)w.y32.p, (const float*)w.p32.p, (float*)w.r32.p,
+                                          (const float*)w.w32.p, dinv, cg, P);
+        fin2(FIN_MX_UPDATE, fb);
+        k_mx_fold<<<fb, FLAT_T, 0, s>>>(G, e64, (float*)w.y32.p, cg);
+        k_true_resid<OP, false><<<gs, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, (float*)w.r32.p, dinv, cg, P);
+        fin2(FIN_TRUE_RELIABLE, nbs);
+        k_mx_dir<<<fb, FLAT_T, 0, s>>>(G, (float*)w.p32.p, (const float*)w.r32.p, dinv, cg);
       }
       ELSIM_CHECK_LAUNCH();
-      ELSIM_HIP(hipMemcpyAsync(w.host, cg, sizeof(CgDev), hipMemcpyDeviceToHost, s));
-      ELSIM_HIP(hipStreamSynchronize(s));
+      ELSIM_TRY(poll(w, s));
       if (w.host->done) break;
     }
-    k_add_e<<<grid, blk, 0, s>>>(d, a.x, e64);
+    k_add_e<<<fb, FLAT_T, 0, s>>>(G, a.x, e64);
   }
   ELSIM_CHECK_LAUNCH();
-  ELSIM_HIP(hipMemcpyAsync(w.host, cg, sizeof(CgDev), hipMemcpyDeviceToHost, s));
-  ELSIM_HIP(hipStreamSynchronize(s));
+  ELSIM_TRY(poll(w, s));
   const CgDev& r = *w.host;
   out->iters = r.iters;
   out->n_reliable = r.n_reliable;
@@ -539,8 +654,8 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
 
 int cg_solve(CgWork& w, const SolveArgs& a, int mode, SolveResult* res) {
   ELSIM_TRY(w.alloc(a.d));
-  if (mode == ELSIM_MODE_MIXED && a.stop_rule != ELSIM_STOP_RELRES) {
-    set_error("mixed precision requires the relative-residual stop rule");
+  if (mode != ELSIM_MODE_FP64 && a.stop_rule != ELSIM_STOP_RELRES) {
+    set_error("the preconditioned solvers use the relative-residual stop rule only");
     return ELSIM_ERR_ARG;
   }
   switch (a.op) {
@@ -556,15 +671,15 @@ template <typename T>
 static int launch_mv(int op, const Dims& d, const T* p, const T* kap, const T* pse, T scale, T* out, double* dot,
                      bool hom) {
   ELSIM_TRY(ensure_init());
-  Workspace& ws = workspace();
-  const dim3 grid = col_grid(d), blk(TX, TY, 1);
+  double* P = workspace().partials;
+  const dim3 grid = col_grid(d, KCS), blk(TX, TY, 1);
   if (blocks_of(grid) > kMaxRedBlocks) {
     set_error("grid too large for the reduction workspace");
     return ELSIM_ERR_ARG;
   }
   hipStream_t s = stream();
 #define MV(OPV, H) \
-  k_matvec<T, OPV, H><<<grid, blk, 0, s>>>(d, p, kap, pse, scale, out, dot, nullptr, 0, ws.partials, ws.counter)
+  k_matvec<T, OPV, H, false><<<grid, blk, 0, s>>>(d, p, kap, pse, scale, out, dot, nullptr, 0, P)
   if (op == ELSIM_OP_CNE) {
     if (hom) MV(ELSIM_OP_CNE, true); else MV(ELSIM_OP_CNE, false);
   } else if (op == ELSIM_OP_PHP) {
@@ -576,6 +691,7 @@ static int launch_mv(int op, const Dims& d, const T* p, const T* kap, const T* p
     return ELSIM_ERR_ARG;
   }
 #undef MV
+  if (dot) k_fin<1><<<1, FIN_T, 0, s>>>(FIN_DOT, blocks_of(grid), P, nullptr, dot, 0.0);
   ELSIM_CHECK_LAUNCH();
   return ELSIM_OK;
 }

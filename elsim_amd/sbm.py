@@ -24,14 +24,18 @@ def reference_params():
 
 def synthetic_params(**kw):
     """Nondimensional parameters of the synthetic electrochemical step (h = 1
-    voxel): psi = 0.5(1+tanh(d/2)), i0 = 1 (table) x 1e-3 (SBM:617), alpha = 0.5,
-    F/RT = 1, dt = 1e-3, every CG to relative residual 1e-8."""
+    voxel): psi = 0.5(1+tanh(d/2)), alpha = 0.5, F/RT = 1, dt = 1e-3, every CG to
+    relative residual 1e-8.  The exchange current is the table value 1 times
+    i0_scale = 5e-6 (the reference scales by 1e-3, SBM:617): it keeps the
+    kinetic-to-conductive ratio a i0 (F/RT) L^2 / kappa near the reference's
+    physical regime (~0.04) up to 512^3, where the block Gauss-Seidel coupling
+    loop (SBM:627) contracts; at i0 = 1e-3 it diverges beyond ~64^3."""
     tm = T_MINUS
     p = SbmParams(
         dx=1.0, dt=1.0e-3, zeta=2.0, psp_off=1.0e-7, pse_off=1.0e-6, avp_min=1.0e-2, eps=1.0, rho=1.0,
         ch_psi_min=1.0e-5, frd=1.0, cst1=1.0, t_minus=tm, d0=1.0, tc1=(2 * tm - 1.0) / (2 * tm * (1.0 - tm)),
         tc2=1.0 / (2 * tm * (1.0 - tm)), dab_c0=0.0, dab_c1=0.2, dab_c2=0.0, dab_c3=0.0, kap_scale=1.0,
-        i0_scale=1.0e-3, kfw_cref=1.0, bvp=0.0, bve=-0.5, vsr=0.1, trgi=0.0, cne_target=1.0, rtol=1.0e-8,
+        i0_scale=5.0e-6, kfw_cref=1.0, bvp=0.0, bve=-0.5, vsr=0.1, trgi=0.0, cne_target=1.0, rtol=1.0e-8,
         crt_e=1.0e-7, crt_p=1.0e-7, crt_l=1.0e-6, outer_tol=1.0e-8, inner_rtol=0.1, mass_corr_period=100,
         skip_cne_first=1, stop_rule=STOP_RELRES, max_it_e=50000, max_it_p=30000, max_it_l=100000, max_outer=30)
     for k, v in kw.items():

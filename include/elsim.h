@@ -53,6 +53,7 @@ extern "C" {
 /* Precision / warm-start modes of an electrochemical step. */
 #define ELSIM_MODE_FP64      0 /* reference CG (unpreconditioned) in FP64 */
 #define ELSIM_MODE_MIXED     1 /* FP32 Jacobi-PCG inner iterations, FP64 reliable updates */
+#define ELSIM_MODE_FP64_PCG  2 /* Jacobi-PCG in FP64 (same preconditioner as MIXED, all FP64) */
 
 /* Which linear system (operator + homogeneous ghost rule of the search
  * direction) a CG solve works on. */

@@ -53,7 +53,7 @@ def test_ch_update_bit_exact(gpu, oracle_lib):
     assert np.array_equal(g.get(F_CNP), o.get(F_CNP))
 
 
-@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_MIXED])
+@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_MIXED, E.MODE_FP64_PCG])
 def test_step_matches_oracle(gpu, oracle_lib, mode):
     n = 24
     sdf, sdl, prm = case(n, seed=2)
@@ -67,6 +67,10 @@ def test_step_matches_oracle(gpu, oracle_lib, mode):
         sg = g.step(mode)
         so = o.step()
         assert sg.status == 0 and so.status == 0
+        # the coupling loop follows the same path (SBM:627)
+        assert abs(sg.n_outer - so.n_outer) <= 1, (sg.n_outer, so.n_outer)
+        assert sg.xrf == pytest.approx(so.xrf, rel=1e-12)
+        assert sg.scrnt == pytest.approx(so.scrnt, rel=1e-6, abs=1e-12)
         for r in (sg.relres_php, sg.relres_phe):
             assert r <= 1e-8
         if step > 0:
