@@ -140,6 +140,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fast,fp64,ref", help="comma list of fast, fp64, ref")
     ap.add_argument("--inner-rtol", type=float, default=None)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the operator timing and a 512 MiB device copy (for counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path has no multi-GPU mode.",
@@ -149,6 +151,16 @@ def main():
     L = E.lib()
     E.check(L.elsim_init(0), "elsim_init")
     n = a.n
+    if a.roofline_only:
+        t_mv, gbs = matvec_roofline(n, reps=10)
+        nb = 512 << 20
+        src, dst = E.DeviceArray(nb, np.uint8), E.DeviceArray(nb, np.uint8)
+        E.check(L.elsim_memset(src.ptr, 1, nb))
+        for _ in range(10):
+            E.check(L.elsim_memcpy_d2d(dst.ptr, src.ptr, nb))
+        E.check(L.elsim_synchronize())
+        print(json.dumps({"matvec_us": t_mv * 1e6, "matvec_gbs": gbs, "copy_bytes": nb}))
+        return
     t0 = time.perf_counter()
     sdf, ns, frac, target = make_case(n, a.seed)
     log(f"case: n={n} spheres={ns} solid={frac:.3f} (target {target:.3f}) gen {time.perf_counter() - t0:.1f}s")

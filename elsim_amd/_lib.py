@@ -72,6 +72,7 @@ _SIGS = {
     "elsim_free": (I, [P]),
     "elsim_memcpy_h2d": (I, [P, P, C.c_size_t]),
     "elsim_memcpy_d2h": (I, [P, P, C.c_size_t]),
+    "elsim_memcpy_d2d": (I, [P, P, C.c_size_t]),
     "elsim_memset": (I, [P, I, C.c_size_t]),
     "elsim_timer_start": (I, []),
     "elsim_timer_stop": (I, [C.POINTER(C.c_float)]),

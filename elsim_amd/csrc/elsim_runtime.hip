@@ -141,6 +141,12 @@ int elsim_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return ELSIM_OK;
 }
 
+int elsim_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  ELSIM_TRY(ensure_init());
+  ELSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+  return ELSIM_OK;
+}
+
 int elsim_memset(void* dst, int value, size_t bytes) {
   ELSIM_TRY(ensure_init());
   ELSIM_HIP(hipMemsetAsync(dst, value, bytes, g_stream));

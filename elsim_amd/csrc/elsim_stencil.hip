@@ -136,6 +136,13 @@ __global__ __launch_bounds__(256) void k_matvec(Dims d, const T* __restrict__ p,
       out[OUTG ? c : n] = v;
       acc[0] += (double)pc * (double)v;
     });
+  if (OUTG && ci.j <= d.n2 && (ci.i == 1 || ci.i == d.n1)) {
+    // write the row's two (zero) ghost cells as well, so every cache line of
+    // the row is written whole instead of leaving partial lines at row ends
+    const int64_t sz = d.g12();
+    int64_t c = d.gi(ci.i == 1 ? 0 : d.n1 + 1, ci.j, ci.k0);
+    for (int k = ci.k0; k <= ci.k1; k++, c += sz) out[c] = T(0);
+  }
   if (dot || cg) block_partials<1>(acc, partials);
 }
 

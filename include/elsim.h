@@ -137,6 +137,7 @@ int         elsim_malloc(void** ptr, size_t bytes);
 int         elsim_free(void* ptr);
 int         elsim_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int         elsim_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int         elsim_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int         elsim_memset(void* dst, int value, size_t bytes);
 /* Event timer on the library's stream: start, stop -> milliseconds. */
 int         elsim_timer_start(void);
