@@ -4,16 +4,18 @@ One step = CH update -> CnE solve -> outer {Butler-Volmer -> phP solve -> phE
 solve} -> current control (SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90:397-840),
 every CG to relative residual 1e-8.  Two paths run on the same synthetic
 microstructure from the same initial state:
-  fp64  - the reference CG (unpreconditioned, FP64);
+  fp64  - FP64 Jacobi-preconditioned CG (the same preconditioner, all FP64);
   fast  - FP32 Jacobi-PCG with FP64 reliable updates, plus the learned warm
-          start when a trained network is available.
+          start when a trained network is available;
+  ref   - the reference CG (unpreconditioned FP64): cost per iteration only,
+          since its solves do not reach 1e-8 within the reference's caps.
 `value` is the fast path's seconds per step (lower is better).  Inputs are
 resident in HBM before the timed region; W warm-up steps run first, then K
 timed steps bracketed by device synchronisation.
 
-Also reported: the live roofline of the dominant kernel (the FP32 7-point
-operator, timed with HIP events on the library stream) and the FP64 CPU
-oracle timed on a bounded sample on this host.
+Also reported: the roofline of the dominant CG kernel, timed with HIP events
+on the library stream inside the CG loop of one extra (untimed) step, and the
+FP64 CPU oracle timed on a bounded sample on this host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
 """
@@ -44,9 +46,11 @@ def make_case(n, seed):
     return sdf, ns, frac, target
 
 
-def matvec_roofline(n, reps=50):
-    """Average duration of the FP32 potential operator at n^3 (HIP events on
-    the library stream) and its algorithmic bandwidth: read p and kappa, write
+def matvec_roofline(n, reps=20):
+    """The FP32 potential operator alone at n^3, each launch timed with HIP
+    events on the library stream after a 1 GiB device copy has evicted its
+    operands from the 256 MB last-level cache (as inside the CG loop, whose
+    vectors do not fit there).  Algorithmic bytes: read p and kappa, write
     Ap = 12 B per voxel."""
     import elsim_amd as E
     L = E.lib()
@@ -55,17 +59,47 @@ def matvec_roofline(n, reps=50):
     p = E.DeviceArray.from_host(rng.standard_normal(G).astype(np.float32))
     k = E.DeviceArray.from_host(rng.uniform(1e-7, 1.0, G).astype(np.float32))
     out = E.DeviceArray(n ** 3, np.float32)
-    for _ in range(5):
+    nb = 1 << 30
+    fa, fb = E.DeviceArray(nb, np.uint8), E.DeviceArray(nb, np.uint8)
+    E.check(L.elsim_memset(fa.ptr, 0, nb))
+    for _ in range(3):
         E.check(L.elsim_dev_matvec_f32(E.OP_PHP, n, n, n, p.ptr, k.ptr, None, C.c_float(1.0), out.ptr, None))
-    E.check(L.elsim_synchronize())
-    E.check(L.elsim_timer_start())
-    for _ in range(reps):
-        L.elsim_dev_matvec_f32(E.OP_PHP, n, n, n, p.ptr, k.ptr, None, C.c_float(1.0), out.ptr, None)
+    tot = 0.0
     ms = C.c_float()
-    E.check(L.elsim_timer_stop(C.byref(ms)))
-    t = ms.value * 1e-3 / reps
-    bytes_alg = 12.0 * n ** 3
-    return t, bytes_alg / t / 1e9
+    for _ in range(reps):
+        E.check(L.elsim_memcpy_d2d(fb.ptr, fa.ptr, nb))
+        E.check(L.elsim_timer_start())
+        E.check(L.elsim_dev_matvec_f32(E.OP_PHP, n, n, n, p.ptr, k.ptr, None, C.c_float(1.0), out.ptr, None))
+        E.check(L.elsim_timer_stop(C.byref(ms)))
+        tot += ms.value
+    t = tot * 1e-3 / reps
+    return t, 12.0 * n ** 3 / t / 1e9
+
+
+# Algorithmic bytes per interior voxel of the CG kernels (operands read and
+# written once).  Operator: p, kappa in, Ap out (the CnE operator also reads
+# psE: 4/8 B more).  Update: y/x, p, r, w, dinv in; y/x, r out.  Direction:
+# r, p, dinv in; p out.
+KERNEL_BYTES = {"op32": 12, "upd32": 28, "dir32": 16, "op64": 24, "upd64": 56, "dir64": 32}
+KERNEL_NAMES = {"op32": "k_matvec<float> (7-point operator + p.Ap)", "upd32": "k_mx_update (y += a p, r -= a w, r.z)",
+                "dir32": "k_mx_dir (p = z + b p)", "op64": "k_matvec<double>", "upd64": "k_pcg64_update",
+                "dir64": "k_pcg64_dir"}
+
+
+def profiled_step(E, s, mode):
+    """One more step with event brackets around the first CG iteration of
+    every batch: mean in-loop duration of the operator, update and direction
+    kernels."""
+    L = E.lib()
+    P = E._lib
+    E.check(L.elsim_prof_enable(1))
+    prof = P.KernelProf()
+    E.check(L.elsim_prof_read(C.byref(prof)))  # reset
+    s.step(mode)
+    E.check(L.elsim_prof_read(C.byref(prof)))
+    E.check(L.elsim_prof_enable(0))
+    keys = ["op32", "upd32", "dir32", "op64", "upd64", "dir64"]
+    return {k: {"us": prof.mean_us(i), "launches": int(prof.count[i])} for i, k in enumerate(keys) if prof.count[i]}
 
 
 def cpu_baseline(sdf, prm, tab, iters_per_step, budget_iters=12):
@@ -110,7 +144,7 @@ def reference_cg_cost(E, sdf, prm, tab, cnp0, cap=400):
             "relres_phe_after_cap": st.relres_phe, "cap_per_solve": cap}
 
 
-def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0):
+def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0, profile=True):
     from elsim_amd._lib import F_CNP
     s = E.Sbm(sdf, -sdf, prm, tab)
     s.set(F_CNP, cnp0)
@@ -126,8 +160,11 @@ def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0):
         log(f"  step {k}: {st.as_dict()}")
     E.check(E.lib().elsim_synchronize())
     t = time.perf_counter() - t0
+    kern = profiled_step(E, s, mode) if profile else {}
+    if kern:
+        log("  in-loop kernels: " + ", ".join(f"{k} {v['us']:.1f} us" for k, v in kern.items()))
     s.close()
-    return t / steps, stats
+    return t / steps, stats, kern
 
 
 def main():
@@ -172,19 +209,36 @@ def main():
         prm.inner_rtol = a.inner_rtol
     paths = a.paths.split(",")
     nan = float("nan")
-    tfast, stf, t64, st64, ref_it = nan, [], nan, [], None
+    tfast, stf, t64, st64, ref_it, kf, k64 = nan, [], nan, [], None, {}, {}
     if "fast" in paths:
         log("fast path (mixed precision)")
-        tfast, stf = run_path(E, sdf, prm, tab, E.MODE_MIXED, a.warmup, a.steps, cnp0)
+        tfast, stf, kf = run_path(E, sdf, prm, tab, E.MODE_MIXED, a.warmup, a.steps, cnp0)
     if "fp64" in paths:
         log("fp64 path (Jacobi-PCG, all FP64)")
-        t64, st64 = run_path(E, sdf, prm, tab, E.MODE_FP64_PCG, a.warmup, a.steps, cnp0)
+        t64, st64, k64 = run_path(E, sdf, prm, tab, E.MODE_FP64_PCG, a.warmup, a.steps, cnp0)
     if "ref" in paths:
         log("reference CG (unpreconditioned FP64): cost per iteration on capped solves")
         ref_it = reference_cg_cost(E, sdf, prm, tab, cnp0)
 
     t_mv, gbs = matvec_roofline(n)
-    log(f"matvec f32 {n}^3: {t_mv * 1e6:.1f} us, {gbs:.0f} GB/s")
+    log(f"matvec f32 {n}^3 standalone, cold cache: {t_mv * 1e6:.1f} us, {gbs:.0f} GB/s")
+    N = n ** 3
+    kernels = {}
+    for k, v in list(kf.items()) + list(k64.items()):
+        kernels[k] = {"kernel": KERNEL_NAMES[k], "us": v["us"], "bytes_per_voxel": KERNEL_BYTES[k],
+                      "GBps": KERNEL_BYTES[k] * N / (v["us"] * 1e-6) / 1e9, "launches_timed": v["launches"]}
+    # dominant kernel of the fast path: largest share of the CG iteration
+    dom = max(kf, key=lambda k: kf[k]["us"]) if kf else None
+    if dom:
+        roof = {"bound": "hbm", "achieved": kernels[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": kernels[dom]["GBps"] / HBM_PEAK_GBS, "traffic": None, "kernel": kernels[dom]["kernel"],
+                "kernel_us": kernels[dom]["us"], "bytes_per_launch": KERNEL_BYTES[dom] * N,
+                "stencil": {"kernel": kernels["op32"]["kernel"], "us": kernels["op32"]["us"],
+                            "GBps": kernels["op32"]["GBps"], "frac": kernels["op32"]["GBps"] / HBM_PEAK_GBS,
+                            "standalone_cold_us": t_mv * 1e6}}
+    else:
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                "traffic": None, "kernel": "k_matvec<float, phP> standalone, cold cache", "kernel_us": t_mv * 1e6}
 
     iters64 = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in st64])) if st64 else nan
     itersf = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in stf])) if stf else nan
@@ -194,7 +248,6 @@ def main():
         cpu = {"value": v, "unit": "s/step", "cores": 1, "kind": "port",
                "sample": f"{k} FP64 CG iterations of the phP system at {n}^3 on the oracle ({t:.1f} s, one thread), "
                          f"scaled by the FP64 path's {iters64:.0f} CG iterations per step"}
-    N = n ** 3
     out = {
         "metric": "s/electrochem step @256³: FP64 CG vs FP32-refined+NN warm start, rel res 1e-8",
         "value": tfast,
@@ -217,9 +270,8 @@ def main():
         "reference_cg": ref_it,
         "cg_iters_per_step": {"fp64": iters64, "fast": itersf},
         "voxel_updates_per_s": {"fp64": N * iters64 / t64, "fast": N * itersf / tfast},
-        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_matvec<float, phP> (7-point operator, FP32)",
-                     "kernel_us": t_mv * 1e6},
+        "roofline": roof,
+        "kernels_in_loop": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -55,6 +55,17 @@ class StepStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+PROF_OP32, PROF_UPD32, PROF_DIR32, PROF_OP64, PROF_UPD64, PROF_DIR64 = range(6)
+
+
+class KernelProf(C.Structure):
+    """Mirror of elsim_kernel_prof."""
+    _fields_ = [("count", C.c_int32 * 6), ("ms_total", C.c_double * 6)]
+
+    def mean_us(self, slot):
+        return self.ms_total[slot] * 1e3 / self.count[slot] if self.count[slot] else float("nan")
+
+
 _lib = None
 
 P = C.c_void_p
@@ -99,6 +110,9 @@ _SIGS = {
     "elsim_sbm_ch_update": (I, [C.POINTER(C.c_int64)]),
     "elsim_sbm_matvec": (I, [C.POINTER(C.c_int64), IP, P, P]),
     "elsim_dev_matvec_f64": (I, [I, I, I, I, P, P, P, D, P, P]),
+    "elsim_dev_matvec_stored_f64": (I, [I, I, I, I, P, P, P, D, P]),
+    "elsim_prof_enable": (I, [I]),
+    "elsim_prof_read": (I, [C.POINTER(KernelProf)]),
     "elsim_dev_matvec_f32": (I, [I, I, I, I, P, P, P, C.c_float, P, P]),
     "elsim_gen_sphere_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, I, D, P, IP, DP]),
     "elsim_gen_ellipsoid_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, D, P, IP, DP]),

@@ -208,7 +208,7 @@ void elsim_sbm_default_params(elsim_sbm_params* p) {
   p->crt_p = 1.0e-7;
   p->crt_l = 1.0e-6;
   p->outer_tol = 1.0e-8;
-  p->inner_rtol = 0.1;
+  p->inner_rtol = 0.5;  // an FP64 reliable update each time the FP32 residual halves
   p->mass_corr_period = 100;
   p->skip_cne_first = 1;
   p->stop_rule = ELSIM_STOP_REFERENCE;

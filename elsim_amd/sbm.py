@@ -36,7 +36,7 @@ def synthetic_params(**kw):
         ch_psi_min=1.0e-5, frd=1.0, cst1=1.0, t_minus=tm, d0=1.0, tc1=(2 * tm - 1.0) / (2 * tm * (1.0 - tm)),
         tc2=1.0 / (2 * tm * (1.0 - tm)), dab_c0=0.0, dab_c1=0.2, dab_c2=0.0, dab_c3=0.0, kap_scale=1.0,
         i0_scale=5.0e-6, kfw_cref=1.0, bvp=0.0, bve=-0.5, vsr=0.1, trgi=0.0, cne_target=1.0, rtol=1.0e-8,
-        crt_e=1.0e-7, crt_p=1.0e-7, crt_l=1.0e-6, outer_tol=1.0e-8, inner_rtol=0.1, mass_corr_period=100,
+        crt_e=1.0e-7, crt_p=1.0e-7, crt_l=1.0e-6, outer_tol=1.0e-8, inner_rtol=0.5, mass_corr_period=100,
         skip_cne_first=1, stop_rule=STOP_RELRES, max_it_e=50000, max_it_p=30000, max_it_l=100000, max_outer=30)
     for k, v in kw.items():
         setattr(p, k, v)

@@ -247,6 +247,25 @@ int elsim_dev_matvec_f64(int op, int n1, int n2, int n3, const double* p, const 
                          const double* pse, double scale, double* out, double* dot);
 int elsim_dev_matvec_f32(int op, int n1, int n2, int n3, const float* p, const float* kap,
                          const float* pse, float scale, float* out, double* dot);
+/* Kernel timing inside the CG loops.  When enabled, the first iteration of
+ * every batch of CG iterations is bracketed by HIP events on the library
+ * stream (operator, update, direction kernels); read-out averages them over
+ * the batches whose iteration ran, then resets.  Slots: ELSIM_PROF_*. */
+enum {
+  ELSIM_PROF_OP32 = 0, ELSIM_PROF_UPD32 = 1, ELSIM_PROF_DIR32 = 2,
+  ELSIM_PROF_OP64 = 3, ELSIM_PROF_UPD64 = 4, ELSIM_PROF_DIR64 = 5, ELSIM_PROF_N = 6
+};
+typedef struct elsim_kernel_prof {
+  int32_t count[ELSIM_PROF_N];
+  double  ms_total[ELSIM_PROF_N];
+} elsim_kernel_prof;
+int elsim_prof_enable(int on);
+int elsim_prof_read(elsim_kernel_prof* out);
+
+/* The same operator with every ghost of p read as stored (the form of the
+ * initial residual b - A x, SBM:492-499, 649-655, 729-735).  out: interior. */
+int elsim_dev_matvec_stored_f64(int op, int n1, int n2, int n3, const double* p, const double* kap,
+                                const double* pse, double scale, double* out);
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic inputs (host only; deterministic for a seed)                   */

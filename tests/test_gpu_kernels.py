@@ -21,8 +21,13 @@ def rand_fields(shape, seed):
     return p, kap, pse
 
 
+# shapes cover partial waves, wave and tile edges in d1 (64, 128, 256 < 300),
+# partial row blocks in d2 and partial plane blocks in d3
+SHAPES = [(7, 5, 6), (64, 9, 17), (70, 33, 20), (128, 6, 18), (300, 7, 5)]
+
+
 @pytest.mark.parametrize("op", [E.OP_CNE, E.OP_PHP, E.OP_PHE])
-@pytest.mark.parametrize("shape", [(7, 5, 6), (64, 9, 17), (70, 33, 20)])
+@pytest.mark.parametrize("shape", SHAPES)
 def test_matvec_f64_bit_exact(gpu, oracle_lib, op, shape):
     O = oracle_lib
     p, kap, pse = rand_fields(shape, 1 + op)
@@ -37,6 +42,22 @@ def test_matvec_f64_bit_exact(gpu, oracle_lib, op, shape):
     assert np.array_equal(got, ref), np.abs(got - ref).max()
     pin = O.prj_bc(op, p)[1:-1, 1:-1, 1:-1]
     assert dot.to_host()[0] == pytest.approx(float(np.sum(pin * ref)), rel=1e-12)
+
+
+@pytest.mark.parametrize("op", [E.OP_CNE, E.OP_PHP, E.OP_PHE])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_matvec_stored_ghosts_bit_exact(gpu, oracle_lib, op, shape):
+    """The operator of the initial residual reads every ghost of p as stored."""
+    O = oracle_lib
+    p, kap, pse = rand_fields(shape, 11 + op)
+    dt, dx = 1e-3, 0.7
+    scale = dt / (dx * dx) if op == E.OP_CNE else 1.0 / (dx * dx)
+    ref = O.apply_op(op, kap, pse, p, dt, dx)
+    dp, dk, ds = (E.DeviceArray.from_host(O.flat(a)) for a in (p, kap, pse))
+    out = E.DeviceArray(int(np.prod(shape)), np.float64)
+    E.check(gpu.elsim_dev_matvec_stored_f64(op, *shape, dp.ptr, dk.ptr, ds.ptr, scale, out.ptr))
+    got = O.unflat(out.to_host(), shape)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
 
 
 @pytest.mark.parametrize("op", [E.OP_CNE, E.OP_PHP, E.OP_PHE])
