@@ -49,7 +49,9 @@ class SbmParams(C.Structure):
 class StepStats(C.Structure):
     """Mirror of elsim_step_stats."""
     _fields_ = [(n, C.c_int32) for n in "it_cne it_php it_phe n_outer n_reliable status".split()] + \
-               [(n, C.c_double) for n in "xrf scrnt cvgp cvgl relres_cne relres_php relres_phe seconds".split()]
+               [(n, C.c_double) for n in "xrf scrnt cvgp cvgl relres_cne relres_php relres_phe seconds".split()] + \
+               [("ws_used", C.c_int32), ("ws_pad_", C.c_int32), ("ws_res_plain", C.c_double),
+                ("ws_res_nn", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -112,6 +114,20 @@ _SIGS = {
     "elsim_dev_matvec_f64": (I, [I, I, I, I, P, P, P, D, P, P]),
     "elsim_dev_matvec_stored_f64": (I, [I, I, I, I, P, P, P, D, P]),
     "elsim_prof_enable": (I, [I]),
+    "elsim_nn_param_count": (I, []),
+    "elsim_nn_create": (I, [C.POINTER(C.c_int64)]),
+    "elsim_nn_destroy": (I, [C.POINTER(C.c_int64)]),
+    "elsim_nn_set_params": (I, [C.POINTER(C.c_int64), P]),
+    "elsim_nn_get_params": (I, [C.POINTER(C.c_int64), P]),
+    "elsim_nn_get_grads": (I, [C.POINTER(C.c_int64), P]),
+    "elsim_nn_forward": (I, [C.POINTER(C.c_int64), I, I, I, P, P]),
+    "elsim_nn_loss_grad": (I, [C.POINTER(C.c_int64), I, I, I, P, P, P, DP]),
+    "elsim_nn_adam": (I, [C.POINTER(C.c_int64), C.c_float]),
+    "elsim_nn_conv3": (I, [I, I, I, P, I, P, P, I, I, P]),
+    "elsim_sbm_set_warm_start": (I, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "elsim_sbm_nn_features": (I, [C.POINTER(C.c_int64), P, DP]),
+    "elsim_nn_conv3_bwd_data": (I, [I, I, I, P, I, P, I, P]),
+    "elsim_nn_conv3_bwd_weight": (I, [I, I, I, P, I, P, I, P]),
     "elsim_prof_read": (I, [C.POINTER(KernelProf)]),
     "elsim_dev_matvec_f32": (I, [I, I, I, I, P, P, P, C.c_float, P, P]),
     "elsim_gen_sphere_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, I, D, P, IP, DP]),

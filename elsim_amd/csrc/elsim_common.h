@@ -45,7 +45,7 @@ int ensure_init();
 // Device scratch owned by the library: reduction partials and counters.
 // Kernels of one stream run in order, so one partials buffer is shared.
 constexpr int kMaxRedBlocks = 262144;
-constexpr int kMaxRedVals = 4;
+constexpr int kMaxRedVals = 8;
 struct Workspace {
   double* partials = nullptr;   // kMaxRedVals * kMaxRedBlocks
   unsigned* counter = nullptr;  // 1

@@ -1,0 +1,29 @@
+// elsim_nn.h -- interface of the learned warm start shared by the SBM step
+// driver: the network object and its inference call.
+#pragma once
+
+#include <cstdint>
+
+#include "elsim_common.h"
+
+namespace elsim {
+
+constexpr int NN_NLAYERS = 7;  // e0 e1 e2 e3 d2 d1 d0 (3x3x3 convolutions)
+constexpr int NN_IN_CH = 16;   // stored input channels (features zero-padded)
+
+// bfloat16 <-> float, round to nearest even.
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+struct NnNet;
+int64_t nn_param_count();
+// pred[v] (FP32) for the input features x (bf16 [voxel][16]); grid dims
+// multiples of 8.
+int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred);
+
+}  // namespace elsim

@@ -1,0 +1,780 @@
+// elsim_nn.hip -- the learned CG warm start: a 4-level 3-D convolutional
+// encoder-decoder (channels 16/32/64/64, 3x3x3 "same" convolutions, 2x2x2
+// average pooling, nearest upsampling, concatenated skips, ReLU, 1x1x1
+// output), trained and run on the GPU by the kernels of this file.  It has
+// no counterpart in the reference.
+//
+// Storage: activations, convolution weights and activation gradients in
+// bfloat16, channels-last ([voxel][channel], voxel = i + n1 (j + n2 k)), all
+// accumulation in FP32 on the MFMA units (v_mfma_f32_16x16x16_bf16).
+// Master weights, weight gradients and Adam moments are FP32.
+//
+// Convolution as implicit GEMM: D[voxel, co] = sum_K A[voxel, K] B[K, co]
+// with K = tap * cin + ci.  A block owns a 16 x 4 x MT voxel tile; the tile
+// plus its one-voxel halo is staged in LDS 16 or 32 channels at a time, so
+// each input value is read from HBM/L2 once per block and 27 times from LDS.
+// Every wave owns MT 16-voxel rows of the tile and all output channels.
+// The backward-data pass is the same kernel with the taps flipped and the
+// channel roles exchanged (stride 1, zero padding), writing FP32.
+//
+// Channel counts are multiples of 16 (the input features are padded to 16).
+#include <cmath>
+#include <algorithm>
+#include <vector>
+
+#include "elsim_common.h"
+#include "elsim_nn.h"
+
+namespace elsim {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// ----------------------------------------------------------- convolution --
+
+constexpr int CONV_TX = 16, CONV_TY = 4;  // voxel tile in d1, d2 (d3: MT)
+constexpr int LDS_PAD = 8;                // bf16 elements of padding per staged voxel
+constexpr int CS_MAX = 32;                // channels staged per slab
+
+struct View {
+  int cs, co;  // channel stride and offset of a channel slice
+};
+
+// MODE 0: out bf16 = relu?(acc + bias); MODE 1: out FP32 = acc.
+template <int COUT, int MT, int MODE>
+__global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
+                                               View iv, const uint16_t* __restrict__ wt,
+                                               const float* __restrict__ bias, int relu, void* __restrict__ out,
+                                               View ov) {
+  constexpr int NT = COUT / 16;
+  constexpr int HX = CONV_TX + 2, HY = CONV_TY + 2, HZ = MT + 2, HN = HX * HY * HZ;
+  __shared__ __align__(16) uint16_t lds[HN * (CS_MAX + LDS_PAD)];
+  const int tx = (n1 + CONV_TX - 1) / CONV_TX, ty = (n2 + CONV_TY - 1) / CONV_TY;
+  const int b = blockIdx.x;
+  const int x0 = (b % tx) * CONV_TX, y0 = ((b / tx) % ty) * CONV_TY, z0 = (b / (tx * ty)) * MT;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int cs = (cin % 32 == 0) ? 32 : 16;
+  const int rs = cs + LDS_PAD;  // LDS row stride (elements)
+  const int K = 27 * cin;
+  f4v acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; m++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[m][t] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  for (int slab = 0; slab < cin; slab += cs) {
+    __syncthreads();
+    const int nq = cs / 8;  // 16-byte chunks per staged voxel
+    for (int it = tid; it < HN * nq; it += 256) {
+      const int h = it / nq, q = it % nq;
+      const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3) {
+        const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
+        v = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
+      }
+      *(uint4*)(lds + h * rs + 8 * q) = v;
+    }
+    __syncthreads();
+    for (int tap = 0; tap < 27; tap++) {
+      const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;  // halo offsets 0..2 (= -1..1 + 1)
+      for (int cc = 0; cc < cs; cc += 16) {
+        s4v a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+          const int row = wave * MT + m, y = row % CONV_TY, z = row / CONV_TY;
+          const int h = ((z + dz) * HY + (y + dy)) * HX + (lr + dx);
+          a[m] = *(const s4v*)(lds + h * rs + cc + 4 * lq);
+        }
+        const int k = tap * cin + slab + cc + 4 * lq;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const s4v bw = *(const s4v*)(wt + (int64_t)(t * 16 + lr) * K + k);
+#pragma unroll
+          for (int m = 0; m < MT; m++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bw, acc[m][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D[row = 4 lq + i][col = lr]: voxel x = 4 lq + i of the wave's row, channel t*16 + lr
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int row = wave * MT + m, y = y0 + row % CONV_TY, z = z0 + row / CONV_TY;
+    if (y >= n2 || z >= n3) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int x = x0 + 4 * lq + i;
+      if (x >= n1) continue;
+      const int64_t vox = x + (int64_t)n1 * (y + (int64_t)n2 * z);
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const int co = t * 16 + lr;
+        if (MODE == 0) {
+          float v = acc[m][t][i] + bias[co];
+          if (relu) v = fmaxf(v, 0.f);
+          ((uint16_t*)out)[vox * ov.cs + ov.co + co] = f2bf(v);
+        } else {
+          ((float*)out)[vox * ov.cs + ov.co + co] = acc[m][t][i];
+        }
+      }
+    }
+  }
+}
+
+template <int MODE, int COUT, int MT>
+static void conv_launch(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, const uint16_t* w,
+                        const float* bias, int relu, void* out, View ov, hipStream_t s) {
+  const int nb = ((n1 + CONV_TX - 1) / CONV_TX) * ((n2 + CONV_TY - 1) / CONV_TY) * ((n3 + MT - 1) / MT);
+  k_conv3<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov);
+}
+
+template <int MODE>
+static int conv(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, const uint16_t* w, const float* bias,
+                int relu, int cout, void* out, View ov, hipStream_t s) {
+  if (cin % 16 || cin <= 0 || iv.cs % 8 || iv.co % 8) {
+    set_error("convolution: input channels %d (stride %d, offset %d) must be multiples of 16 (8)", cin, iv.cs, iv.co);
+    return ELSIM_ERR_ARG;
+  }
+  switch (cout) {
+    case 16: conv_launch<MODE, 16, 4>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
+    case 32: conv_launch<MODE, 32, 4>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
+    case 48: conv_launch<MODE, 48, 4>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
+    case 64: conv_launch<MODE, 64, 4>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
+    case 96: conv_launch<MODE, 96, 2>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
+    case 128: conv_launch<MODE, 128, 2>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
+    default: set_error("convolution: %d output channels not supported", cout); return ELSIM_ERR_ARG;
+  }
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+
+// ------------------------------------------------------- weight gradient --
+
+constexpr int WG_VOX = 4096;  // voxels per block of the weight-gradient kernel
+constexpr int WG_MAXP = 8;    // (co tile, ci tile) pairs per wave
+
+// part[g][tap][co][ci] = sum_{v in group g} dz[v][co] x[v + o_tap][ci]
+// (block: one group, one tap; MFMA with the voxels as the reduction index).
+__global__ __launch_bounds__(256) void k_wgrad(int n1, int n2, int n3, const uint16_t* __restrict__ x, int cin,
+                                               View xv, const uint16_t* __restrict__ dz, int cout,
+                                               float* __restrict__ part) {
+  const int g = blockIdx.x, tap = blockIdx.y;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+  const int ox = tap % 3 - 1, oy = (tap / 3) % 3 - 1, oz = tap / 9 - 1;
+  const int nct = cout / 16, nkt = cin / 16, np = nct * nkt;
+  const int64_t N = (int64_t)n1 * n2 * n3;
+  f4v acc[WG_MAXP];
+#pragma unroll
+  for (int q = 0; q < WG_MAXP; q++) acc[q] = f4v{0.f, 0.f, 0.f, 0.f};
+  const int64_t vbeg = (int64_t)g * WG_VOX;
+  for (int kc = 0; kc < WG_VOX; kc += 16) {
+    int64_t vv[4], nb[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int64_t v = vbeg + kc + 4 * lq + i;
+      vv[i] = v < N ? v : -1;
+      nb[i] = -1;
+      if (v < N) {
+        const int xi = (int)(v % n1), yi = (int)((v / n1) % n2), zi = (int)(v / ((int64_t)n1 * n2));
+        const int xa = xi + ox, ya = yi + oy, za = zi + oz;
+        if (xa >= 0 && xa < n1 && ya >= 0 && ya < n2 && za >= 0 && za < n3)
+          nb[i] = xa + (int64_t)n1 * (ya + (int64_t)n2 * za);
+      }
+    }
+    if (vbeg + kc >= N) break;
+#pragma unroll
+    for (int q = 0; q < WG_MAXP; q++) {
+      const int p = wave + 4 * q;
+      if (p >= np) break;
+      const int ct = p / nkt, kt = p % nkt;
+      s4v a, bb;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        a[i] = vv[i] >= 0 ? (short)dz[vv[i] * cout + ct * 16 + lr] : (short)0;
+        bb[i] = nb[i] >= 0 ? (short)x[nb[i] * xv.cs + xv.co + kt * 16 + lr] : (short)0;
+      }
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[q], 0, 0, 0);
+    }
+  }
+  // D[row = co 4 lq + i][col = ci lr]
+#pragma unroll
+  for (int q = 0; q < WG_MAXP; q++) {
+    const int p = wave + 4 * q;
+    if (p >= np) break;
+    const int ct = p / nkt, kt = p % nkt;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int co = ct * 16 + 4 * lq + i, ci = kt * 16 + lr;
+      part[(((int64_t)g * 27 + tap) * cout + co) * cin + ci] = acc[q][i];
+    }
+  }
+}
+
+// dw[co][tap][ci] = sum_g part[g][tap][co][ci] (fixed order).
+__global__ void k_wgrad_reduce(int ng, int cin, int cout, const float* __restrict__ part, float* __restrict__ dw) {
+  const int64_t n = (int64_t)27 * cout * cin;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int ci = (int)(e % cin), tap = (int)((e / cin) % 27), co = (int)(e / ((int64_t)27 * cin));
+  const int64_t pe = ((int64_t)tap * cout + co) * cin + ci;
+  float s = 0.f;
+  for (int g = 0; g < ng; g++) s += part[(int64_t)g * n + pe];
+  dw[e] = s;
+}
+
+// ---------------------------------------------- column sums (bias grads) --
+
+constexpr int CSUM_VOX = 4096;
+
+// part[g][c] = sum over the group's voxels of a[v][c] (bf16), c < C <= 128.
+__global__ __launch_bounds__(256) void k_colsum(int64_t N, int C, const uint16_t* __restrict__ a,
+                                                float* __restrict__ part) {
+  __shared__ float sm[256];
+  const int t = threadIdx.x, S = 256 / C;
+  const int c = t % C, s = t / C;
+  float acc = 0.f;
+  const int64_t v0 = (int64_t)blockIdx.x * CSUM_VOX;
+  if (s < S)
+    for (int64_t v = v0 + s; v < v0 + CSUM_VOX && v < N; v += S) acc += bf2f(a[v * C + c]);
+  sm[t] = acc;
+  __syncthreads();
+  if (t < C) {
+    float tot = 0.f;
+    for (int q = 0; q < S; q++) tot += sm[q * C + t];
+    part[(int64_t)blockIdx.x * C + t] = tot;
+  }
+}
+
+__global__ void k_colsum_reduce(int ng, int C, const float* __restrict__ part, float* __restrict__ out) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f;
+  for (int g = 0; g < ng; g++) s += part[(int64_t)g * C + c];
+  out[c] = s;
+}
+
+// ------------------------------------------------- pooling and upsampling --
+
+// coarse[v][c] = bf16(mean of the 8 children), children from a channel view.
+__global__ void k_pool(int n1, int n2, int n3, int C, const uint16_t* __restrict__ fine, View fv,
+                       uint16_t* __restrict__ coarse) {
+  const int m1 = n1 / 2, m2 = n2 / 2, m3 = n3 / 2;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)m1 * m2 * m3 * C;
+  if (e >= n) return;
+  const int c = (int)(e % C);
+  const int64_t vc = e / C;
+  const int x = (int)(vc % m1), y = (int)((vc / m1) % m2), z = (int)(vc / ((int64_t)m1 * m2));
+  float s = 0.f;
+  for (int q = 0; q < 8; q++) {
+    const int fx = 2 * x + (q & 1), fy = 2 * y + ((q >> 1) & 1), fz = 2 * z + (q >> 2);
+    const int64_t vf = fx + (int64_t)n1 * (fy + (int64_t)n2 * fz);
+    s += bf2f(fine[vf * fv.cs + fv.co + c]);
+  }
+  coarse[vc * C + c] = f2bf(s * 0.125f);
+}
+
+// dfine[child] += dcoarse / 8 (FP32, into a channel view).
+__global__ void k_pool_bwd(int n1, int n2, int n3, int C, const float* __restrict__ dcoarse, float* __restrict__ dfine,
+                           View fv) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)n1 * n2 * n3 * C;
+  if (e >= n) return;
+  const int c = (int)(e % C);
+  const int64_t vf = e / C;
+  const int x = (int)(vf % n1), y = (int)((vf / n1) % n2), z = (int)(vf / ((int64_t)n1 * n2));
+  const int m1 = n1 / 2, m2 = n2 / 2;
+  const int64_t vc = x / 2 + (int64_t)m1 * (y / 2 + (int64_t)m2 * (z / 2));
+  dfine[vf * fv.cs + fv.co + c] += dcoarse[vc * C + c] * 0.125f;
+}
+
+// fine view [v][c] = coarse[parent][c] (bf16 copy); n* are the fine dims.
+__global__ void k_up(int n1, int n2, int n3, int C, const uint16_t* __restrict__ coarse, uint16_t* __restrict__ fine,
+                     View fv) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)n1 * n2 * n3 * C;
+  if (e >= n) return;
+  const int c = (int)(e % C);
+  const int64_t vf = e / C;
+  const int x = (int)(vf % n1), y = (int)((vf / n1) % n2), z = (int)(vf / ((int64_t)n1 * n2));
+  const int m1 = n1 / 2, m2 = n2 / 2;
+  const int64_t vc = x / 2 + (int64_t)m1 * (y / 2 + (int64_t)m2 * (z / 2));
+  fine[vf * fv.cs + fv.co + c] = coarse[vc * C + c];
+}
+
+// dcoarse[v][c] = sum of the 8 children of a fine FP32 view (fixed order).
+__global__ void k_up_bwd(int n1, int n2, int n3, int C, const float* __restrict__ dfine, View fv,
+                         float* __restrict__ dcoarse) {
+  const int m1 = n1 / 2, m2 = n2 / 2, m3 = n3 / 2;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)m1 * m2 * m3 * C;
+  if (e >= n) return;
+  const int c = (int)(e % C);
+  const int64_t vc = e / C;
+  const int x = (int)(vc % m1), y = (int)((vc / m1) % m2), z = (int)(vc / ((int64_t)m1 * m2));
+  float s = 0.f;
+  for (int q = 0; q < 8; q++) {
+    const int fx = 2 * x + (q & 1), fy = 2 * y + ((q >> 1) & 1), fz = 2 * z + (q >> 2);
+    const int64_t vf = fx + (int64_t)n1 * (fy + (int64_t)n2 * fz);
+    s += dfine[vf * fv.cs + fv.co + c];
+  }
+  dcoarse[vc * C + c] = s;
+}
+
+// dz[v][c] = bf16(dact[v][c] * (act[v][c] > 0)).
+__global__ void k_relu_bwd(int64_t N, int C, const float* __restrict__ dact, View dv, const uint16_t* __restrict__ act,
+                           View av, uint16_t* __restrict__ dz) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * C) return;
+  const int c = (int)(e % C);
+  const int64_t v = e / C;
+  const uint16_t a = act[v * av.cs + av.co + c];
+  const bool pos = a != 0 && !(a & 0x8000u);
+  dz[e] = f2bf(pos ? dact[v * dv.cs + dv.co + c] : 0.f);
+}
+
+// ----------------------------------------------------------- output layer --
+
+// pred = d0 . w + b; with a target also dpred = 2 (pred - t) / N and the
+// per-block partial sum of (pred - t)^2.
+__global__ __launch_bounds__(256) void k_out_fwd(int64_t N, const uint16_t* __restrict__ d0, const float* __restrict__ w,
+                                                 const float* __restrict__ target, float* __restrict__ pred,
+                                                 float* __restrict__ dpred, double* partials) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (v < N) {
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 16; c++) s += w[c] * bf2f(d0[v * 16 + c]);
+    s += w[16];
+    pred[v] = s;
+    if (target) {
+      const float r = s - target[v];
+      dpred[v] = 2.f * r / (float)N;
+      acc[0] = (double)r * r;
+    }
+  }
+  if (target) block_partials<1>(acc, partials);
+}
+
+// dd0[v][c] = dpred[v] w[c]; partial sums of dpred d0[v][c] (16) and dpred (1).
+__global__ __launch_bounds__(256) void k_out_bwd(int64_t N, const uint16_t* __restrict__ d0, const float* __restrict__ w,
+                                                 const float* __restrict__ dpred, float* __restrict__ dd0,
+                                                 float* __restrict__ part) {
+  __shared__ float sm[17][256];
+  const int t = threadIdx.x;
+  float acc[17];
+#pragma unroll
+  for (int c = 0; c < 17; c++) acc[c] = 0.f;
+  const int64_t v0 = (int64_t)blockIdx.x * CSUM_VOX;
+  for (int64_t v = v0 + t; v < v0 + CSUM_VOX && v < N; v += 256) {
+    const float g = dpred[v];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      dd0[v * 16 + c] = g * w[c];
+      acc[c] += g * bf2f(d0[v * 16 + c]);
+    }
+    acc[16] += g;
+  }
+#pragma unroll
+  for (int c = 0; c < 17; c++) sm[c][t] = acc[c];
+  __syncthreads();
+  if (t < 17) {
+    float s = 0.f;
+    for (int q = 0; q < 256; q++) s += sm[t][q];
+    part[(int64_t)blockIdx.x * 17 + t] = s;
+  }
+}
+
+// ------------------------------------------------------------- optimiser --
+
+__global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gi = g[e];
+  const float mi = b1 * m[e] + (1.f - b1) * gi;
+  const float vi = b2 * v[e] + (1.f - b2) * gi * gi;
+  m[e] = mi;
+  v[e] = vi;
+  p[e] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copies of one layer's weights: wt[co][tap*cin + ci] = W[co][tap][ci]
+// (forward) and wb[ci][tap*cout + co] = W[co][26 - tap][ci] (backward data).
+__global__ void k_pack(int cin, int cout, const float* __restrict__ W, uint16_t* __restrict__ wt,
+                       uint16_t* __restrict__ wb) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)27 * cin * cout;
+  if (e >= n) return;
+  const int ci = (int)(e % cin), tap = (int)((e / cin) % 27), co = (int)(e / ((int64_t)27 * cin));
+  const uint16_t h = f2bf(W[e]);
+  wt[e] = h;
+  wb[(int64_t)ci * 27 * cout + (int64_t)(26 - tap) * cout + co] = h;
+}
+
+// ------------------------------------------------------------------ host --
+
+namespace {
+
+inline int nblk(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+
+struct LayerDef {
+  int cin, cout;
+};
+constexpr LayerDef kLayers[NN_NLAYERS] = {{16, 16}, {16, 32}, {32, 64}, {64, 64}, {128, 64}, {96, 32}, {48, 16}};
+enum { L_E0 = 0, L_E1, L_E2, L_E3, L_D2, L_D1, L_D0 };
+
+}  // namespace
+
+int64_t nn_param_count() {
+  int64_t n = 0;
+  for (const LayerDef& l : kLayers) n += (int64_t)27 * l.cin * l.cout + l.cout;
+  return n + 17;
+}
+
+struct NnNet {
+  DevBuf params, grads, m, v;  // FP32, nn_param_count()
+  DevBuf wt, wb;               // bf16 packed convolution weights
+  int64_t w_off[NN_NLAYERS], b_off[NN_NLAYERS], wt_off[NN_NLAYERS], out_off = 0;
+  int t = 0;
+  bool packed = false;
+  int n1 = 0, n2 = 0, n3 = 0;
+  // activations (bf16) and gradients (FP32) for the current grid
+  DevBuf cat0, p1, cat1, p2, cat2, p3, e3, d2, d1, d0, dpred;
+  DevBuf dd0, dcat0, dd1, dcat1, dd2, dcat2, de3, dp3, dp2, dp1, dz;
+  DevBuf wpart, cpart, opart;
+  DevBuf loss;  // device double
+
+  NnNet() {
+    int64_t o = 0, ow = 0;
+    for (int l = 0; l < NN_NLAYERS; l++) {
+      w_off[l] = o;
+      o += (int64_t)27 * kLayers[l].cin * kLayers[l].cout;
+      b_off[l] = o;
+      o += kLayers[l].cout;
+      wt_off[l] = ow;
+      ow += (int64_t)27 * kLayers[l].cin * kLayers[l].cout;
+    }
+    out_off = o;
+  }
+  int64_t nw() const { return wt_off[NN_NLAYERS - 1] + (int64_t)27 * kLayers[NN_NLAYERS - 1].cin * kLayers[NN_NLAYERS - 1].cout; }
+  float* P() { return (float*)params.p; }
+  float* Gr() { return (float*)grads.p; }
+  uint16_t* WT(int l) { return (uint16_t*)wt.p + wt_off[l]; }
+  uint16_t* WB(int l) { return (uint16_t*)wb.p + wt_off[l]; }
+
+  int init() {
+    const int64_t n = nn_param_count();
+    for (DevBuf* b : {&params, &grads, &m, &v}) {
+      ELSIM_TRY(dev_alloc(*b, n * sizeof(float)));
+      ELSIM_HIP(hipMemsetAsync(b->p, 0, n * sizeof(float), stream()));
+    }
+    ELSIM_TRY(dev_alloc(wt, nw() * 2));
+    ELSIM_TRY(dev_alloc(wb, nw() * 2));
+    ELSIM_TRY(dev_alloc(loss, sizeof(double)));
+    return ELSIM_OK;
+  }
+  int pack() {
+    if (packed) return ELSIM_OK;
+    for (int l = 0; l < NN_NLAYERS; l++) {
+      const int64_t n = (int64_t)27 * kLayers[l].cin * kLayers[l].cout;
+      k_pack<<<nblk(n), 256, 0, stream()>>>(kLayers[l].cin, kLayers[l].cout, P() + w_off[l], WT(l), WB(l));
+    }
+    ELSIM_CHECK_LAUNCH();
+    packed = true;
+    return ELSIM_OK;
+  }
+  int ensure(int a1, int a2, int a3, bool train) {
+    if (a1 % 8 || a2 % 8 || a3 % 8 || a1 <= 0 || a2 <= 0 || a3 <= 0) {
+      set_error("warm-start network: grid %dx%dx%d must be a positive multiple of 8 in every direction", a1, a2, a3);
+      return ELSIM_ERR_ARG;
+    }
+    const int64_t N0 = (int64_t)a1 * a2 * a3, N1 = N0 / 8, N2 = N1 / 8, N3 = N2 / 8;
+    ELSIM_TRY(dev_alloc(cat0, N0 * 48 * 2));
+    ELSIM_TRY(dev_alloc(p1, N1 * 16 * 2));
+    ELSIM_TRY(dev_alloc(cat1, N1 * 96 * 2));
+    ELSIM_TRY(dev_alloc(p2, N2 * 32 * 2));
+    ELSIM_TRY(dev_alloc(cat2, N2 * 128 * 2));
+    ELSIM_TRY(dev_alloc(p3, N3 * 64 * 2));
+    ELSIM_TRY(dev_alloc(e3, N3 * 64 * 2));
+    ELSIM_TRY(dev_alloc(d2, N2 * 64 * 2));
+    ELSIM_TRY(dev_alloc(d1, N1 * 32 * 2));
+    ELSIM_TRY(dev_alloc(d0, N0 * 16 * 2));
+    if (train) {
+      ELSIM_TRY(dev_alloc(dpred, N0 * 4));
+      ELSIM_TRY(dev_alloc(dd0, N0 * 16 * 4));
+      ELSIM_TRY(dev_alloc(dcat0, N0 * 48 * 4));
+      ELSIM_TRY(dev_alloc(dd1, N1 * 32 * 4));
+      ELSIM_TRY(dev_alloc(dcat1, N1 * 96 * 4));
+      ELSIM_TRY(dev_alloc(dd2, N2 * 64 * 4));
+      ELSIM_TRY(dev_alloc(dcat2, N2 * 128 * 4));
+      ELSIM_TRY(dev_alloc(de3, N3 * 64 * 4));
+      ELSIM_TRY(dev_alloc(dp3, N3 * 64 * 4));
+      ELSIM_TRY(dev_alloc(dp2, N2 * 32 * 4));
+      ELSIM_TRY(dev_alloc(dp1, N1 * 16 * 4));
+      ELSIM_TRY(dev_alloc(dz, N0 * 16 * 2));
+      int64_t wp = 0;
+      const int64_t Nl[4] = {N0, N1, N2, N3};
+      const int lvl[NN_NLAYERS] = {0, 1, 2, 3, 2, 1, 0};
+      for (int l = 0; l < NN_NLAYERS; l++) {
+        const int64_t ng = (Nl[lvl[l]] + WG_VOX - 1) / WG_VOX;
+        wp = std::max(wp, ng * 27 * kLayers[l].cin * kLayers[l].cout);
+      }
+      ELSIM_TRY(dev_alloc(wpart, wp * 4));
+      ELSIM_TRY(dev_alloc(cpart, ((N0 + CSUM_VOX - 1) / CSUM_VOX) * 128 * 4));
+      ELSIM_TRY(dev_alloc(opart, ((N0 + CSUM_VOX - 1) / CSUM_VOX) * 17 * 4));
+    }
+    n1 = a1;
+    n2 = a2;
+    n3 = a3;
+    return ELSIM_OK;
+  }
+
+  // forward into pred (FP32); with a target also dpred and the loss partials
+  int forward(const uint16_t* x0, float* pred, const float* target, double* loss_host) {
+    ELSIM_TRY(pack());
+    hipStream_t s = stream();
+    const int a1 = n1, a2 = n2, a3 = n3;
+    const int m1 = a1 / 2, m2 = a2 / 2, m3 = a3 / 2, q1 = a1 / 4, q2 = a2 / 4, q3 = a3 / 4, r1 = a1 / 8, r2 = a2 / 8,
+              r3 = a3 / 8;
+    const int64_t N0 = (int64_t)a1 * a2 * a3, N1 = N0 / 8, N2 = N1 / 8, N3 = N2 / 8;
+    uint16_t *C0 = (uint16_t*)cat0.p, *C1 = (uint16_t*)cat1.p, *C2 = (uint16_t*)cat2.p;
+    const float* B = P();
+    // encoder
+    ELSIM_TRY(conv<0>(a1, a2, a3, x0, 16, View{16, 0}, WT(L_E0), B + b_off[L_E0], 1, 16, C0, View{48, 32}, s));
+    k_pool<<<nblk(N1 * 16), 256, 0, s>>>(a1, a2, a3, 16, C0, View{48, 32}, (uint16_t*)p1.p);
+    ELSIM_TRY(conv<0>(m1, m2, m3, (uint16_t*)p1.p, 16, View{16, 0}, WT(L_E1), B + b_off[L_E1], 1, 32, C1,
+                      View{96, 64}, s));
+    k_pool<<<nblk(N2 * 32), 256, 0, s>>>(m1, m2, m3, 32, C1, View{96, 64}, (uint16_t*)p2.p);
+    ELSIM_TRY(conv<0>(q1, q2, q3, (uint16_t*)p2.p, 32, View{32, 0}, WT(L_E2), B + b_off[L_E2], 1, 64, C2,
+                      View{128, 64}, s));
+    k_pool<<<nblk(N3 * 64), 256, 0, s>>>(q1, q2, q3, 64, C2, View{128, 64}, (uint16_t*)p3.p);
+    ELSIM_TRY(conv<0>(r1, r2, r3, (uint16_t*)p3.p, 64, View{64, 0}, WT(L_E3), B + b_off[L_E3], 1, 64, e3.p,
+                      View{64, 0}, s));
+    // decoder
+    k_up<<<nblk(N2 * 64), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
+    ELSIM_TRY(conv<0>(q1, q2, q3, C2, 128, View{128, 0}, WT(L_D2), B + b_off[L_D2], 1, 64, d2.p, View{64, 0}, s));
+    k_up<<<nblk(N1 * 64), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
+    ELSIM_TRY(conv<0>(m1, m2, m3, C1, 96, View{96, 0}, WT(L_D1), B + b_off[L_D1], 1, 32, d1.p, View{32, 0}, s));
+    k_up<<<nblk(N0 * 32), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
+    ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, View{48, 0}, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s));
+    const int ob = nblk(N0);
+    if (target && ob > kMaxRedBlocks) {
+      set_error("grid too large for the loss reduction");
+      return ELSIM_ERR_ARG;
+    }
+    k_out_fwd<<<ob, 256, 0, s>>>(N0, (uint16_t*)d0.p, B + out_off, target, pred, (float*)dpred.p,
+                                 workspace().partials);
+    ELSIM_CHECK_LAUNCH();
+    (void)N3;
+    if (target && loss_host) {
+      std::vector<double> h(ob);
+      ELSIM_HIP(hipMemcpyAsync(h.data(), workspace().partials, ob * sizeof(double), hipMemcpyDeviceToHost, s));
+      ELSIM_HIP(hipStreamSynchronize(s));
+      double tot = 0.0;
+      for (double x : h) tot += x;
+      *loss_host = tot / (double)N0;
+    }
+    return ELSIM_OK;
+  }
+
+  int layer_grads(int l, int a1, int a2, int a3, const uint16_t* x, View xv, const float* dact, View dv,
+                  const uint16_t* act, View av) {
+    hipStream_t s = stream();
+    const int cin = kLayers[l].cin, cout = kLayers[l].cout;
+    const int64_t N = (int64_t)a1 * a2 * a3;
+    uint16_t* Z = (uint16_t*)dz.p;
+    k_relu_bwd<<<nblk(N * cout), 256, 0, s>>>(N, cout, dact, dv, act, av, Z);
+    const int ng = (int)((N + WG_VOX - 1) / WG_VOX);
+    k_wgrad<<<dim3(ng, 27), 256, 0, s>>>(a1, a2, a3, x, cin, xv, Z, cout, (float*)wpart.p);
+    k_wgrad_reduce<<<nblk((int64_t)27 * cin * cout), 256, 0, s>>>(ng, cin, cout, (float*)wpart.p,
+                                                                   Gr() + w_off[l]);
+    const int cg = (int)((N + CSUM_VOX - 1) / CSUM_VOX);
+    k_colsum<<<cg, 256, 0, s>>>(N, cout, Z, (float*)cpart.p);
+    k_colsum_reduce<<<1, 128, 0, s>>>(cg, cout, (float*)cpart.p, Gr() + b_off[l]);
+    ELSIM_CHECK_LAUNCH();
+    return ELSIM_OK;
+  }
+
+  int backward(const uint16_t* x0) {
+    hipStream_t s = stream();
+    const int a1 = n1, a2 = n2, a3 = n3;
+    const int m1 = a1 / 2, m2 = a2 / 2, m3 = a3 / 2, q1 = a1 / 4, q2 = a2 / 4, q3 = a3 / 4, r1 = a1 / 8, r2 = a2 / 8,
+              r3 = a3 / 8;
+    const int64_t N0 = (int64_t)a1 * a2 * a3, N1 = N0 / 8, N2 = N1 / 8, N3 = N2 / 8;
+    uint16_t *C0 = (uint16_t*)cat0.p, *C1 = (uint16_t*)cat1.p, *C2 = (uint16_t*)cat2.p;
+    float *DC0 = (float*)dcat0.p, *DC1 = (float*)dcat1.p, *DC2 = (float*)dcat2.p;
+    uint16_t* Z = (uint16_t*)dz.p;
+    // output layer
+    const int og = (int)((N0 + CSUM_VOX - 1) / CSUM_VOX);
+    k_out_bwd<<<og, 256, 0, s>>>(N0, (uint16_t*)d0.p, P() + out_off, (float*)dpred.p, (float*)dd0.p,
+                                 (float*)opart.p);
+    k_colsum_reduce<<<1, 128, 0, s>>>(og, 17, (float*)opart.p, Gr() + out_off);
+    // d0
+    ELSIM_TRY(layer_grads(L_D0, a1, a2, a3, C0, View{48, 0}, (float*)dd0.p, View{16, 0}, (uint16_t*)d0.p,
+                          View{16, 0}));
+    ELSIM_TRY(conv<1>(a1, a2, a3, Z, 16, View{16, 0}, WB(L_D0), nullptr, 0, 48, DC0, View{48, 0}, s));
+    k_up_bwd<<<nblk(N1 * 32), 256, 0, s>>>(a1, a2, a3, 32, DC0, View{48, 0}, (float*)dd1.p);
+    // d1
+    ELSIM_TRY(layer_grads(L_D1, m1, m2, m3, C1, View{96, 0}, (float*)dd1.p, View{32, 0}, (uint16_t*)d1.p,
+                          View{32, 0}));
+    ELSIM_TRY(conv<1>(m1, m2, m3, Z, 32, View{32, 0}, WB(L_D1), nullptr, 0, 96, DC1, View{96, 0}, s));
+    k_up_bwd<<<nblk(N2 * 64), 256, 0, s>>>(m1, m2, m3, 64, DC1, View{96, 0}, (float*)dd2.p);
+    // d2
+    ELSIM_TRY(layer_grads(L_D2, q1, q2, q3, C2, View{128, 0}, (float*)dd2.p, View{64, 0}, (uint16_t*)d2.p,
+                          View{64, 0}));
+    ELSIM_TRY(conv<1>(q1, q2, q3, Z, 64, View{64, 0}, WB(L_D2), nullptr, 0, 128, DC2, View{128, 0}, s));
+    k_up_bwd<<<nblk(N3 * 64), 256, 0, s>>>(q1, q2, q3, 64, DC2, View{128, 0}, (float*)de3.p);
+    // e3
+    ELSIM_TRY(layer_grads(L_E3, r1, r2, r3, (uint16_t*)p3.p, View{64, 0}, (float*)de3.p, View{64, 0},
+                          (uint16_t*)e3.p, View{64, 0}));
+    ELSIM_TRY(conv<1>(r1, r2, r3, Z, 64, View{64, 0}, WB(L_E3), nullptr, 0, 64, dp3.p, View{64, 0}, s));
+    k_pool_bwd<<<nblk(N2 * 64), 256, 0, s>>>(q1, q2, q3, 64, (float*)dp3.p, DC2, View{128, 64});
+    // e2
+    ELSIM_TRY(layer_grads(L_E2, q1, q2, q3, (uint16_t*)p2.p, View{32, 0}, DC2, View{128, 64}, C2, View{128, 64}));
+    ELSIM_TRY(conv<1>(q1, q2, q3, Z, 64, View{64, 0}, WB(L_E2), nullptr, 0, 32, dp2.p, View{32, 0}, s));
+    k_pool_bwd<<<nblk(N1 * 32), 256, 0, s>>>(m1, m2, m3, 32, (float*)dp2.p, DC1, View{96, 64});
+    // e1
+    ELSIM_TRY(layer_grads(L_E1, m1, m2, m3, (uint16_t*)p1.p, View{16, 0}, DC1, View{96, 64}, C1, View{96, 64}));
+    ELSIM_TRY(conv<1>(m1, m2, m3, Z, 32, View{32, 0}, WB(L_E1), nullptr, 0, 16, dp1.p, View{16, 0}, s));
+    k_pool_bwd<<<nblk(N0 * 16), 256, 0, s>>>(a1, a2, a3, 16, (float*)dp1.p, DC0, View{48, 32});
+    // e0
+    ELSIM_TRY(layer_grads(L_E0, a1, a2, a3, x0, View{16, 0}, DC0, View{48, 32}, C0, View{48, 32}));
+    ELSIM_CHECK_LAUNCH();
+    return ELSIM_OK;
+  }
+
+  int adam(float lr) {
+    t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    const int64_t n = nn_param_count();
+    k_adam<<<nblk(n), 256, 0, stream()>>>(n, P(), Gr(), (float*)m.p, (float*)v.p, lr, b1, b2, 1e-8f, c1, c2);
+    ELSIM_CHECK_LAUNCH();
+    packed = false;
+    return ELSIM_OK;
+  }
+
+  void release() {
+    for (DevBuf* b : {&params, &grads, &m, &v, &wt, &wb, &cat0, &p1, &cat1, &p2, &cat2, &p3, &e3, &d2, &d1, &d0,
+                      &dpred, &dd0, &dcat0, &dd1, &dcat1, &dd2, &dcat2, &de3, &dp3, &dp2, &dp1, &dz, &wpart, &cpart,
+                      &opart, &loss})
+      dev_free(*b);
+  }
+};
+
+int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred) {
+  ELSIM_TRY(net->ensure(n1, n2, n3, false));
+  return net->forward(x, pred, nullptr, nullptr);
+}
+
+}  // namespace elsim
+
+using namespace elsim;
+
+namespace {
+NnNet* net_of(const int64_t* h) { return (h && *h) ? (NnNet*)(intptr_t)*h : nullptr; }
+}  // namespace
+
+extern "C" {
+
+int elsim_nn_param_count(void) { return (int)nn_param_count(); }
+
+int elsim_nn_create(int64_t* handle) {
+  if (!handle) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  NnNet* n = new NnNet();
+  const int rc = n->init();
+  if (rc < 0) {
+    n->release();
+    delete n;
+    return rc;
+  }
+  *handle = (int64_t)(intptr_t)n;
+  return ELSIM_OK;
+}
+
+int elsim_nn_destroy(const int64_t* handle) {
+  NnNet* n = net_of(handle);
+  if (!n) return ELSIM_ERR_ARG;
+  (void)hipStreamSynchronize(stream());
+  n->release();
+  delete n;
+  return ELSIM_OK;
+}
+
+int elsim_nn_set_params(const int64_t* handle, const float* host) {
+  NnNet* n = net_of(handle);
+  if (!n || !host) return ELSIM_ERR_ARG;
+  ELSIM_HIP(hipMemcpyAsync(n->params.p, host, nn_param_count() * sizeof(float), hipMemcpyHostToDevice, stream()));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  n->packed = false;
+  return ELSIM_OK;
+}
+
+int elsim_nn_get_params(const int64_t* handle, float* host) {
+  NnNet* n = net_of(handle);
+  if (!n || !host) return ELSIM_ERR_ARG;
+  ELSIM_HIP(hipMemcpyAsync(host, n->params.p, nn_param_count() * sizeof(float), hipMemcpyDeviceToHost, stream()));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  return ELSIM_OK;
+}
+
+int elsim_nn_get_grads(const int64_t* handle, float* host) {
+  NnNet* n = net_of(handle);
+  if (!n || !host) return ELSIM_ERR_ARG;
+  ELSIM_HIP(hipMemcpyAsync(host, n->grads.p, nn_param_count() * sizeof(float), hipMemcpyDeviceToHost, stream()));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  return ELSIM_OK;
+}
+
+int elsim_nn_forward(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, float* pred) {
+  NnNet* n = net_of(handle);
+  if (!n || !x || !pred) return ELSIM_ERR_ARG;
+  return nn_forward(n, n1, n2, n3, x, pred);
+}
+
+int elsim_nn_loss_grad(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, const float* target,
+                       float* pred, double* loss) {
+  NnNet* n = net_of(handle);
+  if (!n || !x || !target || !pred) return ELSIM_ERR_ARG;
+  ELSIM_TRY(n->ensure(n1, n2, n3, true));
+  ELSIM_TRY(n->forward(x, pred, target, loss));
+  return n->backward(x);
+}
+
+int elsim_nn_adam(const int64_t* handle, float lr) {
+  NnNet* n = net_of(handle);
+  if (!n || !(lr > 0.f)) return ELSIM_ERR_ARG;
+  return n->adam(lr);
+}
+
+int elsim_nn_conv3(int n1, int n2, int n3, const uint16_t* x, int cin, const uint16_t* wt, const float* bias,
+                   int cout, int relu, uint16_t* out) {
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !x || !wt || !bias || !out || cout > 64) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  return conv<0>(n1, n2, n3, x, cin, View{cin, 0}, wt, bias, relu, cout, out, View{cout, 0}, stream());
+}
+
+int elsim_nn_conv3_bwd_data(int n1, int n2, int n3, const uint16_t* dz, int cout, const uint16_t* wb, int cin,
+                            float* dx) {
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !dz || !wb || !dx) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  return conv<1>(n1, n2, n3, dz, cout, View{cout, 0}, wb, nullptr, 0, cin, dx, View{cin, 0}, stream());
+}
+
+int elsim_nn_conv3_bwd_weight(int n1, int n2, int n3, const uint16_t* x, int cin, const uint16_t* dz, int cout,
+                              float* dw) {
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !x || !dz || !dw || cin % 16 || cout % 16 ||
+      (cin / 16) * (cout / 16) > 4 * WG_MAXP)
+    return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  const int64_t N = (int64_t)n1 * n2 * n3;
+  const int ng = (int)((N + WG_VOX - 1) / WG_VOX);
+  float* part = (float*)scratch(0, (size_t)ng * 27 * cin * cout * sizeof(float));
+  if (!part) return ELSIM_ERR_ALLOC;
+  k_wgrad<<<dim3(ng, 27), 256, 0, stream()>>>(n1, n2, n3, x, cin, View{cin, 0}, dz, cout, part);
+  k_wgrad_reduce<<<nblk((int64_t)27 * cin * cout), 256, 0, stream()>>>(ng, cin, cout, part, dw);
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@
 #include <memory>
 #include <mutex>
 
+#include "elsim_nn.h"
 #include "elsim_solver.h"
 
 namespace elsim {
@@ -351,6 +352,62 @@ __global__ __launch_bounds__(256) void k_bv(Dims d, const double* __restrict__ p
 
 // ------------------------------------------------------------------ state --
 
+// ------------------------------------------------ learned warm start --
+
+// Sums over the interior for the feature normalisation: phi, phi^2, c, c^2
+// and (phi - phi_prev)^2.
+__global__ __launch_bounds__(256) void k_feat_sums(Dims d, const double* __restrict__ php,
+                                                   const double* __restrict__ prev, const double* __restrict__ cnp,
+                                                   int have_prev, double* partials) {
+  const Col ci = colx(d);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  each(d, ci, [&](int64_t c, int64_t, int) {
+    const double f = php[c], q = cnp[c];
+    acc[0] += f;
+    acc[1] += f * f;
+    acc[2] += q;
+    acc[3] += q * q;
+    if (have_prev) {
+      const double t = f - prev[c];
+      acc[4] += t * t;
+    }
+  });
+  block_partials<5>(acc, partials);
+}
+
+// Network input x[voxel][16] (bf16): psi_P, standardised phi_P and c_P, the
+// through-thickness coordinate (d2, the axis of the boundary conditions) in
+// (-1, 1), and the last step's change of phi_P in units of its rms; the
+// other channels are zero.
+__global__ __launch_bounds__(256) void k_feat_pack(Dims d, const double* __restrict__ psp,
+                                                   const double* __restrict__ php, const double* __restrict__ prev,
+                                                   const double* __restrict__ cnp, double mphi, double iphi, double mc,
+                                                   double ic, double idphi, uint16_t* __restrict__ x) {
+  const Col ci = colx(d);
+  const float depth = (float)(2.0 * (ci.j - 0.5) / d.n2 - 1.0);
+  each(d, ci, [&](int64_t c, int64_t n, int) {
+    uint16_t h[16];
+    h[0] = f2bf((float)psp[c]);
+    h[1] = f2bf((float)((php[c] - mphi) * iphi));
+    h[2] = f2bf((float)((cnp[c] - mc) * ic));
+    h[3] = f2bf(depth);
+    h[4] = f2bf(idphi > 0.0 ? (float)((php[c] - prev[c]) * idphi) : 0.f);
+    for (int q = 5; q < 16; q++) h[q] = 0;
+    uint4* o = (uint4*)(x + n * 16);
+    o[0] = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
+                      h[6] | (uint32_t)h[7] << 16);
+    o[1] = make_uint4(h[8] | (uint32_t)h[9] << 16, h[10] | (uint32_t)h[11] << 16, h[12] | (uint32_t)h[13] << 16,
+                      h[14] | (uint32_t)h[15] << 16);
+  });
+}
+
+// cand(interior) = phi + scale * pred (ghosts as copied from phi).
+__global__ __launch_bounds__(256) void k_ws_apply(Dims d, const double* __restrict__ php, const float* __restrict__ pred,
+                                                  double scale, double* __restrict__ cand) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t c, int64_t n, int) { cand[c] = php[c] + scale * (double)pred[n]; });
+}
+
 struct SbmState {
   Dims d{0, 0, 0};
   elsim_sbm_params P{};
@@ -361,9 +418,14 @@ struct SbmState {
   DevBuf dab01, scal;
   CgWork cg;
   SbmScal* hs = nullptr;  // pinned mirror
+  // learned warm start: phi_P of the previous step, the network, its input,
+  // output and the candidate field
+  DevBuf php_prev, nn_x, nn_pred, ws_cand;
+  int nprev = 0;
+  NnNet* ws_net = nullptr;
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
-                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal};
+                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &nn_x, &nn_pred, &ws_cand};
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
     if (hs) (void)hipHostFree(hs);
@@ -502,6 +564,45 @@ BcArgs cne_bc_args(SbmState& s) {
   return b;
 }
 
+// Network input of the current state (phi_P and c_P at the end of the last
+// step, phi_P of the step before); *dphi_rms = rms of the last change of
+// phi_P (0 when only one field is known: the input channel is then zero).
+int features(SbmState& s, uint16_t* x, double* dphi_rms) {
+  const Dims& d = s.d;
+  const dim3 grid = col_grid(d), blk(TX, TY, 1);
+  const int nb = blocks_of(grid);
+  const int have = s.nprev >= 1;
+  double* P = workspace().partials;
+  k_feat_sums<<<grid, blk, 0, stream()>>>(d, s.g(s.php), s.g(s.php_prev), s.g(s.cnp), have, P);
+  ELSIM_CHECK_LAUNCH();
+  std::vector<double> h((size_t)5 * nb);
+  for (int q = 0; q < 5; q++)
+    ELSIM_HIP(hipMemcpyAsync(h.data() + (size_t)q * nb, P + (size_t)q * kMaxRedBlocks, nb * sizeof(double),
+                             hipMemcpyDeviceToHost, stream()));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  double t[5] = {0, 0, 0, 0, 0};
+  for (int q = 0; q < 5; q++)
+    for (int b = 0; b < nb; b++) t[q] += h[(size_t)q * nb + b];
+  const double N = (double)d.N();
+  const double mphi = t[0] / N, vphi = std::max(t[1] / N - mphi * mphi, 0.0);
+  const double mc = t[2] / N, vc = std::max(t[3] / N - mc * mc, 0.0);
+  const double rms = have ? std::sqrt(t[4] / N) : 0.0;
+  const double iphi = vphi > 0 ? 1.0 / std::sqrt(vphi) : 0.0, ic = vc > 0 ? 1.0 / std::sqrt(vc) : 0.0;
+  k_feat_pack<<<grid, blk, 0, stream()>>>(d, s.g(s.psp), s.g(s.php), s.g(s.php_prev), s.g(s.cnp), mphi, iphi, mc, ic,
+                                          rms > 0 ? 1.0 / rms : 0.0, x);
+  ELSIM_CHECK_LAUNCH();
+  *dphi_rms = rms;
+  return ELSIM_OK;
+}
+
+int ensure_ws_buffers(SbmState& s) {
+  const size_t N = (size_t)s.d.N(), G = (size_t)s.d.G();
+  ELSIM_TRY(dev_alloc(s.nn_x, N * NN_IN_CH * sizeof(uint16_t)));
+  ELSIM_TRY(dev_alloc(s.nn_pred, N * sizeof(float)));
+  ELSIM_TRY(dev_alloc(s.ws_cand, G * sizeof(double)));
+  return ELSIM_OK;
+}
+
 int step(SbmState& s, int mode, elsim_step_stats* st) {
   const Dims& d = s.d;
   const elsim_sbm_params& P = s.P;
@@ -515,6 +616,18 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   ELSIM_HIP(hipEventCreate(&e0));
   ELSIM_HIP(hipEventCreate(&e1));
   ELSIM_HIP(hipEventRecord(e0, stv));
+
+  // 0. learned warm start: predict this step's change of phi_P from the
+  //    state at its start, then remember phi_P for the next step's input
+  double ws_scale = 0.0;
+  const bool ws_on = s.ws_net != nullptr && s.nprev >= 1;
+  if (ws_on) {
+    ELSIM_TRY(ensure_ws_buffers(s));
+    ELSIM_TRY(features(s, (uint16_t*)s.nn_x.p, &ws_scale));
+    if (ws_scale > 0.0) ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, (uint16_t*)s.nn_x.p, (float*)s.nn_pred.p));
+  }
+  ELSIM_HIP(hipMemcpyAsync(s.php_prev.p, s.php.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
+  s.nprev++;
 
   // 1. Cahn-Hilliard (SBM:397-443)
   ELSIM_TRY(ch_update(s));
@@ -568,6 +681,29 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       // the mirror ghosts of phP are already exact (last BC application),
       // so re-mirroring them changes no value
       ELSIM_TRY(bc(s, s.g(s.php), only_hi));
+    }
+    if (nlp == 0 && ws_on && ws_scale > 0.0) {
+      // candidate = phi_P + predicted change; CG starts from whichever of the
+      // two has the smaller FP64 residual (it always solves to the same
+      // tolerance, so the network can only change the iteration count)
+      double* cand = s.g(s.ws_cand);
+      ELSIM_HIP(hipMemcpyAsync(cand, s.g(s.php), (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
+      k_ws_apply<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const float*)s.nn_pred.p, ws_scale, cand);
+      ELSIM_CHECK_LAUNCH();
+      BcArgs cbc = mirror_bc();
+      cbc.hi = 1;
+      cbc.hi_v = P.bvp;
+      ELSIM_TRY(bc(s, cand, cbc));
+      double r_plain = 0.0, r_nn = 0.0;
+      const double sc = 1.0 / (P.dx * P.dx);
+      ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_plain));
+      ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, cand, s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_nn));
+      z.ws_res_plain = std::sqrt(r_plain);
+      z.ws_res_nn = std::sqrt(r_nn);
+      if (r_nn < r_plain) {
+        ELSIM_HIP(hipMemcpyAsync(s.g(s.php), cand, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
+        z.ws_used = 1;
+      }
     }
     ELSIM_TRY(solve(s, ELSIM_OP_PHP, mode, &r));
     z.it_php += r.iters;
@@ -652,7 +788,7 @@ int elsim_sbm_create(const int* n1, const int* n2, const int* n3, const elsim_sb
   }
   const size_t G = (size_t)d.G() * 8, N = (size_t)d.N() * 8;
   ELSIM_TRY(dev_alloc(s->tab, 8 * (size_t)*sz * 8));
-  for (DevBuf* b : {&s->psp, &s->pse, &s->cnp, &s->cne, &s->php, &s->phe, &s->kapp, &s->pce, &s->pcl})
+  for (DevBuf* b : {&s->psp, &s->pse, &s->cnp, &s->cne, &s->php, &s->phe, &s->kapp, &s->pce, &s->pcl, &s->php_prev})
     ELSIM_TRY(dev_alloc(*b, G));
   for (DevBuf* b : {&s->avp, &s->rxn, &s->rxc, &s->rhsp, &s->rhse, &s->rhsc, &s->rhe, &s->kfw, &s->kbw, &s->mu, &s->pm})
     ELSIM_TRY(dev_alloc(*b, N));
@@ -775,6 +911,27 @@ int elsim_sbm_ch_update(const int64_t* handle) {
   SbmState* s = lookup(handle);
   if (!s) return ELSIM_ERR_ARG;
   ELSIM_TRY(ch_update(*s));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  return ELSIM_OK;
+}
+
+int elsim_sbm_set_warm_start(const int64_t* handle, const int64_t* nn) {
+  SbmState* s = lookup(handle);
+  if (!s) return ELSIM_ERR_ARG;
+  NnNet* net = (nn && *nn) ? (NnNet*)(intptr_t)*nn : nullptr;
+  if (net && (s->d.n1 % 8 || s->d.n2 % 8 || s->d.n3 % 8)) {
+    set_error("the warm-start network needs grid dimensions that are multiples of 8 (%dx%dx%d)", s->d.n1, s->d.n2,
+              s->d.n3);
+    return ELSIM_ERR_ARG;
+  }
+  s->ws_net = net;
+  return ELSIM_OK;
+}
+
+int elsim_sbm_nn_features(const int64_t* handle, uint16_t* x_dev, double* dphi_rms) {
+  SbmState* s = lookup(handle);
+  if (!s || !x_dev || !dphi_rms) return ELSIM_ERR_ARG;
+  ELSIM_TRY(features(*s, x_dev, dphi_rms));
   ELSIM_HIP(hipStreamSynchronize(stream()));
   return ELSIM_OK;
 }

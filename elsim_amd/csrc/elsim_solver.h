@@ -79,6 +79,10 @@ int launch_matvec_f64(int op, const Dims& d, const double* p, const double* kap,
 int launch_matvec_f32(int op, const Dims& d, const float* p, const float* kap, const float* pse,
                       float scale, float* out, double* dot_dev, bool hom);
 
+// ||rhs - A x||^2 over the interior, x's ghosts as stored (host result).
+int launch_resid_norm2(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
+                       const double* rhs, double* out);
+
 // Flat kernels shared with the SBM driver.
 int launch_copy_interior_g2n(const Dims& d, const double* g, double* n);  // n = g(interior)
 

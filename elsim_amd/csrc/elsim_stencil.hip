@@ -32,6 +32,7 @@
 // scalars (alpha, beta, stop tests) on the device, so the host only polls a
 // done flag every few iterations.
 #include <cmath>
+#include <vector>
 
 #include "elsim_solver.h"
 
@@ -305,6 +306,22 @@ __global__ __launch_bounds__(256) void k_true_resid(Dims d, const double* __rest
       }
     });
   block_partials<2>(acc, partials);
+}
+
+// sum over the interior of (rhs - A x)^2 with every ghost of x as stored.
+template <int OP>
+__global__ __launch_bounds__(256) void k_resid_norm(Dims d, const double* __restrict__ x, const double* __restrict__ kap,
+                                                    const double* __restrict__ pse, double scale,
+                                                    const double* __restrict__ rhs, double* partials) {
+  double acc[1] = {0.0};
+  const int nt = stencil_ntiles<double>(d);
+  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x)
+    sweep<double, OP, false>(d, stencil_idx<Rows<double>::SJ>(d, tile), x, kap, pse, scale,
+                             [&](int64_t, int64_t n, double, double v) {
+                               const double r = rhs[n] - v;
+                               acc[0] += r * r;
+                             });
+  block_partials<1>(acc, partials);
 }
 
 // ------------------------------------------------------------- Jacobi --
@@ -774,6 +791,31 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
 }
 
 }  // namespace
+
+int launch_resid_norm2(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
+                       const double* rhs, double* out) {
+  const int nb = stencil_ntiles<double>(d);
+  if (nb > kMaxRedBlocks) {
+    set_error("grid too large for the reduction workspace");
+    return ELSIM_ERR_ARG;
+  }
+  double* P = workspace().partials;
+  hipStream_t s = stream();
+  switch (op) {
+    case ELSIM_OP_CNE: k_resid_norm<ELSIM_OP_CNE><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, P); break;
+    case ELSIM_OP_PHP: k_resid_norm<ELSIM_OP_PHP><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, P); break;
+    case ELSIM_OP_PHE: k_resid_norm<ELSIM_OP_PHE><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, P); break;
+    default: set_error("unknown operator %d", op); return ELSIM_ERR_ARG;
+  }
+  ELSIM_CHECK_LAUNCH();
+  std::vector<double> h(nb);
+  ELSIM_HIP(hipMemcpyAsync(h.data(), P, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double v : h) t += v;
+  *out = t;
+  return ELSIM_OK;
+}
 
 int cg_solve(CgWork& w, const SolveArgs& a, int mode, SolveResult* res) {
   ELSIM_TRY(w.alloc(a.d));

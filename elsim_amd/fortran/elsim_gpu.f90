@@ -22,6 +22,8 @@ MODULE elsim_gpu
    TYPE, BIND(C) :: elsim_step_stats
       INTEGER(c_int32_t) :: it_cne, it_php, it_phe, n_outer, n_reliable, status
       REAL(c_double) :: xrf, scrnt, cvgp, cvgl, relres_cne, relres_php, relres_phe, seconds
+      INTEGER(c_int32_t) :: ws_used, ws_pad_
+      REAL(c_double) :: ws_res_plain, ws_res_nn
    END TYPE elsim_step_stats
 
    INTERFACE

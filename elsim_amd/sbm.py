@@ -136,10 +136,24 @@ class Sbm:
         check(rc, "elsim_sbm_step")
         return st
 
+    def set_warm_start(self, net):
+        """Attach a WarmStartNet (None: detach); the state keeps a reference."""
+        self._ws = net
+        nh = C.byref(net.h) if net is not None else None
+        check(lib().elsim_sbm_set_warm_start(C.byref(self.h), nh), "elsim_sbm_set_warm_start")
+
+    def nn_features(self, x_dev):
+        """Network input of the current state into a device bf16 [N][16]
+        buffer; returns the rms of phi_P's last change (0 before two steps)."""
+        s = C.c_double()
+        check(lib().elsim_sbm_nn_features(C.byref(self.h), x_dev.ptr, C.byref(s)), "elsim_sbm_nn_features")
+        return s.value
+
     def close(self):
         if self.h is not None and self.h.value:
             lib().elsim_sbm_destroy(C.byref(self.h))
         self.h = None
+        self._ws = None
 
     def __del__(self):
         try:

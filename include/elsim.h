@@ -122,6 +122,9 @@ typedef struct elsim_step_stats {
   double  cvgp, cvgl;  /* SBM:723, 804 at exit */
   double  relres_cne, relres_php, relres_phe; /* true FP64 relative residual at exit of the last solve */
   double  seconds;     /* device time of the step (hipEvent) */
+  int32_t ws_used;     /* learned warm start taken for the first phP solve */
+  int32_t ws_pad_;
+  double  ws_res_plain, ws_res_nn; /* ||b - A x|| of the two phP starting points */
 } elsim_step_stats;
 
 void elsim_sbm_default_params(elsim_sbm_params* p);
@@ -266,6 +269,53 @@ int elsim_prof_read(elsim_kernel_prof* out);
  * initial residual b - A x, SBM:492-499, 649-655, 729-735).  out: interior. */
 int elsim_dev_matvec_stored_f64(int op, int n1, int n2, int n3, const double* p, const double* kap,
                                 const double* pse, double scale, double* out);
+
+/* ------------------------------------------------------------------------ */
+/* Learned CG warm start (no counterpart in the reference).                 */
+/* A 4-level 3-D convolutional encoder-decoder (3x3x3 convolutions,         */
+/* channels 16/32/64/64, average pooling, nearest upsampling, concatenated  */
+/* skips, ReLU, 1x1x1 output) in bfloat16 with FP32 accumulation on MFMA.   */
+/* Tensors on device, channels-last: x[voxel][16] (bf16 bits), voxel =      */
+/* i + n1 (j + n2 k), grid dims multiples of 8.  Parameters (FP32, flat):   */
+/* for the layers e0 e1 e2 e3 d2 d1 d0 (cin->cout: 16->16, 16->32, 32->64,  */
+/* 64->64, 128->64, 96->32, 48->16) W[cout][27][cin] (tap = dx + 3 (dy + 3 dz),*/
+/* d1 fastest) then b[cout]; then the output layer w[16], b.                */
+/* ------------------------------------------------------------------------ */
+int elsim_nn_param_count(void);
+int elsim_nn_create(int64_t* handle);
+int elsim_nn_destroy(const int64_t* handle);
+int elsim_nn_set_params(const int64_t* handle, const float* host_params);
+int elsim_nn_get_params(const int64_t* handle, float* host_params);
+int elsim_nn_get_grads(const int64_t* handle, float* host_grads);
+/* pred[voxel] (FP32, device) of the input features x (device). */
+int elsim_nn_forward(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, float* pred);
+/* Forward, loss = mean((pred - target)^2) (host double) and the gradients of
+ * every parameter (kept on device, read with elsim_nn_get_grads). */
+int elsim_nn_loss_grad(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, const float* target,
+                       float* pred, double* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the last gradients. */
+int elsim_nn_adam(const int64_t* handle, float lr);
+/* Attach a network to an SBM state (nn NULL or 0: detach).  Each step then
+ * predicts the change of phi_P from the state at its start; the first phP
+ * solve starts from the prediction when its FP64 residual is smaller than
+ * that of the previous phi_P, and CG solves to the same tolerance either way.
+ * The network must outlive the attachment. */
+int elsim_sbm_set_warm_start(const int64_t* sbm_handle, const int64_t* nn_handle);
+/* The network input of the SBM state's current fields (bf16 [voxel][16],
+ * device) and the rms of phi_P's last change (the unit of the prediction;
+ * 0 before two steps have run). */
+int elsim_sbm_nn_features(const int64_t* sbm_handle, uint16_t* x_dev, double* dphi_rms);
+/* Single layers on device data (parity tests).  conv3: out[v][co] =
+ * bf16(relu?(b[co] + sum_{tap,ci} wt[co][tap cin + ci] x[v + o_tap][ci]));
+ * bwd_data: dx[v][ci] (FP32) = sum_{tap,co} wb[ci][tap cout + co] dz[v + o_tap][co]
+ * with wb the tap-flipped, transposed weights; bwd_weight: dw[co][tap][ci]
+ * (FP32) = sum_v dz[v][co] x[v + o_tap][ci].  Zero padding, stride 1. */
+int elsim_nn_conv3(int n1, int n2, int n3, const uint16_t* x, int cin, const uint16_t* wt, const float* bias,
+                   int cout, int relu, uint16_t* out);
+int elsim_nn_conv3_bwd_data(int n1, int n2, int n3, const uint16_t* dz, int cout, const uint16_t* wb, int cin,
+                            float* dx);
+int elsim_nn_conv3_bwd_weight(int n1, int n2, int n3, const uint16_t* x, int cin, const uint16_t* dz, int cout,
+                              float* dw);
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic inputs (host only; deterministic for a seed)                   */
