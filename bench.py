@@ -102,6 +102,22 @@ def profiled_step(E, s, mode):
     return {k: {"us": prof.mean_us(i), "launches": int(prof.count[i])} for i, k in enumerate(keys) if prof.count[i]}
 
 
+def nn_forward_time(E, net, n, reps=5):
+    """Seconds per forward pass of the warm-start network at n^3 (HIP
+    events on the library stream), on zero input."""
+    L = E.lib()
+    x = E.DeviceArray((n ** 3 * 16,), np.uint16)
+    E.check(L.elsim_memset(x.ptr, 0, x.nbytes))
+    pred = E.DeviceArray((n ** 3,), np.float32)
+    net.forward((n, n, n), x, pred)
+    E.check(L.elsim_timer_start())
+    for _ in range(reps):
+        net.forward((n, n, n), x, pred)
+    ms = C.c_float()
+    E.check(L.elsim_timer_stop(C.byref(ms)))
+    return ms.value * 1e-3 / reps
+
+
 def cpu_baseline(sdf, prm, tab, iters_per_step, budget_iters=12):
     """FP64 oracle, one thread: time `budget_iters` CG iterations of the phP
     system at full size and scale by the FP64 path's iterations per step."""
@@ -144,10 +160,12 @@ def reference_cg_cost(E, sdf, prm, tab, cnp0, cap=400):
             "relres_phe_after_cap": st.relres_phe, "cap_per_solve": cap}
 
 
-def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0, profile=True):
+def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0, profile=True, net=None):
     from elsim_amd._lib import F_CNP
     s = E.Sbm(sdf, -sdf, prm, tab)
     s.set(F_CNP, cnp0)
+    if net is not None:
+        s.set_warm_start(net)
     stats = []
     for w in range(warmup):
         st = s.step(mode)
@@ -161,6 +179,7 @@ def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0, profile=True):
     E.check(E.lib().elsim_synchronize())
     t = time.perf_counter() - t0
     kern = profiled_step(E, s, mode) if profile else {}
+    s.set_warm_start(None)
     if kern:
         log("  in-loop kernels: " + ", ".join(f"{k} {v['us']:.1f} us" for k, v in kern.items()))
     s.close()
@@ -171,11 +190,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fast,fp64,ref", help="comma list of fast, fp64, ref")
+    ap.add_argument("--paths", default="fast,fast_extrap,fast_plain,fp64,ref",
+                    help="comma list of fast, fast_extrap, fast_plain, fp64, ref")
+    ap.add_argument("--weights", default=None, help="warm-start network (default: the packaged one)")
     ap.add_argument("--inner-rtol", type=float, default=None)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the operator timing and a 512 MiB device copy (for counter runs)")
@@ -210,12 +231,26 @@ def main():
     paths = a.paths.split(",")
     nan = float("nan")
     tfast, stf, t64, st64, ref_it, kf, k64 = nan, [], nan, [], None, {}, {}
+    from elsim_amd import nn as NN
+    net = NN.WarmStartNet.load(a.weights or NN.DEFAULT_WEIGHTS)
+    variants = {}
     if "fast" in paths:
-        log("fast path (mixed precision)")
-        tfast, stf, kf = run_path(E, sdf, prm, tab, E.MODE_MIXED, a.warmup, a.steps, cnp0)
+        log("fast path (mixed precision + learned warm start)")
+        tfast, stf, kf = run_path(E, sdf, prm.copy(ws_mode=E.WS_LEARNED), tab, E.MODE_MIXED, a.warmup, a.steps, cnp0,
+                                  net=net)
+        variants["learned"] = (tfast, stf)
+    for key, wm, label in (("fast_extrap", E.WS_EXTRAP, "extrapolated"), ("fast_plain", E.WS_PLAIN, "plain")):
+        if key in paths:
+            log(f"mixed precision, {label} warm start")
+            t, st, _ = run_path(E, sdf, prm.copy(ws_mode=wm), tab, E.MODE_MIXED, a.warmup, a.steps, cnp0,
+                                profile=False)
+            variants[label] = (t, st)
     if "fp64" in paths:
-        log("fp64 path (Jacobi-PCG, all FP64)")
-        t64, st64, k64 = run_path(E, sdf, prm, tab, E.MODE_FP64_PCG, a.warmup, a.steps, cnp0)
+        log("fp64 path (Jacobi-PCG, all FP64, plain warm start)")
+        t64, st64, k64 = run_path(E, sdf, prm.copy(ws_mode=E.WS_PLAIN), tab, E.MODE_FP64_PCG, a.warmup, a.steps,
+                                  cnp0)
+    t_nn = nn_forward_time(E, net, n) if n % 8 == 0 else nan
+    log(f"warm-start network forward at {n}^3: {t_nn * 1e3:.2f} ms")
     if "ref" in paths:
         log("reference CG (unpreconditioned FP64): cost per iteration on capped solves")
         ref_it = reference_cg_cost(E, sdf, prm, tab, cnp0)
@@ -263,11 +298,20 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"sbm_electrochem_step_{n}^3", "grid": [n, n, n], "spheres": ns,
                    "solid_fraction": round(frac, 4), "sphere_radius": [6, 16], "dt": prm.dt, "rtol": prm.rtol,
-                   "fast_path": "fp32 jacobi-pcg + fp64 reliable updates", "learned_warm_start": False},
+                   "fast_path": "fp32 jacobi-pcg + fp64 reliable updates + learned warm start",
+                   "learned_warm_start": True},
         "fp64_s_per_step": t64,
-        "fp64_path": "fp64 jacobi-pcg (same preconditioner, all FP64)",
+        "fp64_path": "fp64 jacobi-pcg (same preconditioner, all FP64), plain warm start as the reference",
         "speedup_vs_fp64": t64 / tfast,
         "reference_cg": ref_it,
+        "warm_start": {
+            k: {"s_per_step": v[0],
+                "php_iters_per_step": float(np.mean([x["it_php"] for x in v[1]])) if v[1] else nan,
+                "cg_iters_per_step": float(np.mean([x["it_cne"] + x["it_php"] + x["it_phe"] for x in v[1]]))
+                if v[1] else nan}
+            for k, v in variants.items()},
+        "nn_forward_ms": t_nn * 1e3,
+        "nn_weights": os.path.relpath(a.weights or NN.DEFAULT_WEIGHTS, ROOT),
         "cg_iters_per_step": {"fp64": iters64, "fast": itersf},
         "voxel_updates_per_s": {"fp64": N * iters64 / t64, "fast": N * itersf / tfast},
         "roofline": roof,

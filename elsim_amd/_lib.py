@@ -17,6 +17,7 @@ ERR_ARG, ERR_HIP, ERR_NAN, ERR_ALLOC = -1, -2, -3, -4
 STOP_REFERENCE, STOP_RELRES = 0, 1
 MODE_FP64, MODE_MIXED, MODE_FP64_PCG = 0, 1, 2
 OP_CNE, OP_PHP, OP_PHE = 0, 1, 2
+WS_PLAIN, WS_EXTRAP, WS_LEARNED = 0, 1, 2
 F_PSP, F_PSE, F_AVP, F_CNP, F_CNE, F_PHP, F_PHE, F_RXN, F_KAPP = range(9)
 GHOSTED_FIELDS = {F_PSP, F_PSE, F_CNP, F_CNE, F_PHP, F_PHE, F_KAPP}
 
@@ -28,7 +29,7 @@ class ElsimError(RuntimeError):
 _PARAM_DOUBLES = ("dx dt zeta psp_off pse_off avp_min eps rho ch_psi_min frd cst1 t_minus d0 tc1 tc2 "
                   "dab_c0 dab_c1 dab_c2 dab_c3 kap_scale i0_scale kfw_cref bvp bve vsr trgi cne_target "
                   "rtol crt_e crt_p crt_l outer_tol inner_rtol").split()
-_PARAM_INTS = "mass_corr_period skip_cne_first stop_rule max_it_e max_it_p max_it_l max_outer pad_".split()
+_PARAM_INTS = "mass_corr_period skip_cne_first stop_rule max_it_e max_it_p max_it_l max_outer ws_mode".split()
 
 
 class SbmParams(C.Structure):
@@ -43,7 +44,7 @@ class SbmParams(C.Structure):
         return p
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad_"}
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class StepStats(C.Structure):
@@ -51,7 +52,7 @@ class StepStats(C.Structure):
     _fields_ = [(n, C.c_int32) for n in "it_cne it_php it_phe n_outer n_reliable status".split()] + \
                [(n, C.c_double) for n in "xrf scrnt cvgp cvgl relres_cne relres_php relres_phe seconds".split()] + \
                [("ws_used", C.c_int32), ("ws_pad_", C.c_int32), ("ws_res_plain", C.c_double),
-                ("ws_res_nn", C.c_double)]
+                ("ws_res_nn", C.c_double), ("ws_alpha", C.c_double), ("ws_energy_gain", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -123,9 +124,12 @@ _SIGS = {
     "elsim_nn_forward": (I, [C.POINTER(C.c_int64), I, I, I, P, P]),
     "elsim_nn_loss_grad": (I, [C.POINTER(C.c_int64), I, I, I, P, P, P, DP]),
     "elsim_nn_adam": (I, [C.POINTER(C.c_int64), C.c_float]),
+    "elsim_nn_train_forward": (I, [C.POINTER(C.c_int64), I, I, I, P, P]),
+    "elsim_nn_energy_loss": (I, [C.POINTER(C.c_int64), P, P, P, DP]),
+    "elsim_nn_backward": (I, [C.POINTER(C.c_int64), P]),
     "elsim_nn_conv3": (I, [I, I, I, P, I, P, P, I, I, P]),
     "elsim_sbm_set_warm_start": (I, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
-    "elsim_sbm_nn_features": (I, [C.POINTER(C.c_int64), P, DP]),
+    "elsim_sbm_ws_record": (I, [C.POINTER(C.c_int64), P, P]),
     "elsim_nn_conv3_bwd_data": (I, [I, I, I, P, I, P, I, P]),
     "elsim_nn_conv3_bwd_weight": (I, [I, I, I, P, I, P, I, P]),
     "elsim_prof_read": (I, [C.POINTER(KernelProf)]),

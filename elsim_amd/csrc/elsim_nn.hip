@@ -24,6 +24,7 @@
 
 #include "elsim_common.h"
 #include "elsim_nn.h"
+#include "elsim_solver.h"
 
 namespace elsim {
 
@@ -384,6 +385,22 @@ __global__ __launch_bounds__(256) void k_out_bwd(int64_t N, const uint16_t* __re
   }
 }
 
+// ------------------------------------------------- energy-norm training --
+
+// dg (ghosted (n+2)^3, zero ghosts) interior = pred - target.
+__global__ void k_delta_g(int n1, int n2, int n3, const float* __restrict__ pred, const float* __restrict__ target,
+                          float* __restrict__ dg) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, N = (int64_t)n1 * n2 * n3;
+  if (v >= N) return;
+  const int i = (int)(v % n1), j = (int)((v / n1) % n2), k = (int)(v / ((int64_t)n1 * n2));
+  dg[(i + 1) + (int64_t)(n1 + 2) * ((j + 1) + (int64_t)(n2 + 2) * (k + 1))] = pred[v] - target[v];
+}
+
+__global__ void k_scale_f(int64_t n, const float* __restrict__ a, float s, float* __restrict__ b) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) b[e] = s * a[e];
+}
+
 // ------------------------------------------------------------- optimiser --
 
 __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -441,7 +458,8 @@ struct NnNet {
   DevBuf cat0, p1, cat1, p2, cat2, p3, e3, d2, d1, d0, dpred;
   DevBuf dd0, dcat0, dd1, dcat1, dd2, dcat2, de3, dp3, dp2, dp1, dz;
   DevBuf wpart, cpart, opart;
-  DevBuf loss;  // device double
+  DevBuf loss;            // device double
+  DevBuf eg_dg, eg_w;     // energy loss: ghosted error, operator output
 
   NnNet() {
     int64_t o = 0, ow = 0;
@@ -642,6 +660,28 @@ struct NnNet {
     return ELSIM_OK;
   }
 
+  // loss = e.(-A e)/N with e = pred - target and A the homogeneous phP
+  // operator of the coefficient kap (ghosted FP32, scale 1); dpred = -2 A e/N.
+  int energy_loss(const float* pred, const float* target, const float* kap, double* loss_host) {
+    hipStream_t s = stream();
+    const Dims d{n1, n2, n3};
+    const int64_t N = d.N(), G = d.G();
+    ELSIM_TRY(dev_alloc(eg_dg, G * sizeof(float)));
+    ELSIM_TRY(dev_alloc(eg_w, N * sizeof(float)));
+    ELSIM_HIP(hipMemsetAsync(eg_dg.p, 0, G * sizeof(float), s));
+    k_delta_g<<<nblk(N), 256, 0, s>>>(n1, n2, n3, pred, target, (float*)eg_dg.p);
+    ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(launch_matvec_f32(ELSIM_OP_PHP, d, (const float*)eg_dg.p, kap, nullptr, 1.0f, (float*)eg_w.p,
+                                (double*)loss.p, true));
+    k_scale_f<<<nblk(N), 256, 0, s>>>(N, (const float*)eg_w.p, -2.f / (float)N, (float*)dpred.p);
+    ELSIM_CHECK_LAUNCH();
+    double dot = 0.0;
+    ELSIM_HIP(hipMemcpyAsync(&dot, loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    ELSIM_HIP(hipStreamSynchronize(s));
+    if (loss_host) *loss_host = -dot / (double)N;
+    return ELSIM_OK;
+  }
+
   int adam(float lr) {
     t += 1;
     const float b1 = 0.9f, b2 = 0.999f;
@@ -656,7 +696,7 @@ struct NnNet {
   void release() {
     for (DevBuf* b : {&params, &grads, &m, &v, &wt, &wb, &cat0, &p1, &cat1, &p2, &cat2, &p3, &e3, &d2, &d1, &d0,
                       &dpred, &dd0, &dcat0, &dd1, &dcat1, &dd2, &dcat2, &de3, &dp3, &dp2, &dp1, &dz, &wpart, &cpart,
-                      &opart, &loss})
+                      &opart, &loss, &eg_dg, &eg_w})
       dev_free(*b);
   }
 };
@@ -738,6 +778,26 @@ int elsim_nn_loss_grad(const int64_t* handle, int n1, int n2, int n3, const uint
   if (!n || !x || !target || !pred) return ELSIM_ERR_ARG;
   ELSIM_TRY(n->ensure(n1, n2, n3, true));
   ELSIM_TRY(n->forward(x, pred, target, loss));
+  return n->backward(x);
+}
+
+int elsim_nn_train_forward(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, float* pred) {
+  NnNet* n = net_of(handle);
+  if (!n || !x || !pred) return ELSIM_ERR_ARG;
+  ELSIM_TRY(n->ensure(n1, n2, n3, true));
+  return n->forward(x, pred, nullptr, nullptr);
+}
+
+int elsim_nn_energy_loss(const int64_t* handle, const float* pred, const float* target, const float* kap,
+                         double* loss) {
+  NnNet* n = net_of(handle);
+  if (!n || !pred || !target || !kap || !n->dpred.p) return ELSIM_ERR_ARG;
+  return n->energy_loss(pred, target, kap, loss);
+}
+
+int elsim_nn_backward(const int64_t* handle, const uint16_t* x) {
+  NnNet* n = net_of(handle);
+  if (!n || !x || !n->dpred.p) return ELSIM_ERR_ARG;
   return n->backward(x);
 }
 

@@ -354,44 +354,25 @@ __global__ __launch_bounds__(256) void k_bv(Dims d, const double* __restrict__ p
 
 // ------------------------------------------------ learned warm start --
 
-// Sums over the interior for the feature normalisation: phi, phi^2, c, c^2
-// and (phi - phi_prev)^2.
-__global__ __launch_bounds__(256) void k_feat_sums(Dims d, const double* __restrict__ php,
-                                                   const double* __restrict__ prev, const double* __restrict__ cnp,
-                                                   int have_prev, double* partials) {
-  const Col ci = colx(d);
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  each(d, ci, [&](int64_t c, int64_t, int) {
-    const double f = php[c], q = cnp[c];
-    acc[0] += f;
-    acc[1] += f * f;
-    acc[2] += q;
-    acc[3] += q * q;
-    if (have_prev) {
-      const double t = f - prev[c];
-      acc[4] += t * t;
-    }
-  });
-  block_partials<5>(acc, partials);
-}
-
-// Network input x[voxel][16] (bf16): psi_P, standardised phi_P and c_P, the
-// through-thickness coordinate (d2, the axis of the boundary conditions) in
-// (-1, 1), and the last step's change of phi_P in units of its rms; the
+// Network input x[voxel][16] (bf16) at the first phP solve of a step:
+// psi_P, the FP64 residual of the extrapolated start in units of its rms,
+// the through-thickness coordinate (d2, the axis of the boundary
+// conditions) in (-1, 1), log10 of the phP coefficient over 7 (its range is
+// 7 decades) and the last step's change of phi_P in units of its rms; the
 // other channels are zero.
 __global__ __launch_bounds__(256) void k_feat_pack(Dims d, const double* __restrict__ psp,
-                                                   const double* __restrict__ php, const double* __restrict__ prev,
-                                                   const double* __restrict__ cnp, double mphi, double iphi, double mc,
-                                                   double ic, double idphi, uint16_t* __restrict__ x) {
+                                                   const double* __restrict__ r0, const double* __restrict__ kap,
+                                                   const double* __restrict__ dphi, double ir, double idp,
+                                                   uint16_t* __restrict__ x) {
   const Col ci = colx(d);
   const float depth = (float)(2.0 * (ci.j - 0.5) / d.n2 - 1.0);
   each(d, ci, [&](int64_t c, int64_t n, int) {
     uint16_t h[16];
     h[0] = f2bf((float)psp[c]);
-    h[1] = f2bf((float)((php[c] - mphi) * iphi));
-    h[2] = f2bf((float)((cnp[c] - mc) * ic));
-    h[3] = f2bf(depth);
-    h[4] = f2bf(idphi > 0.0 ? (float)((php[c] - prev[c]) * idphi) : 0.f);
+    h[1] = f2bf((float)(r0[n] * ir));
+    h[2] = f2bf(depth);
+    h[3] = f2bf((float)(log10(fmax(kap[c], 1e-30)) / 7.0));
+    h[4] = f2bf((float)(dphi[c] * idp));
     for (int q = 5; q < 16; q++) h[q] = 0;
     uint4* o = (uint4*)(x + n * 16);
     o[0] = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
@@ -401,11 +382,43 @@ __global__ __launch_bounds__(256) void k_feat_pack(Dims d, const double* __restr
   });
 }
 
-// cand(interior) = phi + scale * pred (ghosts as copied from phi).
-__global__ __launch_bounds__(256) void k_ws_apply(Dims d, const double* __restrict__ php, const float* __restrict__ pred,
-                                                  double scale, double* __restrict__ cand) {
+// sum over the interior of a^2 (ghosted) -- the rms of the last change.
+__global__ __launch_bounds__(256) void k_sum_sq_g(Dims d, const double* __restrict__ a, double* partials) {
   const Col ci = colx(d);
-  each(d, ci, [&](int64_t c, int64_t n, int) { cand[c] = php[c] + scale * (double)pred[n]; });
+  double acc[1] = {0.0};
+  each(d, ci, [&](int64_t c, int64_t, int) { acc[0] += a[c] * a[c]; });
+  block_partials<1>(acc, partials);
+}
+
+// interior copy out[n] = a[c].
+__global__ __launch_bounds__(256) void k_interior(Dims d, const double* __restrict__ a, double* __restrict__ out) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t c, int64_t n, int) { out[n] = a[c]; });
+}
+
+// out(interior) = base + scale * pred (ghosts untouched).
+__global__ __launch_bounds__(256) void k_ws_apply(Dims d, const double* base, const float* __restrict__ pred,
+                                                  double scale, double* out) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t c, int64_t n, int) { out[c] = base[c] + scale * (double)pred[n]; });
+}
+
+// out = a - b over the whole ghosted box.
+__global__ __launch_bounds__(256) void k_ws_diff(int64_t G, const double* __restrict__ a, const double* __restrict__ b,
+                                                 double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * (FLAT_T * FLAT_U) + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < FLAT_U; u++) {
+    const int64_t q = e + u * FLAT_T;
+    if (q < G) out[q] = a[q] - b[q];
+  }
+}
+
+// x(interior) += alpha d.
+__global__ __launch_bounds__(256) void k_ws_axpy(Dims d, double* __restrict__ x, const double* __restrict__ dv,
+                                                 double alpha) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t c, int64_t, int) { x[c] += alpha * dv[c]; });
 }
 
 struct SbmState {
@@ -420,12 +433,14 @@ struct SbmState {
   SbmScal* hs = nullptr;  // pinned mirror
   // learned warm start: phi_P of the previous step, the network, its input,
   // output and the candidate field
-  DevBuf php_prev, nn_x, nn_pred, ws_cand;
+  DevBuf php_prev, nn_x, nn_pred, ws_cand, ws_dir, ws_r0;
+  uint16_t* rec_x = nullptr;  // armed recording of the next step's network input
+  double* rec_start = nullptr;
   int nprev = 0;
   NnNet* ws_net = nullptr;
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
-                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &nn_x, &nn_pred, &ws_cand};
+                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_r0};
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
     if (hs) (void)hipHostFree(hs);
@@ -564,34 +579,28 @@ BcArgs cne_bc_args(SbmState& s) {
   return b;
 }
 
-// Network input of the current state (phi_P and c_P at the end of the last
-// step, phi_P of the step before); *dphi_rms = rms of the last change of
-// phi_P (0 when only one field is known: the input channel is then zero).
-int features(SbmState& s, uint16_t* x, double* dphi_rms) {
+double host_sum(const double* partials, int nb) {
+  std::vector<double> h(nb);
+  if (hipMemcpyAsync(h.data(), partials, nb * sizeof(double), hipMemcpyDeviceToHost, stream()) != hipSuccess ||
+      hipStreamSynchronize(stream()) != hipSuccess)
+    return NAN;
+  double t = 0.0;
+  for (double v : h) t += v;
+  return t;
+}
+
+// The network input at the first phP solve: the start's residual r0
+// (interior, rms rms_r) and the last change of phi_P (ws_dir, ghosted).
+int ws_features(SbmState& s, const double* r0, double rms_r, uint16_t* x) {
   const Dims& d = s.d;
   const dim3 grid = col_grid(d), blk(TX, TY, 1);
-  const int nb = blocks_of(grid);
-  const int have = s.nprev >= 1;
-  double* P = workspace().partials;
-  k_feat_sums<<<grid, blk, 0, stream()>>>(d, s.g(s.php), s.g(s.php_prev), s.g(s.cnp), have, P);
+  k_sum_sq_g<<<grid, blk, 0, stream()>>>(d, s.g(s.ws_dir), workspace().partials);
   ELSIM_CHECK_LAUNCH();
-  std::vector<double> h((size_t)5 * nb);
-  for (int q = 0; q < 5; q++)
-    ELSIM_HIP(hipMemcpyAsync(h.data() + (size_t)q * nb, P + (size_t)q * kMaxRedBlocks, nb * sizeof(double),
-                             hipMemcpyDeviceToHost, stream()));
-  ELSIM_HIP(hipStreamSynchronize(stream()));
-  double t[5] = {0, 0, 0, 0, 0};
-  for (int q = 0; q < 5; q++)
-    for (int b = 0; b < nb; b++) t[q] += h[(size_t)q * nb + b];
-  const double N = (double)d.N();
-  const double mphi = t[0] / N, vphi = std::max(t[1] / N - mphi * mphi, 0.0);
-  const double mc = t[2] / N, vc = std::max(t[3] / N - mc * mc, 0.0);
-  const double rms = have ? std::sqrt(t[4] / N) : 0.0;
-  const double iphi = vphi > 0 ? 1.0 / std::sqrt(vphi) : 0.0, ic = vc > 0 ? 1.0 / std::sqrt(vc) : 0.0;
-  k_feat_pack<<<grid, blk, 0, stream()>>>(d, s.g(s.psp), s.g(s.php), s.g(s.php_prev), s.g(s.cnp), mphi, iphi, mc, ic,
-                                          rms > 0 ? 1.0 / rms : 0.0, x);
+  const double sd = host_sum(workspace().partials, blocks_of(grid));
+  const double rms_d = std::sqrt(sd / (double)d.N());
+  k_feat_pack<<<grid, blk, 0, stream()>>>(d, s.g(s.psp), r0, s.g(s.kapp), s.g(s.ws_dir),
+                                          rms_r > 0 ? 1.0 / rms_r : 0.0, rms_d > 0 ? 1.0 / rms_d : 0.0, x);
   ELSIM_CHECK_LAUNCH();
-  *dphi_rms = rms;
   return ELSIM_OK;
 }
 
@@ -600,6 +609,8 @@ int ensure_ws_buffers(SbmState& s) {
   ELSIM_TRY(dev_alloc(s.nn_x, N * NN_IN_CH * sizeof(uint16_t)));
   ELSIM_TRY(dev_alloc(s.nn_pred, N * sizeof(float)));
   ELSIM_TRY(dev_alloc(s.ws_cand, G * sizeof(double)));
+  ELSIM_TRY(dev_alloc(s.ws_dir, G * sizeof(double)));
+  ELSIM_TRY(dev_alloc(s.ws_r0, N * sizeof(double)));
   return ELSIM_OK;
 }
 
@@ -617,14 +628,15 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   ELSIM_HIP(hipEventCreate(&e1));
   ELSIM_HIP(hipEventRecord(e0, stv));
 
-  // 0. learned warm start: predict this step's change of phi_P from the
-  //    state at its start, then remember phi_P for the next step's input
-  double ws_scale = 0.0;
-  const bool ws_on = s.ws_net != nullptr && s.nprev >= 1;
+  // 0. warm start of the first phP solve: keep phi_P's change over the last
+  //    step (from the third step on: the first starts from the initial
+  //    fields, the second's predecessor change is that start-up transient)
+  const bool ws_learned = P.ws_mode == ELSIM_WS_LEARNED && (s.ws_net != nullptr || s.rec_x != nullptr);
+  const bool ws_on = (P.ws_mode == ELSIM_WS_EXTRAP || ws_learned) && s.nprev >= 2;
   if (ws_on) {
     ELSIM_TRY(ensure_ws_buffers(s));
-    ELSIM_TRY(features(s, (uint16_t*)s.nn_x.p, &ws_scale));
-    if (ws_scale > 0.0) ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, (uint16_t*)s.nn_x.p, (float*)s.nn_pred.p));
+    k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.php), s.g(s.php_prev), s.g(s.ws_dir));
+    ELSIM_CHECK_LAUNCH();
   }
   ELSIM_HIP(hipMemcpyAsync(s.php_prev.p, s.php.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
   s.nprev++;
@@ -682,28 +694,67 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       // so re-mirroring them changes no value
       ELSIM_TRY(bc(s, s.g(s.php), only_hi));
     }
-    if (nlp == 0 && ws_on && ws_scale > 0.0) {
-      // candidate = phi_P + predicted change; CG starts from whichever of the
-      // two has the smaller FP64 residual (it always solves to the same
-      // tolerance, so the network can only change the iteration count)
-      double* cand = s.g(s.ws_cand);
-      ELSIM_HIP(hipMemcpyAsync(cand, s.g(s.php), (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
-      k_ws_apply<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const float*)s.nn_pred.p, ws_scale, cand);
+    if (nlp == 0 && ws_on) {
+      // 1. linear extrapolation in time: x = phi + dphi (dphi's ghosts are
+      //    differences of consistent ghosts; the BC is re-applied).
+      const double sc = 1.0 / (P.dx * P.dx);
+      double r_plain = 0.0, r_ws = 0.0;
+      ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_plain));
+      k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.ws_dir), 1.0);
       ELSIM_CHECK_LAUNCH();
       BcArgs cbc = mirror_bc();
       cbc.hi = 1;
       cbc.hi_v = P.bvp;
-      ELSIM_TRY(bc(s, cand, cbc));
-      double r_plain = 0.0, r_nn = 0.0;
-      const double sc = 1.0 / (P.dx * P.dx);
-      ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_plain));
-      ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, cand, s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_nn));
-      z.ws_res_plain = std::sqrt(r_plain);
-      z.ws_res_nn = std::sqrt(r_nn);
-      if (r_nn < r_plain) {
-        ELSIM_HIP(hipMemcpyAsync(s.g(s.php), cand, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
-        z.ws_used = 1;
+      ELSIM_TRY(bc(s, s.g(s.php), cbc));
+      z.ws_used = 1;
+      z.ws_alpha = 0.0;
+      if (ws_learned) {
+        // 2. learned correction: the network maps (psi_P, r0 = b - A x, the
+        //    coefficient, the last change) to the shape p of the remaining
+        //    error; the Galerkin step alpha = p.r0 / p.(-A) p along it
+        //    minimises the A-norm error (A = -L is SPD on corrections with the
+        //    search direction's homogeneous ghosts), so it never makes the
+        //    start worse in the norm CG minimises.
+        double* r0 = s.g(s.ws_r0);
+        double rr = 0.0;
+        ELSIM_TRY(launch_resid_store(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), r0, &rr));
+        const double rms_r = std::sqrt(rr / (double)d.N());
+        uint16_t* xin = s.rec_x ? s.rec_x : (uint16_t*)s.nn_x.p;
+        ELSIM_TRY(ws_features(s, r0, rms_r, xin));
+        if (s.rec_start) {
+          k_interior<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.rec_start);
+          ELSIM_CHECK_LAUNCH();
+        }
+        if (s.ws_net && rms_r > 0.0) {
+          ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p));
+          double* pd = s.g(s.ws_cand);  // correction direction, zero ghosts
+          ELSIM_HIP(hipMemsetAsync(pd, 0, (size_t)d.G() * sizeof(double), stv));
+          k_ws_apply<<<grid, blk, 0, stv>>>(d, pd, (const float*)s.nn_pred.p, 1.0, pd);
+          ELSIM_CHECK_LAUNCH();
+          double num = 0.0, den = 0.0;
+          double* scr = (double*)scratch(1, (size_t)d.N() * sizeof(double) + sizeof(double));
+          if (!scr) return ELSIM_ERR_ALLOC;
+          double* ddot = scr + d.N();
+          ELSIM_TRY(launch_dot_resid(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), pd, &num));
+          ELSIM_TRY(launch_matvec_f64(ELSIM_OP_PHP, d, pd, s.g(s.kapp), nullptr, sc, scr, ddot, true));
+          ELSIM_HIP(hipMemcpyAsync(&den, ddot, sizeof(double), hipMemcpyDeviceToHost, stv));
+          ELSIM_HIP(hipStreamSynchronize(stv));
+          const double alpha = (den < 0.0 && std::isfinite(num / den)) ? num / den : 0.0;  // den = p.L p < 0
+          if (alpha != 0.0) {
+            k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), pd, alpha);
+            ELSIM_CHECK_LAUNCH();
+            ELSIM_TRY(bc(s, s.g(s.php), cbc));
+            z.ws_used = 2;
+            z.ws_alpha = alpha;
+            z.ws_energy_gain = num * num / -den;  // decrease of the squared A-norm error
+          }
+        }
+        s.rec_x = nullptr;
+        s.rec_start = nullptr;
       }
+      ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_ws));
+      z.ws_res_plain = std::sqrt(r_plain);
+      z.ws_res_nn = std::sqrt(r_ws);
     }
     ELSIM_TRY(solve(s, ELSIM_OP_PHP, mode, &r));
     z.it_php += r.iters;
@@ -928,11 +979,11 @@ int elsim_sbm_set_warm_start(const int64_t* handle, const int64_t* nn) {
   return ELSIM_OK;
 }
 
-int elsim_sbm_nn_features(const int64_t* handle, uint16_t* x_dev, double* dphi_rms) {
+int elsim_sbm_ws_record(const int64_t* handle, uint16_t* x_dev, double* start_dev) {
   SbmState* s = lookup(handle);
-  if (!s || !x_dev || !dphi_rms) return ELSIM_ERR_ARG;
-  ELSIM_TRY(features(*s, x_dev, dphi_rms));
-  ELSIM_HIP(hipStreamSynchronize(stream()));
+  if (!s) return ELSIM_ERR_ARG;
+  s->rec_x = x_dev;
+  s->rec_start = x_dev ? start_dev : nullptr;
   return ELSIM_OK;
 }
 

@@ -83,6 +83,13 @@ int launch_matvec_f32(int op, const Dims& d, const float* p, const float* kap, c
 int launch_resid_norm2(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
                        const double* rhs, double* out);
 
+// r = rhs - A x (interior), x's ghosts as stored; *out = sum r^2 (host).
+int launch_resid_store(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
+                       const double* rhs, double* r, double* out);
+// sum over the interior of dv * (rhs - A x), x's ghosts as stored (host result).
+int launch_dot_resid(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
+                     const double* rhs, const double* dv, double* out);
+
 // Flat kernels shared with the SBM driver.
 int launch_copy_interior_g2n(const Dims& d, const double* g, double* n);  // n = g(interior)
 

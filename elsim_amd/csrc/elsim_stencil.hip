@@ -324,6 +324,38 @@ __global__ __launch_bounds__(256) void k_resid_norm(Dims d, const double* __rest
   block_partials<1>(acc, partials);
 }
 
+// r(interior) = rhs - A x (x's ghosts as stored) and the sum of r^2.
+template <int OP>
+__global__ __launch_bounds__(256) void k_resid_store(Dims d, const double* __restrict__ x, const double* __restrict__ kap,
+                                                     const double* __restrict__ pse, double scale,
+                                                     const double* __restrict__ rhs, double* __restrict__ r,
+                                                     double* partials) {
+  double acc[1] = {0.0};
+  const int nt = stencil_ntiles<double>(d);
+  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x)
+    sweep<double, OP, false>(d, stencil_idx<Rows<double>::SJ>(d, tile), x, kap, pse, scale,
+                             [&](int64_t, int64_t n, double, double v) {
+                               const double q = rhs[n] - v;
+                               r[n] = q;
+                               acc[0] += q * q;
+                             });
+  block_partials<1>(acc, partials);
+}
+
+// sum over the interior of dv * (rhs - A x), x's ghosts as stored.
+template <int OP>
+__global__ __launch_bounds__(256) void k_dot_resid(Dims d, const double* __restrict__ x, const double* __restrict__ kap,
+                                                   const double* __restrict__ pse, double scale,
+                                                   const double* __restrict__ rhs, const double* __restrict__ dv,
+                                                   double* partials) {
+  double acc[1] = {0.0};
+  const int nt = stencil_ntiles<double>(d);
+  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x)
+    sweep<double, OP, false>(d, stencil_idx<Rows<double>::SJ>(d, tile), x, kap, pse, scale,
+                             [&](int64_t c, int64_t n, double, double v) { acc[0] += dv[c] * (rhs[n] - v); });
+  block_partials<1>(acc, partials);
+}
+
 // ------------------------------------------------------------- Jacobi --
 
 // dinv = 1 / diag(A_hom) (FP64), stored as T.  Mirror faces do not
@@ -805,6 +837,56 @@ int launch_resid_norm2(int op, const Dims& d, const double* x, const double* kap
     case ELSIM_OP_CNE: k_resid_norm<ELSIM_OP_CNE><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, P); break;
     case ELSIM_OP_PHP: k_resid_norm<ELSIM_OP_PHP><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, P); break;
     case ELSIM_OP_PHE: k_resid_norm<ELSIM_OP_PHE><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, P); break;
+    default: set_error("unknown operator %d", op); return ELSIM_ERR_ARG;
+  }
+  ELSIM_CHECK_LAUNCH();
+  std::vector<double> h(nb);
+  ELSIM_HIP(hipMemcpyAsync(h.data(), P, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double v : h) t += v;
+  *out = t;
+  return ELSIM_OK;
+}
+
+int launch_resid_store(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
+                       const double* rhs, double* r, double* out) {
+  const int nb = stencil_ntiles<double>(d);
+  if (nb > kMaxRedBlocks) {
+    set_error("grid too large for the reduction workspace");
+    return ELSIM_ERR_ARG;
+  }
+  double* P = workspace().partials;
+  hipStream_t s = stream();
+  switch (op) {
+    case ELSIM_OP_CNE: k_resid_store<ELSIM_OP_CNE><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, r, P); break;
+    case ELSIM_OP_PHP: k_resid_store<ELSIM_OP_PHP><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, r, P); break;
+    case ELSIM_OP_PHE: k_resid_store<ELSIM_OP_PHE><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, r, P); break;
+    default: set_error("unknown operator %d", op); return ELSIM_ERR_ARG;
+  }
+  ELSIM_CHECK_LAUNCH();
+  std::vector<double> h(nb);
+  ELSIM_HIP(hipMemcpyAsync(h.data(), P, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double v : h) t += v;
+  *out = t;
+  return ELSIM_OK;
+}
+
+int launch_dot_resid(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
+                     const double* rhs, const double* dv, double* out) {
+  const int nb = stencil_ntiles<double>(d);
+  if (nb > kMaxRedBlocks) {
+    set_error("grid too large for the reduction workspace");
+    return ELSIM_ERR_ARG;
+  }
+  double* P = workspace().partials;
+  hipStream_t s = stream();
+  switch (op) {
+    case ELSIM_OP_CNE: k_dot_resid<ELSIM_OP_CNE><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, dv, P); break;
+    case ELSIM_OP_PHP: k_dot_resid<ELSIM_OP_PHP><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, dv, P); break;
+    case ELSIM_OP_PHE: k_dot_resid<ELSIM_OP_PHE><<<nb, TX, 0, s>>>(d, x, kap, pse, scale, rhs, dv, P); break;
     default: set_error("unknown operator %d", op); return ELSIM_ERR_ARG;
   }
   ELSIM_CHECK_LAUNCH();

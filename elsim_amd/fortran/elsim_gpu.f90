@@ -16,14 +16,14 @@ MODULE elsim_gpu
          d0, tc1, tc2, dab_c0, dab_c1, dab_c2, dab_c3, kap_scale, i0_scale, kfw_cref, bvp, bve, vsr, trgi, &
          cne_target, rtol, crt_e, crt_p, crt_l, outer_tol, inner_rtol
       INTEGER(c_int32_t) :: mass_corr_period, skip_cne_first, stop_rule, max_it_e, max_it_p, max_it_l, &
-         max_outer, pad_
+         max_outer, ws_mode
    END TYPE elsim_sbm_params
 
    TYPE, BIND(C) :: elsim_step_stats
       INTEGER(c_int32_t) :: it_cne, it_php, it_phe, n_outer, n_reliable, status
       REAL(c_double) :: xrf, scrnt, cvgp, cvgl, relres_cne, relres_php, relres_phe, seconds
       INTEGER(c_int32_t) :: ws_used, ws_pad_
-      REAL(c_double) :: ws_res_plain, ws_res_nn
+      REAL(c_double) :: ws_res_plain, ws_res_nn, ws_alpha, ws_energy_gain
    END TYPE elsim_step_stats
 
    INTERFACE

@@ -18,7 +18,7 @@ LAYERS = (  # name, cin, cout -- same order as the C-ABI flat parameter vector
     ("d2", 128, 64), ("d1", 96, 32), ("d0", 48, 16),
 )
 IN_CH = 16
-FEATURES = ("psi_p", "phi_p", "c_p", "depth", "dphi_prev")  # channels 0..4 of the input
+FEATURES = ("psi_p", "residual", "depth", "log10_kappa", "dphi_prev")  # channels 0..4 of the input
 
 
 def to_bf16(x):
@@ -109,6 +109,20 @@ class WarmStartNet:
         check(lib().elsim_nn_loss_grad(C.byref(self.h), *dims, x.ptr, target.ptr, pred.ptr, C.byref(loss)),
               "elsim_nn_loss_grad")
         return loss.value
+
+    def train_forward(self, dims, x, pred):
+        check(lib().elsim_nn_train_forward(C.byref(self.h), *dims, x.ptr, pred.ptr), "elsim_nn_train_forward")
+
+    def energy_loss(self, pred, target, kap):
+        """Squared A-norm error (phP operator of the ghosted FP32 coefficient
+        kap) of pred against target; leaves its gradient for backward()."""
+        loss = C.c_double()
+        check(lib().elsim_nn_energy_loss(C.byref(self.h), pred.ptr, target.ptr, kap.ptr, C.byref(loss)),
+              "elsim_nn_energy_loss")
+        return loss.value
+
+    def backward(self, x):
+        check(lib().elsim_nn_backward(C.byref(self.h), x.ptr), "elsim_nn_backward")
 
     def adam(self, lr):
         check(lib().elsim_nn_adam(C.byref(self.h), C.c_float(lr)), "elsim_nn_adam")

@@ -142,12 +142,11 @@ class Sbm:
         nh = C.byref(net.h) if net is not None else None
         check(lib().elsim_sbm_set_warm_start(C.byref(self.h), nh), "elsim_sbm_set_warm_start")
 
-    def nn_features(self, x_dev):
-        """Network input of the current state into a device bf16 [N][16]
-        buffer; returns the rms of phi_P's last change (0 before two steps)."""
-        s = C.c_double()
-        check(lib().elsim_sbm_nn_features(C.byref(self.h), x_dev.ptr, C.byref(s)), "elsim_sbm_nn_features")
-        return s.value
+    def ws_record(self, x_dev, start_dev):
+        """Arm recording of the next step's warm-start network input (device
+        bf16 [N][16]) and extrapolated start (device FP64 [N]); None disarms."""
+        check(lib().elsim_sbm_ws_record(C.byref(self.h), x_dev.ptr if x_dev is not None else None,
+                                        start_dev.ptr if start_dev is not None else None), "elsim_sbm_ws_record")
 
     def close(self):
         if self.h is not None and self.h.value:

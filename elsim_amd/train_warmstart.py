@@ -3,15 +3,18 @@
     python -m elsim_amd.train_warmstart --n 64 --train-seeds 40 --test-seeds 10 \\
         --steps 50 --epochs 3 --out weights.npz
 
-Pairs: for each seeded sphere pack and each step t of a run with the fast
-solver, the input is the network features of the state at the start of the
-step (psi_P, phi_P, c_P, the through-thickness coordinate and phi_P's last
-change, see elsim_sbm_nn_features) and the target is this step's change of
-phi_P in units of the last change's rms.  Samples stay in device memory.
-Training: Adam on the mean squared error, one volume per update, with the
-project's own forward/backward kernels.  Evaluation: held-out seeds run the
-same steps with and without the warm start; phP CG iterations and time are
-reported.
+Pairs: for each seeded sphere pack and each step (from the third) of a run
+with the fast solver, the input is the network features of the extrapolated
+start of the step's first phP solve (psi_P, its FP64 residual, the
+through-thickness coordinate, log10 of the coefficient and phi_P's last
+change; see elsim_sbm_ws_record) and the target is the converged phi_P minus
+that start, in units of its rms.  Samples stay in device memory.
+Training: Adam on the squared A-norm error of the prediction (the norm CG
+minimises, A the phP operator of the sample's microstructure) or on the
+mean squared error, one volume per update, with the project's own
+forward/backward kernels.  Evaluation: held-out seeds run the
+same steps with the plain, the extrapolated and the learned warm start; phP
+CG iterations and time are reported.
 """
 import argparse
 import json
@@ -22,7 +25,7 @@ import numpy as np
 
 import elsim_amd as E
 from elsim_amd import nn as NN
-from elsim_amd._lib import F_CNP, F_PHP, DeviceArray
+from elsim_amd._lib import F_CNP, F_KAPP, F_PHP, DeviceArray
 
 
 def log(*a):
@@ -43,32 +46,43 @@ def interior(a):
 
 
 def generate(n, seeds, steps, rmin, rmax, mode):
-    """Device-resident samples [(x bf16 [N,16], target f32 [N])] and stats."""
+    """Device-resident samples [(x bf16 [N,16], target f32 [N], kap f32
+    ghosted)].  At every step from the third on, the solver records the
+    network input of its extrapolated phP start (elsim_sbm_ws_record); the
+    target is the converged phi_P minus that start, in units of its rms (the
+    network predicts the shape of the remaining error, the solver's Galerkin
+    step its size).  kap: the phP coefficient of the microstructure."""
     N = n ** 3
     samples, its = [], []
     t0 = time.perf_counter()
     for k, seed in enumerate(seeds):
         s, ns, frac = make_state(n, seed, rmin, rmax)
+        s.set_params(s.params.copy(ws_mode=E.WS_LEARNED))
+        kap = DeviceArray.from_host(s.get(F_KAPP).ravel(order="F").astype(np.float32))
         for t in range(steps):
-            x = DeviceArray((N * NN.IN_CH,), np.uint16)
-            scale = s.nn_features(x)
-            before = interior(s.get(F_PHP))
+            x = start = None
+            if t >= 2:
+                x = DeviceArray((N * NN.IN_CH,), np.uint16)
+                start = DeviceArray((N,), np.float64)
+                s.ws_record(x, start)
             st = s.step(mode)
             its.append(st.it_php)
-            after = interior(s.get(F_PHP))
-            if scale > 0.0:
+            if x is not None:
                 # voxel = i + n1 (j + n2 k): Fortran order of the interior
-                tgt = ((after - before) / scale).ravel(order="F").astype(np.float32)
-                samples.append((x, DeviceArray.from_host(tgt)))
-            else:
-                x.free()
+                e = interior(s.get(F_PHP)).ravel(order="F") - start.to_host()
+                start.free()
+                rms = float(np.sqrt(np.mean(e ** 2)))
+                if rms > 0.0 and np.isfinite(rms):
+                    samples.append((x, DeviceArray.from_host((e / rms).astype(np.float32)), kap))
+                else:
+                    x.free()
         s.close()
         log(f"  seed {seed}: {ns} spheres, solid {frac:.3f}; {len(samples)} samples, "
             f"{time.perf_counter() - t0:.0f} s")
     return samples, its
 
 
-def train(net, samples, n, epochs, lr, seed=0):
+def train(net, samples, n, epochs, lr, loss_kind="energy", seed=0):
     rng = np.random.default_rng(seed)
     dims = (n, n, n)
     pred = DeviceArray((n ** 3,), np.float32)
@@ -78,10 +92,15 @@ def train(net, samples, n, epochs, lr, seed=0):
         order = rng.permutation(len(samples))
         run = []
         for i in order:
-            x, y = samples[i]
+            x, y, kap = samples[i]
             # cosine decay over the whole run
             cur = lr * 0.5 * (1.0 + np.cos(np.pi * k / max(total, 1)))
-            loss = net.loss_grad(dims, x, y, pred)
+            if loss_kind == "energy":
+                net.train_forward(dims, x, pred)
+                loss = net.energy_loss(pred, y, kap)
+                net.backward(x)
+            else:
+                loss = net.loss_grad(dims, x, y, pred)
             net.adam(max(cur, 1e-6))
             run.append(loss)
             k += 1
@@ -90,29 +109,32 @@ def train(net, samples, n, epochs, lr, seed=0):
     return hist
 
 
-def evaluate(net, n, seeds, steps, rmin, rmax, mode, warmup=2):
-    """phP CG iterations and seconds per step with and without the warm
-    start, on the same seeds from the same initial states (the first
-    `warmup` steps, which cannot use it, are excluded)."""
-    res = {"plain": {"it_php": 0, "seconds": 0.0}, "learned": {"it_php": 0, "seconds": 0.0, "used": 0, "steps": 0}}
+def evaluate(net, n, seeds, steps, rmin, rmax, mode, warmup=2, modes=(E.WS_PLAIN, E.WS_EXTRAP, E.WS_LEARNED)):
+    """phP CG iterations and seconds per step of the same runs (same seeds,
+    same initial states) for each warm-start mode; the first `warmup` steps,
+    which no warm start applies to, are excluded."""
+    res = {int(m): {"it_php": 0, "seconds": 0.0, "used": 0, "steps": 0} for m in modes}
     for seed in seeds:
-        for key in ("plain", "learned"):
+        for m in modes:
             s, _, _ = make_state(n, seed, rmin, rmax)
-            if key == "learned":
+            s.set_params(s.params.copy(ws_mode=int(m)))
+            if m == E.WS_LEARNED:
                 s.set_warm_start(net)
             for t in range(steps):
                 st = s.step(mode)
                 if t >= warmup:
-                    res[key]["it_php"] += st.it_php
-                    res[key]["seconds"] += st.seconds
-                    if key == "learned":
-                        res[key]["used"] += st.ws_used
-                        res[key]["steps"] += 1
+                    r = res[int(m)]
+                    r["it_php"] += st.it_php
+                    r["seconds"] += st.seconds
+                    r["used"] += int(st.ws_used == (2 if m == E.WS_LEARNED else 1))
+                    r["steps"] += 1
             s.close()
-        log(f"  eval seed {seed}: plain {res['plain']['it_php']} learned {res['learned']['it_php']} phP iterations")
-    p, l = res["plain"], res["learned"]
-    res["iterations_saved"] = 1.0 - l["it_php"] / max(p["it_php"], 1)
-    res["time_saved"] = 1.0 - l["seconds"] / max(p["seconds"], 1e-30)
+        log(f"  eval seed {seed}: " + ", ".join(f"mode {m} {res[int(m)]['it_php']}" for m in modes))
+    base = res[int(modes[0])]
+    for m in modes:
+        r = res[int(m)]
+        r["iterations_saved_vs_plain"] = 1.0 - r["it_php"] / max(base["it_php"], 1)
+        r["time_saved_vs_plain"] = 1.0 - r["seconds"] / max(base["seconds"], 1e-30)
     return res
 
 
@@ -135,6 +157,7 @@ def main(argv=None):
     ap.add_argument("--rmin", type=float, default=4.0)
     ap.add_argument("--rmax", type=float, default=8.0)
     ap.add_argument("--init-seed", type=int, default=0)
+    ap.add_argument("--loss", choices=["energy", "mse"], default="mse")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     a = ap.parse_args(argv)
@@ -149,20 +172,21 @@ def main(argv=None):
     log(f"{len(samples)} samples in {t_gen:.0f} s; training {a.epochs} epochs")
     net = NN.WarmStartNet(seed=a.init_seed)
     t0 = time.perf_counter()
-    hist = train(net, samples, a.n, a.epochs, a.lr)
+    hist = train(net, samples, a.n, a.epochs, a.lr, a.loss)
     t_train = time.perf_counter() - t0
-    for x, y in samples:
+    for x, y, _ in samples:
         x.free()
         y.free()
     log(f"evaluating on {len(test_seeds)} held-out microstructures x {a.eval_steps} steps")
     ev = evaluate(net, a.n, test_seeds, a.eval_steps, a.rmin, a.rmax, mode)
     report = {"n": a.n, "train_seeds": [train_seeds[0], train_seeds[-1]], "test_seeds": [test_seeds[0], test_seeds[-1]],
-              "steps": a.steps, "samples": len(samples), "epochs": a.epochs, "lr": a.lr, "loss_per_epoch": hist,
+              "steps": a.steps, "samples": len(samples), "epochs": a.epochs, "loss": a.loss, "lr": a.lr, "loss_per_epoch": hist,
               "generate_s": t_gen, "train_s": t_train, "updates_per_s": a.epochs * len(samples) / max(t_train, 1e-9),
               "radius": [a.rmin, a.rmax], "eval": ev}
     log(json.dumps(report))
     if a.out:
-        net.meta = {"n": a.n, "features": list(NN.FEATURES), "target": "dphi_P / rms(last dphi_P)",
+        net.meta = {"n": a.n, "features": list(NN.FEATURES),
+                    "target": "(converged phi_P - extrapolated start) / rms",
                     "train": report}
         net.save(a.out)
     if a.report:

@@ -106,8 +106,18 @@ typedef struct elsim_sbm_params {
   int32_t max_it_p;         /* SBM:710 */
   int32_t max_it_l;         /* SBM:791 */
   int32_t max_outer;        /* cap on SBM:627 (the reference has none) */
-  int32_t pad_;
+  int32_t ws_mode;          /* warm start of the first phP solve: ELSIM_WS_* */
 } elsim_sbm_params;
+
+/* The starting point of the first phP solve of a step (phi = phi_P at the
+ * end of the last step, dphi = its change over that step), from the third
+ * step on. */
+enum {
+  ELSIM_WS_PLAIN = 0,    /* phi (as the reference) */
+  ELSIM_WS_EXTRAP = 1,   /* x = phi + dphi (linear extrapolation in time) */
+  ELSIM_WS_LEARNED = 2   /* x + alpha p: p the attached network's prediction of the remaining
+                            error from x's FP64 residual, alpha = p.r0 / p.(-A) p (A-norm optimal) */
+};
 
 /* Per-step statistics returned by the step entry points. */
 typedef struct elsim_step_stats {
@@ -122,9 +132,11 @@ typedef struct elsim_step_stats {
   double  cvgp, cvgl;  /* SBM:723, 804 at exit */
   double  relres_cne, relres_php, relres_phe; /* true FP64 relative residual at exit of the last solve */
   double  seconds;     /* device time of the step (hipEvent) */
-  int32_t ws_used;     /* learned warm start taken for the first phP solve */
+  int32_t ws_used;     /* learned warm start applied to the first phP solve */
   int32_t ws_pad_;
-  double  ws_res_plain, ws_res_nn; /* ||b - A x|| of the two phP starting points */
+  double  ws_res_plain, ws_res_nn; /* ||b - A x|| before / after it */
+  double  ws_alpha;     /* Galerkin step length along the prediction */
+  double  ws_energy_gain; /* decrease of the squared A-norm error it gave */
 } elsim_step_stats;
 
 void elsim_sbm_default_params(elsim_sbm_params* p);
@@ -293,18 +305,29 @@ int elsim_nn_forward(const int64_t* handle, int n1, int n2, int n3, const uint16
  * every parameter (kept on device, read with elsim_nn_get_grads). */
 int elsim_nn_loss_grad(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, const float* target,
                        float* pred, double* loss);
+/* Training with another loss: forward keeping the activations, a loss that
+ * leaves dL/dpred on the device, then the backward pass of the last forward.
+ * energy_loss: L = e.(-A e)/N with e = pred - target and A the homogeneous
+ * phP operator (SBM:670-676 with unit scale) of the ghosted FP32 coefficient
+ * kap -- the squared A-norm error that CG minimises. */
+int elsim_nn_train_forward(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, float* pred);
+int elsim_nn_energy_loss(const int64_t* handle, const float* pred, const float* target, const float* kap,
+                         double* loss);
+int elsim_nn_backward(const int64_t* handle, const uint16_t* x);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the last gradients. */
 int elsim_nn_adam(const int64_t* handle, float lr);
-/* Attach a network to an SBM state (nn NULL or 0: detach).  Each step then
- * predicts the change of phi_P from the state at its start; the first phP
- * solve starts from the prediction when its FP64 residual is smaller than
- * that of the previous phi_P, and CG solves to the same tolerance either way.
- * The network must outlive the attachment. */
+/* Attach a network to an SBM state (nn NULL or 0: detach) for
+ * ELSIM_WS_LEARNED: CG then solves to the same tolerance from a start that is
+ * never worse in the A-norm than the extrapolation, so the network can only
+ * change the iteration count.  The network must outlive the attachment. */
 int elsim_sbm_set_warm_start(const int64_t* sbm_handle, const int64_t* nn_handle);
-/* The network input of the SBM state's current fields (bf16 [voxel][16],
- * device) and the rms of phi_P's last change (the unit of the prediction;
- * 0 before two steps have run). */
-int elsim_sbm_nn_features(const int64_t* sbm_handle, uint16_t* x_dev, double* dphi_rms);
+/* Training data: during the next step (ELSIM_WS_LEARNED, from the third
+ * step on) write the network input (bf16 [voxel][16]: psi_P, the start's
+ * FP64 residual / its rms, the through-thickness coordinate, log10(kappa)/7,
+ * the last change / its rms) to x_dev and the extrapolated start's interior
+ * (FP64 [voxel]) to start_dev; the target is the converged phi_P minus that
+ * start.  x_dev NULL disarms. */
+int elsim_sbm_ws_record(const int64_t* sbm_handle, uint16_t* x_dev, double* start_dev);
 /* Single layers on device data (parity tests).  conv3: out[v][co] =
  * bf16(relu?(b[co] + sum_{tap,ci} wt[co][tap cin + ci] x[v + o_tap][ci]));
  * bwd_data: dx[v][ci] (FP32) = sum_{tap,co} wb[ci][tap cout + co] dz[v + o_tap][co]

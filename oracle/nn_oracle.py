@@ -160,17 +160,19 @@ def loss(pred, target):
     return float(np.mean((pred - target) ** 2))
 
 
-def backward(p, cache, pred, target, exact=False):
-    """Gradients of loss(pred, target) = mean((pred - target)^2) with
-    respect to every parameter, following the GPU's rounding points: the
-    gradient reaching a convolution output is FP32 (here FP64), multiplied by
-    the ReLU mask and rounded to bfloat16 before it enters the backward
+def backward(p, cache, pred, target, exact=False, dpred=None):
+    """Gradients of loss(pred, target) = mean((pred - target)^2) (or of any
+    loss whose gradient with respect to pred is `dpred`) with respect to
+    every parameter, following the GPU's rounding points: the gradient
+    reaching a convolution output is FP32 (here FP64), multiplied by the ReLU
+    mask and rounded to bfloat16 before it enters the backward
     convolutions."""
     R = Rounder(exact)
     W, c = cache["W"], cache
     g = {}
     n = pred.size
-    dpred = 2.0 * (pred - target) / n
+    if dpred is None:
+        dpred = 2.0 * (pred - target) / n
     g["out.w"] = np.tensordot(dpred, c["d0"], axes=([0, 1, 2], [0, 1, 2]))
     g["out.b"] = np.array([dpred.sum()])
     d_d0 = dpred[..., None] * W["out.w"]

@@ -150,3 +150,37 @@ def test_training_reduces_loss(gpu):
         net.adam(3e-3)
     assert losses[-1] < 0.2 * losses[0], losses[::10]
     net.close()
+
+
+def test_energy_loss_and_gradients(gpu, oracle_lib):
+    """Squared A-norm loss e.(-A e)/N (A: homogeneous phP operator of kap,
+    unit scale) and the gradients it sends through the network."""
+    import elsim_amd as E
+    shape = (16, 8, 8)
+    n1, n2, n3 = shape
+    p, x, target = network_case(shape, 9)
+    rng = np.random.default_rng(9)
+    kap = rng.uniform(1e-7, 1.0, (n1 + 2, n2 + 2, n3 + 2))  # ghosted, [i, j, k]
+    net = N.WarmStartNet(params=N.flatten(p))
+    xd, td = dev(N.to_bf16(cl(x))), dev(target.astype(np.float32).ravel())
+    kd = dev(oracle_lib.flat(kap, np.float32))
+    pd = DeviceArray((n1 * n2 * n3,), np.float32)
+    net.train_forward(shape, xd, pd)
+    loss = net.energy_loss(pd, td, kd)
+    net.backward(xd)
+    g = N.unflatten(net.grads())
+    # oracle: the same forward, then dpred = -2 A e / N in FP64
+    pred_ref, cache = NO.forward(p, x)
+    pred_gpu = pd.to_host().reshape(n3, n2, n1).astype(np.float64)
+    e = (pred_gpu - target.astype(np.float32).astype(np.float64)).transpose(2, 1, 0)  # -> [i, j, k]
+    eg = np.zeros((n1 + 2, n2 + 2, n3 + 2))
+    eg[1:-1, 1:-1, 1:-1] = e
+    kap32 = kap.astype(np.float32).astype(np.float64)
+    Ae = oracle_lib.apply_op(E.OP_PHP, kap32, None, oracle_lib.prj_bc(E.OP_PHP, eg), 0.0, 1.0)
+    N0 = n1 * n2 * n3
+    assert loss == pytest.approx(-np.sum(e * Ae) / N0, rel=1e-5)
+    dpred = (-2.0 * Ae / N0).transpose(2, 1, 0)  # -> (n3, n2, n1)
+    g_ref = NO.backward(p, cache, pred_ref, target, dpred=dpred)
+    for k in g_ref:
+        assert rel_l2(g[k], g_ref[k]) < 5e-2, k
+    net.close()
