@@ -82,6 +82,7 @@ _SIGS = {
     "elsim_last_error": (C.c_char_p, []),
     "elsim_device_count": (I, []),
     "elsim_synchronize": (I, []),
+    "elsim_mem_info": (I, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "elsim_malloc": (I, [C.POINTER(P), C.c_size_t]),
     "elsim_free": (I, [P]),
     "elsim_memcpy_h2d": (I, [P, P, C.c_size_t]),
@@ -135,7 +136,7 @@ _SIGS = {
     "elsim_prof_read": (I, [C.POINTER(KernelProf)]),
     "elsim_dev_matvec_f32": (I, [I, I, I, I, P, P, P, C.c_float, P, P]),
     "elsim_gen_sphere_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, I, D, P, IP, DP]),
-    "elsim_gen_ellipsoid_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, D, P, IP, DP]),
+    "elsim_gen_ellipsoid_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, D, D, P, IP, DP]),
     "elsim_gen_tunnel": (I, [I, I, I, C.c_uint64, D, P]),
     "elsim_gen_tables": (I, [I, P]),
 }

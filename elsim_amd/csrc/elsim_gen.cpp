@@ -222,9 +222,10 @@ extern "C" int elsim_gen_sphere_pack(int n1, int n2, int n3, uint64_t seed, doub
 }
 
 extern "C" int elsim_gen_ellipsoid_pack(int n1, int n2, int n3, uint64_t seed, double amin, double amax,
-                                        double cone_deg, double target, double dcut, float* sdf,
+                                        double cone_deg, double target, double overlap, double dcut, float* sdf,
                                         int* nshapes, double* frac) {
-  if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !sdf || amin <= 0 || amax < amin) return ELSIM_ERR_ARG;
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !sdf || amin <= 0 || amax < amin || !(overlap >= 0.0 && overlap <= 1.0))
+    return ELSIM_ERR_ARG;
   struct Ell {
     double c[3], ax[3], R[9];  // rows of R: body axes in world coords
   };
@@ -270,9 +271,11 @@ extern "C" int elsim_gen_ellipsoid_pack(int n1, int n2, int n3, uint64_t seed, d
     e.c[0] = rng.uniform() * n1;
     e.c[1] = rng.uniform() * n2;
     e.c[2] = rng.uniform() * n3;
-    // rasterise: voxel-level non-overlap
+    // rasterise; reject when more than `overlap` of the shape's voxels are
+    // already solid (0: voxel-level non-overlap, 1: free overlap)
     cand.clear();
     bool ok = true;
+    int64_t hit = 0;
     int lo[3], hi[3], n[3] = {n1, n2, n3};
     for (int q = 0; q < 3; q++) {
       lo[q] = std::max(1, (int)std::floor(e.c[q] - major));
@@ -290,19 +293,25 @@ extern "C" int elsim_gen_ellipsoid_pack(int n1, int n2, int n3, uint64_t seed, d
           if (r2 < 1.0) {
             size_t id = (size_t)(i - 1) + (size_t)n1 * ((j - 1) + (size_t)n2 * (k - 1));
             if (occ[id]) {
-              ok = false;
-              break;
+              hit++;
+              if (overlap == 0.0) {
+                ok = false;
+                break;
+              }
             }
             cand.push_back(id);
           }
         }
+    if (ok && overlap < 1.0 && (double)hit > overlap * (double)cand.size()) ok = false;
     if (!ok) {
       fails++;
       continue;
     }
     fails = 0;
-    for (size_t id : cand) occ[id] = 1;
-    covered += (int64_t)cand.size();
+    for (size_t id : cand) {
+      covered += occ[id] == 0;
+      occ[id] = 1;
+    }
     es.push_back(e);
   }
   // approximate SDF: for each ellipsoid, (1 - rho) * a_min with rho the

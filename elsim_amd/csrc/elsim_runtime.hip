@@ -153,6 +153,13 @@ int elsim_memset(void* dst, int value, size_t bytes) {
   return ELSIM_OK;
 }
 
+int elsim_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  ELSIM_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return ELSIM_OK;
+}
+
 int elsim_timer_start(void) {
   ELSIM_TRY(ensure_init());
   ELSIM_HIP(hipEventRecord(g_ev0, g_stream));

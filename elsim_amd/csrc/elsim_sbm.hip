@@ -433,14 +433,14 @@ struct SbmState {
   SbmScal* hs = nullptr;  // pinned mirror
   // learned warm start: phi_P of the previous step, the network, its input,
   // output and the candidate field
-  DevBuf php_prev, nn_x, nn_pred, ws_cand, ws_dir, ws_r0;
+  DevBuf php_prev, phe_prev, nn_x, nn_pred, ws_cand, ws_dir, ws_dir_e, ws_r0;
   uint16_t* rec_x = nullptr;  // armed recording of the next step's network input
   double* rec_start = nullptr;
   int nprev = 0;
   NnNet* ws_net = nullptr;
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
-                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_r0};
+                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0};
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
     if (hs) (void)hipHostFree(hs);
@@ -610,6 +610,7 @@ int ensure_ws_buffers(SbmState& s) {
   ELSIM_TRY(dev_alloc(s.nn_pred, N * sizeof(float)));
   ELSIM_TRY(dev_alloc(s.ws_cand, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir, G * sizeof(double)));
+  ELSIM_TRY(dev_alloc(s.ws_dir_e, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_r0, N * sizeof(double)));
   return ELSIM_OK;
 }
@@ -628,17 +629,20 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   ELSIM_HIP(hipEventCreate(&e1));
   ELSIM_HIP(hipEventRecord(e0, stv));
 
-  // 0. warm start of the first phP solve: keep phi_P's change over the last
-  //    step (from the third step on: the first starts from the initial
-  //    fields, the second's predecessor change is that start-up transient)
+  // 0. warm start of the coupling loop: keep the potentials' change over
+  //    the last step (from the third step on: the first starts from the
+  //    initial fields, the second's predecessor change is that start-up
+  //    transient)
   const bool ws_learned = P.ws_mode == ELSIM_WS_LEARNED && (s.ws_net != nullptr || s.rec_x != nullptr);
   const bool ws_on = (P.ws_mode == ELSIM_WS_EXTRAP || ws_learned) && s.nprev >= 2;
   if (ws_on) {
     ELSIM_TRY(ensure_ws_buffers(s));
     k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.php), s.g(s.php_prev), s.g(s.ws_dir));
+    k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.phe), s.g(s.phe_prev), s.g(s.ws_dir_e));
     ELSIM_CHECK_LAUNCH();
   }
   ELSIM_HIP(hipMemcpyAsync(s.php_prev.p, s.php.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
+  ELSIM_HIP(hipMemcpyAsync(s.phe_prev.p, s.phe.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
   s.nprev++;
 
   // 1. Cahn-Hilliard (SBM:397-443)
@@ -674,7 +678,21 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   k_kfw<<<grid, blk, 0, stv>>>(d, s.g(s.cnp), s.g(s.tab), s.sz, P, s.g(s.kfw), s.g(s.kbw));
   ELSIM_CHECK_LAUNCH();
 
-  // 4. outer loop (SBM:624-812)
+  // 4. warm start: both potentials extrapolated linearly in time before the
+  //    first Butler-Volmer evaluation, so the coupling loop starts near the
+  //    step's solution (fewer outer iterations, shorter solves)
+  const double sc = 1.0 / (P.dx * P.dx);
+  BcArgs pbc0 = mirror_bc();
+  pbc0.hi = 1;
+  pbc0.hi_v = P.bvp;
+  if (ws_on) {
+    k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.ws_dir), 1.0);
+    ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(bc(s, s.g(s.php), pbc0));
+    z.ws_used = 1;
+  }
+
+  // 5. outer loop (SBM:624-812)
   double cvgp = 1.0, cvgl = 1.0;
   int nlp = 0;
   while (cvgp > P.outer_tol && cvgl > P.outer_tol) {
@@ -694,67 +712,53 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       // so re-mirroring them changes no value
       ELSIM_TRY(bc(s, s.g(s.php), only_hi));
     }
-    if (nlp == 0 && ws_on) {
-      // 1. linear extrapolation in time: x = phi + dphi (dphi's ghosts are
-      //    differences of consistent ghosts; the BC is re-applied).
-      const double sc = 1.0 / (P.dx * P.dx);
-      double r_plain = 0.0, r_ws = 0.0;
-      ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_plain));
-      k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.ws_dir), 1.0);
-      ELSIM_CHECK_LAUNCH();
-      BcArgs cbc = mirror_bc();
-      cbc.hi = 1;
-      cbc.hi_v = P.bvp;
-      ELSIM_TRY(bc(s, s.g(s.php), cbc));
-      z.ws_used = 1;
-      z.ws_alpha = 0.0;
-      if (ws_learned) {
-        // 2. learned correction: the network maps (psi_P, r0 = b - A x, the
-        //    coefficient, the last change) to the shape p of the remaining
-        //    error; the Galerkin step alpha = p.r0 / p.(-A) p along it
-        //    minimises the A-norm error (A = -L is SPD on corrections with the
-        //    search direction's homogeneous ghosts), so it never makes the
-        //    start worse in the norm CG minimises.
-        double* r0 = s.g(s.ws_r0);
-        double rr = 0.0;
-        ELSIM_TRY(launch_resid_store(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), r0, &rr));
-        const double rms_r = std::sqrt(rr / (double)d.N());
-        uint16_t* xin = s.rec_x ? s.rec_x : (uint16_t*)s.nn_x.p;
-        ELSIM_TRY(ws_features(s, r0, rms_r, xin));
-        if (s.rec_start) {
-          k_interior<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.rec_start);
-          ELSIM_CHECK_LAUNCH();
-        }
-        if (s.ws_net && rms_r > 0.0) {
-          ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p));
-          double* pd = s.g(s.ws_cand);  // correction direction, zero ghosts
-          ELSIM_HIP(hipMemsetAsync(pd, 0, (size_t)d.G() * sizeof(double), stv));
-          k_ws_apply<<<grid, blk, 0, stv>>>(d, pd, (const float*)s.nn_pred.p, 1.0, pd);
-          ELSIM_CHECK_LAUNCH();
-          double num = 0.0, den = 0.0;
-          double* scr = (double*)scratch(1, (size_t)d.N() * sizeof(double) + sizeof(double));
-          if (!scr) return ELSIM_ERR_ALLOC;
-          double* ddot = scr + d.N();
-          ELSIM_TRY(launch_dot_resid(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), pd, &num));
-          ELSIM_TRY(launch_matvec_f64(ELSIM_OP_PHP, d, pd, s.g(s.kapp), nullptr, sc, scr, ddot, true));
-          ELSIM_HIP(hipMemcpyAsync(&den, ddot, sizeof(double), hipMemcpyDeviceToHost, stv));
-          ELSIM_HIP(hipStreamSynchronize(stv));
-          const double alpha = (den < 0.0 && std::isfinite(num / den)) ? num / den : 0.0;  // den = p.L p < 0
-          if (alpha != 0.0) {
-            k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), pd, alpha);
-            ELSIM_CHECK_LAUNCH();
-            ELSIM_TRY(bc(s, s.g(s.php), cbc));
-            z.ws_used = 2;
-            z.ws_alpha = alpha;
-            z.ws_energy_gain = num * num / -den;  // decrease of the squared A-norm error
-          }
-        }
-        s.rec_x = nullptr;
-        s.rec_start = nullptr;
+    if (nlp == 0 && ws_learned && ws_on) {
+      // learned correction of the extrapolated phi_P: the network maps
+      // (psi_P, r0 = b - A x, the coefficient, the last change) to the shape
+      // p of the remaining error; the Galerkin step alpha = p.r0 / p.(-A) p
+      // along it minimises the A-norm error (A = -L is SPD on corrections
+      // with the search direction's homogeneous ghosts), so it never makes
+      // the start worse in the norm CG minimises.
+      double* r0 = s.g(s.ws_r0);
+      double rr = 0.0, r_ws = 0.0;
+      ELSIM_TRY(launch_resid_store(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), r0, &rr));
+      const double rms_r = std::sqrt(rr / (double)d.N());
+      z.ws_res_plain = std::sqrt(rr);
+      z.ws_res_nn = z.ws_res_plain;
+      uint16_t* xin = s.rec_x ? s.rec_x : (uint16_t*)s.nn_x.p;
+      ELSIM_TRY(ws_features(s, r0, rms_r, xin));
+      if (s.rec_start) {
+        k_interior<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.rec_start);
+        ELSIM_CHECK_LAUNCH();
       }
-      ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_ws));
-      z.ws_res_plain = std::sqrt(r_plain);
-      z.ws_res_nn = std::sqrt(r_ws);
+      if (s.ws_net && rms_r > 0.0) {
+        ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p));
+        double* pd = s.g(s.ws_cand);  // correction direction, zero ghosts
+        ELSIM_HIP(hipMemsetAsync(pd, 0, (size_t)d.G() * sizeof(double), stv));
+        k_ws_apply<<<grid, blk, 0, stv>>>(d, pd, (const float*)s.nn_pred.p, 1.0, pd);
+        ELSIM_CHECK_LAUNCH();
+        double num = 0.0, den = 0.0;
+        double* scr = (double*)scratch(1, (size_t)d.N() * sizeof(double) + sizeof(double));
+        if (!scr) return ELSIM_ERR_ALLOC;
+        double* ddot = scr + d.N();
+        ELSIM_TRY(launch_dot_resid(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), pd, &num));
+        ELSIM_TRY(launch_matvec_f64(ELSIM_OP_PHP, d, pd, s.g(s.kapp), nullptr, sc, scr, ddot, true));
+        ELSIM_HIP(hipMemcpyAsync(&den, ddot, sizeof(double), hipMemcpyDeviceToHost, stv));
+        ELSIM_HIP(hipStreamSynchronize(stv));
+        const double alpha = (den < 0.0 && std::isfinite(num / den)) ? num / den : 0.0;  // den = p.L p < 0
+        if (alpha != 0.0) {
+          k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), pd, alpha);
+          ELSIM_CHECK_LAUNCH();
+          ELSIM_TRY(bc(s, s.g(s.php), pbc0));
+          ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_ws));
+          z.ws_res_nn = std::sqrt(r_ws);
+          z.ws_used = 2;
+          z.ws_alpha = alpha;
+          z.ws_energy_gain = num * num / -den;  // decrease of the squared A-norm error
+        }
+      }
+      s.rec_x = nullptr;
+      s.rec_start = nullptr;
     }
     ELSIM_TRY(solve(s, ELSIM_OP_PHP, mode, &r));
     z.it_php += r.iters;
@@ -792,7 +796,7 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   z.cvgp = cvgp;
   z.cvgl = cvgl;
 
-  // 5. current control (SBM:826-835)
+  // 6. current control (SBM:826-835)
   k_red_rxn<<<grid, blk, 0, stv>>>(d, s.g(s.rxn), s.g(s.avp), R_SCRNT, s.S(), P, ws.partials);
   ELSIM_TRY(fin_sbm(s, R_SCRNT));
   k_shift<<<grid, blk, 0, stv>>>(d, s.g(s.phe), s.S(), 1, 0.0, P.dt);
@@ -839,7 +843,8 @@ int elsim_sbm_create(const int* n1, const int* n2, const int* n3, const elsim_sb
   }
   const size_t G = (size_t)d.G() * 8, N = (size_t)d.N() * 8;
   ELSIM_TRY(dev_alloc(s->tab, 8 * (size_t)*sz * 8));
-  for (DevBuf* b : {&s->psp, &s->pse, &s->cnp, &s->cne, &s->php, &s->phe, &s->kapp, &s->pce, &s->pcl, &s->php_prev})
+  for (DevBuf* b : {&s->psp, &s->pse, &s->cnp, &s->cne, &s->php, &s->phe, &s->kapp, &s->pce, &s->pcl, &s->php_prev,
+                    &s->phe_prev})
     ELSIM_TRY(dev_alloc(*b, G));
   for (DevBuf* b : {&s->avp, &s->rxn, &s->rxc, &s->rhsp, &s->rhse, &s->rhsc, &s->rhe, &s->kfw, &s->kbw, &s->mu, &s->pm})
     ELSIM_TRY(dev_alloc(*b, N));

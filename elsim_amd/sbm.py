@@ -53,11 +53,13 @@ def sphere_pack(n, seed, rmin, rmax, target=0.5, hardcore=0.85, connect=True, dc
     return sdf.reshape((n1, n2, n3), order="F"), ns.value, fr.value
 
 
-def ellipsoid_pack(n, seed, amin=8.0, amax=24.0, cone_deg=15.0, target=0.5, dcut=48.0):
+def ellipsoid_pack(n, seed, amin=8.0, amax=24.0, cone_deg=15.0, target=0.5, overlap=0.0, dcut=48.0):
+    """Interior FP32 SDF (Fortran order) of seeded random ellipsoids aligned
+    within cone_deg of d2; overlap = admitted solid fraction of a new shape."""
     n1, n2, n3 = (n, n, n) if np.isscalar(n) else n
     sdf = np.zeros(n1 * n2 * n3, np.float32)
     ns, fr = C.c_int(), C.c_double()
-    check(lib().elsim_gen_ellipsoid_pack(n1, n2, n3, seed, amin, amax, cone_deg, target, dcut, ptr(sdf),
+    check(lib().elsim_gen_ellipsoid_pack(n1, n2, n3, seed, amin, amax, cone_deg, target, overlap, dcut, ptr(sdf),
                                          C.byref(ns), C.byref(fr)), "gen_ellipsoid_pack")
     return sdf.reshape((n1, n2, n3), order="F"), ns.value, fr.value
 

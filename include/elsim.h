@@ -109,14 +109,15 @@ typedef struct elsim_sbm_params {
   int32_t ws_mode;          /* warm start of the first phP solve: ELSIM_WS_* */
 } elsim_sbm_params;
 
-/* The starting point of the first phP solve of a step (phi = phi_P at the
- * end of the last step, dphi = its change over that step), from the third
- * step on. */
+/* Warm start of a step's coupling loop (SBM:624-812), from the third step
+ * on: phi_P and phi_E are extrapolated linearly in time (phi + its change
+ * over the last step) before the first Butler-Volmer evaluation, and with a
+ * network attached the first phP solve starts from the extrapolated phi_P plus
+ * a Galerkin step along the network's prediction of the remaining error. */
 enum {
-  ELSIM_WS_PLAIN = 0,    /* phi (as the reference) */
-  ELSIM_WS_EXTRAP = 1,   /* x = phi + dphi (linear extrapolation in time) */
-  ELSIM_WS_LEARNED = 2   /* x + alpha p: p the attached network's prediction of the remaining
-                            error from x's FP64 residual, alpha = p.r0 / p.(-A) p (A-norm optimal) */
+  ELSIM_WS_PLAIN = 0,    /* the last step's fields (as the reference) */
+  ELSIM_WS_EXTRAP = 1,   /* linear extrapolation in time */
+  ELSIM_WS_LEARNED = 2   /* extrapolation + learned correction of phi_P */
 };
 
 /* Per-step statistics returned by the step entry points. */
@@ -132,9 +133,9 @@ typedef struct elsim_step_stats {
   double  cvgp, cvgl;  /* SBM:723, 804 at exit */
   double  relres_cne, relres_php, relres_phe; /* true FP64 relative residual at exit of the last solve */
   double  seconds;     /* device time of the step (hipEvent) */
-  int32_t ws_used;     /* learned warm start applied to the first phP solve */
+  int32_t ws_used;     /* 1: extrapolated start, 2: plus the learned correction */
   int32_t ws_pad_;
-  double  ws_res_plain, ws_res_nn; /* ||b - A x|| before / after it */
+  double  ws_res_plain, ws_res_nn; /* ||b - A x|| of the first phP solve before / after the correction */
   double  ws_alpha;     /* Galerkin step length along the prediction */
   double  ws_energy_gain; /* decrease of the squared A-norm error it gave */
 } elsim_step_stats;
@@ -154,6 +155,8 @@ int         elsim_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int         elsim_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int         elsim_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int         elsim_memset(void* dst, int value, size_t bytes);
+/* Free and total device memory of the current device (bytes). */
+int         elsim_mem_info(size_t* free_bytes, size_t* total_bytes);
 /* Event timer on the library's stream: start, stop -> milliseconds. */
 int         elsim_timer_start(void);
 int         elsim_timer_stop(float* ms);
@@ -353,9 +356,13 @@ int elsim_gen_sphere_pack(int n1, int n2, int n3, uint64_t seed, double rmin, do
                           double target, double hardcore, int connect, double dcut, float* sdf,
                           int* nspheres, double* frac);
 /* Random ellipsoids, semi-axes ~ U[amin,amax], major axis within `cone_deg`
- * of the d2 axis; voxel-level non-overlap; approximate SDF. */
+ * of the d2 axis (the through-thickness axis), added until the solid
+ * fraction reaches `target`; a shape is rejected when more than `overlap` of
+ * its voxels are already solid (0: voxel-level non-overlap, 1: free
+ * overlap).  Approximate SDF (1 - normalised radius) * smallest semi-axis,
+ * exact for spheres, max over shapes, clamped below at -dcut. */
 int elsim_gen_ellipsoid_pack(int n1, int n2, int n3, uint64_t seed, double amin, double amax,
-                             double cone_deg, double target, double dcut, float* sdf,
+                             double cone_deg, double target, double overlap, double dcut, float* sdf,
                              int* nshapes, double* frac);
 /* Carve a random-walk tunnel of radius `radius` through the solid from the
  * low to the high d2 face: sdf <- min(sdf, dist(path) - radius). */

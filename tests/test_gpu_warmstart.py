@@ -1,8 +1,9 @@
-"""The learned CG warm start on a microstructure outside its training set:
-it lowers the phP CG iterations of the fast solver against the plain start
-(the last step's phi_P, as the reference) and against linear extrapolation
-in time, while every solve still meets the same tolerance and the fields
-agree with the plain run."""
+"""The warm start of the coupling loop on a microstructure outside the
+network's training set: linear extrapolation in time lowers the phP CG
+iterations of the fast solver against the plain start (the last step's
+fields, as the reference), the learned correction lowers them further, and
+every solve still meets the same tolerance with fields that agree with the
+plain run."""
 import numpy as np
 import pytest
 
