@@ -200,6 +200,9 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=None)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the operator timing and a 512 MiB device copy (for counter runs)")
+    ap.add_argument("--cg-cap", type=int, default=0,
+                    help="profiling runs only: cap every CG at this many iterations and the coupling loop at one "
+                         "iteration (the printed line is then not the metric)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path has no multi-GPU mode.",
@@ -228,6 +231,8 @@ def main():
 
     if a.inner_rtol is not None:
         prm.inner_rtol = a.inner_rtol
+    if a.cg_cap > 0:
+        prm = prm.copy(max_it_e=a.cg_cap, max_it_p=a.cg_cap, max_it_l=a.cg_cap, max_outer=1)
     paths = a.paths.split(",")
     nan = float("nan")
     tfast, stf, t64, st64, ref_it, kf, k64 = nan, [], nan, [], None, {}, {}

@@ -262,74 +262,78 @@ __global__ void k_colsum_reduce(int ng, int C, const float* __restrict__ part, f
 __global__ void k_pool(int n1, int n2, int n3, int C, const uint16_t* __restrict__ fine, View fv,
                        uint16_t* __restrict__ coarse) {
   const int m1 = n1 / 2, m2 = n2 / 2, m3 = n3 / 2;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)m1 * m2 * m3 * C;
-  if (e >= n) return;
-  const int c = (int)(e % C);
-  const int64_t vc = e / C;
-  const int x = (int)(vc % m1), y = (int)((vc / m1) % m2), z = (int)(vc / ((int64_t)m1 * m2));
-  float s = 0.f;
-  for (int q = 0; q < 8; q++) {
-    const int fx = 2 * x + (q & 1), fy = 2 * y + ((q >> 1) & 1), fz = 2 * z + (q >> 2);
-    const int64_t vf = fx + (int64_t)n1 * (fy + (int64_t)n2 * fz);
-    s += bf2f(fine[vf * fv.cs + fv.co + c]);
+  const int64_t n = (int64_t)m1 * m2 * m3 * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t vc = e / C;
+    const int x = (int)(vc % m1), y = (int)((vc / m1) % m2), z = (int)(vc / ((int64_t)m1 * m2));
+    float s = 0.f;
+    for (int q = 0; q < 8; q++) {
+      const int fx = 2 * x + (q & 1), fy = 2 * y + ((q >> 1) & 1), fz = 2 * z + (q >> 2);
+      const int64_t vf = fx + (int64_t)n1 * (fy + (int64_t)n2 * fz);
+      s += bf2f(fine[vf * fv.cs + fv.co + c]);
+    }
+    coarse[vc * C + c] = f2bf(s * 0.125f);
   }
-  coarse[vc * C + c] = f2bf(s * 0.125f);
 }
 
 // dfine[child] += dcoarse / 8 (FP32, into a channel view).
 __global__ void k_pool_bwd(int n1, int n2, int n3, int C, const float* __restrict__ dcoarse, float* __restrict__ dfine,
                            View fv) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)n1 * n2 * n3 * C;
-  if (e >= n) return;
-  const int c = (int)(e % C);
-  const int64_t vf = e / C;
-  const int x = (int)(vf % n1), y = (int)((vf / n1) % n2), z = (int)(vf / ((int64_t)n1 * n2));
-  const int m1 = n1 / 2, m2 = n2 / 2;
-  const int64_t vc = x / 2 + (int64_t)m1 * (y / 2 + (int64_t)m2 * (z / 2));
-  dfine[vf * fv.cs + fv.co + c] += dcoarse[vc * C + c] * 0.125f;
+  const int64_t n = (int64_t)n1 * n2 * n3 * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t vf = e / C;
+    const int x = (int)(vf % n1), y = (int)((vf / n1) % n2), z = (int)(vf / ((int64_t)n1 * n2));
+    const int m1 = n1 / 2, m2 = n2 / 2;
+    const int64_t vc = x / 2 + (int64_t)m1 * (y / 2 + (int64_t)m2 * (z / 2));
+    dfine[vf * fv.cs + fv.co + c] += dcoarse[vc * C + c] * 0.125f;
+  }
 }
 
 // fine view [v][c] = coarse[parent][c] (bf16 copy); n* are the fine dims.
 __global__ void k_up(int n1, int n2, int n3, int C, const uint16_t* __restrict__ coarse, uint16_t* __restrict__ fine,
                      View fv) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)n1 * n2 * n3 * C;
-  if (e >= n) return;
-  const int c = (int)(e % C);
-  const int64_t vf = e / C;
-  const int x = (int)(vf % n1), y = (int)((vf / n1) % n2), z = (int)(vf / ((int64_t)n1 * n2));
-  const int m1 = n1 / 2, m2 = n2 / 2;
-  const int64_t vc = x / 2 + (int64_t)m1 * (y / 2 + (int64_t)m2 * (z / 2));
-  fine[vf * fv.cs + fv.co + c] = coarse[vc * C + c];
+  const int64_t n = (int64_t)n1 * n2 * n3 * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t vf = e / C;
+    const int x = (int)(vf % n1), y = (int)((vf / n1) % n2), z = (int)(vf / ((int64_t)n1 * n2));
+    const int m1 = n1 / 2, m2 = n2 / 2;
+    const int64_t vc = x / 2 + (int64_t)m1 * (y / 2 + (int64_t)m2 * (z / 2));
+    fine[vf * fv.cs + fv.co + c] = coarse[vc * C + c];
+  }
 }
 
 // dcoarse[v][c] = sum of the 8 children of a fine FP32 view (fixed order).
 __global__ void k_up_bwd(int n1, int n2, int n3, int C, const float* __restrict__ dfine, View fv,
                          float* __restrict__ dcoarse) {
   const int m1 = n1 / 2, m2 = n2 / 2, m3 = n3 / 2;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, n = (int64_t)m1 * m2 * m3 * C;
-  if (e >= n) return;
-  const int c = (int)(e % C);
-  const int64_t vc = e / C;
-  const int x = (int)(vc % m1), y = (int)((vc / m1) % m2), z = (int)(vc / ((int64_t)m1 * m2));
-  float s = 0.f;
-  for (int q = 0; q < 8; q++) {
-    const int fx = 2 * x + (q & 1), fy = 2 * y + ((q >> 1) & 1), fz = 2 * z + (q >> 2);
-    const int64_t vf = fx + (int64_t)n1 * (fy + (int64_t)n2 * fz);
-    s += dfine[vf * fv.cs + fv.co + c];
+  const int64_t n = (int64_t)m1 * m2 * m3 * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t vc = e / C;
+    const int x = (int)(vc % m1), y = (int)((vc / m1) % m2), z = (int)(vc / ((int64_t)m1 * m2));
+    float s = 0.f;
+    for (int q = 0; q < 8; q++) {
+      const int fx = 2 * x + (q & 1), fy = 2 * y + ((q >> 1) & 1), fz = 2 * z + (q >> 2);
+      const int64_t vf = fx + (int64_t)n1 * (fy + (int64_t)n2 * fz);
+      s += dfine[vf * fv.cs + fv.co + c];
+    }
+    dcoarse[vc * C + c] = s;
   }
-  dcoarse[vc * C + c] = s;
 }
 
 // dz[v][c] = bf16(dact[v][c] * (act[v][c] > 0)).
 __global__ void k_relu_bwd(int64_t N, int C, const float* __restrict__ dact, View dv, const uint16_t* __restrict__ act,
                            View av, uint16_t* __restrict__ dz) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= N * C) return;
-  const int c = (int)(e % C);
-  const int64_t v = e / C;
-  const uint16_t a = act[v * av.cs + av.co + c];
-  const bool pos = a != 0 && !(a & 0x8000u);
-  dz[e] = f2bf(pos ? dact[v * dv.cs + dv.co + c] : 0.f);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N * C; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t v = e / C;
+    const uint16_t a = act[v * av.cs + av.co + c];
+    const bool pos = a != 0 && !(a & 0x8000u);
+    dz[e] = f2bf(pos ? dact[v * dv.cs + dv.co + c] : 0.f);
+  }
 }
 
 // ----------------------------------------------------------- output layer --
@@ -432,6 +436,10 @@ __global__ void k_pack(int cin, int cout, const float* __restrict__ W, uint16_t*
 namespace {
 
 inline int nblk(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+// Element kernels over voxel x channel index spaces loop over a capped grid:
+// a dispatch holds at most 2^32 - 1 work-items per dimension, which one
+// thread per element would exceed at 512^3 x 32 channels.
+inline int nblk_cap(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 1 << 20); }
 
 struct LayerDef {
   int cin, cout;
@@ -558,21 +566,21 @@ struct NnNet {
     const float* B = P();
     // encoder
     ELSIM_TRY(conv<0>(a1, a2, a3, x0, 16, View{16, 0}, WT(L_E0), B + b_off[L_E0], 1, 16, C0, View{48, 32}, s));
-    k_pool<<<nblk(N1 * 16), 256, 0, s>>>(a1, a2, a3, 16, C0, View{48, 32}, (uint16_t*)p1.p);
+    k_pool<<<nblk_cap(N1 * 16), 256, 0, s>>>(a1, a2, a3, 16, C0, View{48, 32}, (uint16_t*)p1.p);
     ELSIM_TRY(conv<0>(m1, m2, m3, (uint16_t*)p1.p, 16, View{16, 0}, WT(L_E1), B + b_off[L_E1], 1, 32, C1,
                       View{96, 64}, s));
-    k_pool<<<nblk(N2 * 32), 256, 0, s>>>(m1, m2, m3, 32, C1, View{96, 64}, (uint16_t*)p2.p);
+    k_pool<<<nblk_cap(N2 * 32), 256, 0, s>>>(m1, m2, m3, 32, C1, View{96, 64}, (uint16_t*)p2.p);
     ELSIM_TRY(conv<0>(q1, q2, q3, (uint16_t*)p2.p, 32, View{32, 0}, WT(L_E2), B + b_off[L_E2], 1, 64, C2,
                       View{128, 64}, s));
-    k_pool<<<nblk(N3 * 64), 256, 0, s>>>(q1, q2, q3, 64, C2, View{128, 64}, (uint16_t*)p3.p);
+    k_pool<<<nblk_cap(N3 * 64), 256, 0, s>>>(q1, q2, q3, 64, C2, View{128, 64}, (uint16_t*)p3.p);
     ELSIM_TRY(conv<0>(r1, r2, r3, (uint16_t*)p3.p, 64, View{64, 0}, WT(L_E3), B + b_off[L_E3], 1, 64, e3.p,
                       View{64, 0}, s));
     // decoder
-    k_up<<<nblk(N2 * 64), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
+    k_up<<<nblk_cap(N2 * 64), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
     ELSIM_TRY(conv<0>(q1, q2, q3, C2, 128, View{128, 0}, WT(L_D2), B + b_off[L_D2], 1, 64, d2.p, View{64, 0}, s));
-    k_up<<<nblk(N1 * 64), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
+    k_up<<<nblk_cap(N1 * 64), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
     ELSIM_TRY(conv<0>(m1, m2, m3, C1, 96, View{96, 0}, WT(L_D1), B + b_off[L_D1], 1, 32, d1.p, View{32, 0}, s));
-    k_up<<<nblk(N0 * 32), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
+    k_up<<<nblk_cap(N0 * 32), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
     ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, View{48, 0}, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s));
     const int ob = nblk(N0);
     if (target && ob > kMaxRedBlocks) {
@@ -600,7 +608,7 @@ struct NnNet {
     const int cin = kLayers[l].cin, cout = kLayers[l].cout;
     const int64_t N = (int64_t)a1 * a2 * a3;
     uint16_t* Z = (uint16_t*)dz.p;
-    k_relu_bwd<<<nblk(N * cout), 256, 0, s>>>(N, cout, dact, dv, act, av, Z);
+    k_relu_bwd<<<nblk_cap(N * cout), 256, 0, s>>>(N, cout, dact, dv, act, av, Z);
     const int ng = (int)((N + WG_VOX - 1) / WG_VOX);
     k_wgrad<<<dim3(ng, 27), 256, 0, s>>>(a1, a2, a3, x, cin, xv, Z, cout, (float*)wpart.p);
     k_wgrad_reduce<<<nblk((int64_t)27 * cin * cout), 256, 0, s>>>(ng, cin, cout, (float*)wpart.p,
@@ -630,30 +638,30 @@ struct NnNet {
     ELSIM_TRY(layer_grads(L_D0, a1, a2, a3, C0, View{48, 0}, (float*)dd0.p, View{16, 0}, (uint16_t*)d0.p,
                           View{16, 0}));
     ELSIM_TRY(conv<1>(a1, a2, a3, Z, 16, View{16, 0}, WB(L_D0), nullptr, 0, 48, DC0, View{48, 0}, s));
-    k_up_bwd<<<nblk(N1 * 32), 256, 0, s>>>(a1, a2, a3, 32, DC0, View{48, 0}, (float*)dd1.p);
+    k_up_bwd<<<nblk_cap(N1 * 32), 256, 0, s>>>(a1, a2, a3, 32, DC0, View{48, 0}, (float*)dd1.p);
     // d1
     ELSIM_TRY(layer_grads(L_D1, m1, m2, m3, C1, View{96, 0}, (float*)dd1.p, View{32, 0}, (uint16_t*)d1.p,
                           View{32, 0}));
     ELSIM_TRY(conv<1>(m1, m2, m3, Z, 32, View{32, 0}, WB(L_D1), nullptr, 0, 96, DC1, View{96, 0}, s));
-    k_up_bwd<<<nblk(N2 * 64), 256, 0, s>>>(m1, m2, m3, 64, DC1, View{96, 0}, (float*)dd2.p);
+    k_up_bwd<<<nblk_cap(N2 * 64), 256, 0, s>>>(m1, m2, m3, 64, DC1, View{96, 0}, (float*)dd2.p);
     // d2
     ELSIM_TRY(layer_grads(L_D2, q1, q2, q3, C2, View{128, 0}, (float*)dd2.p, View{64, 0}, (uint16_t*)d2.p,
                           View{64, 0}));
     ELSIM_TRY(conv<1>(q1, q2, q3, Z, 64, View{64, 0}, WB(L_D2), nullptr, 0, 128, DC2, View{128, 0}, s));
-    k_up_bwd<<<nblk(N3 * 64), 256, 0, s>>>(q1, q2, q3, 64, DC2, View{128, 0}, (float*)de3.p);
+    k_up_bwd<<<nblk_cap(N3 * 64), 256, 0, s>>>(q1, q2, q3, 64, DC2, View{128, 0}, (float*)de3.p);
     // e3
     ELSIM_TRY(layer_grads(L_E3, r1, r2, r3, (uint16_t*)p3.p, View{64, 0}, (float*)de3.p, View{64, 0},
                           (uint16_t*)e3.p, View{64, 0}));
     ELSIM_TRY(conv<1>(r1, r2, r3, Z, 64, View{64, 0}, WB(L_E3), nullptr, 0, 64, dp3.p, View{64, 0}, s));
-    k_pool_bwd<<<nblk(N2 * 64), 256, 0, s>>>(q1, q2, q3, 64, (float*)dp3.p, DC2, View{128, 64});
+    k_pool_bwd<<<nblk_cap(N2 * 64), 256, 0, s>>>(q1, q2, q3, 64, (float*)dp3.p, DC2, View{128, 64});
     // e2
     ELSIM_TRY(layer_grads(L_E2, q1, q2, q3, (uint16_t*)p2.p, View{32, 0}, DC2, View{128, 64}, C2, View{128, 64}));
     ELSIM_TRY(conv<1>(q1, q2, q3, Z, 64, View{64, 0}, WB(L_E2), nullptr, 0, 32, dp2.p, View{32, 0}, s));
-    k_pool_bwd<<<nblk(N1 * 32), 256, 0, s>>>(m1, m2, m3, 32, (float*)dp2.p, DC1, View{96, 64});
+    k_pool_bwd<<<nblk_cap(N1 * 32), 256, 0, s>>>(m1, m2, m3, 32, (float*)dp2.p, DC1, View{96, 64});
     // e1
     ELSIM_TRY(layer_grads(L_E1, m1, m2, m3, (uint16_t*)p1.p, View{16, 0}, DC1, View{96, 64}, C1, View{96, 64}));
     ELSIM_TRY(conv<1>(m1, m2, m3, Z, 32, View{32, 0}, WB(L_E1), nullptr, 0, 16, dp1.p, View{16, 0}, s));
-    k_pool_bwd<<<nblk(N0 * 16), 256, 0, s>>>(a1, a2, a3, 16, (float*)dp1.p, DC0, View{48, 32});
+    k_pool_bwd<<<nblk_cap(N0 * 16), 256, 0, s>>>(a1, a2, a3, 16, (float*)dp1.p, DC0, View{48, 32});
     // e0
     ELSIM_TRY(layer_grads(L_E0, a1, a2, a3, x0, View{16, 0}, DC0, View{48, 32}, C0, View{48, 32}));
     ELSIM_CHECK_LAUNCH();
