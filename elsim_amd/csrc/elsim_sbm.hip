@@ -414,6 +414,13 @@ __global__ __launch_bounds__(256) void k_ws_diff(int64_t G, const double* __rest
   }
 }
 
+// out(interior) = a - b (ghosts of out untouched).
+__global__ __launch_bounds__(256) void k_diff_interior(Dims d, const double* __restrict__ a,
+                                                       const double* __restrict__ b, double* __restrict__ out) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t c, int64_t, int) { out[c] = a[c] - b[c]; });
+}
+
 // x(interior) += alpha d.
 __global__ __launch_bounds__(256) void k_ws_axpy(Dims d, double* __restrict__ x, const double* __restrict__ dv,
                                                  double alpha) {
@@ -433,14 +440,15 @@ struct SbmState {
   SbmScal* hs = nullptr;  // pinned mirror
   // learned warm start: phi_P of the previous step, the network, its input,
   // output and the candidate field
-  DevBuf php_prev, phe_prev, nn_x, nn_pred, ws_cand, ws_dir, ws_dir_e, ws_r0;
+  DevBuf php_prev, phe_prev, nn_x, nn_pred, ws_cand, ws_dir, ws_dir_e, ws_r0, outer_dp, outer_de;
   uint16_t* rec_x = nullptr;  // armed recording of the next step's network input
   double* rec_start = nullptr;
   int nprev = 0;
   NnNet* ws_net = nullptr;
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
-                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0};
+                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0, &outer_dp,
+                    &outer_de};
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
     if (hs) (void)hipHostFree(hs);
@@ -604,6 +612,32 @@ int ws_features(SbmState& s, const double* r0, double rms_r, uint16_t* x) {
   return ELSIM_OK;
 }
 
+// x(interior) += alpha dv with alpha = dv.(b - A x) / dv.(-A_hom) dv, the
+// A-norm-optimal step along dv (A = -L is SPD on corrections with the search
+// direction's homogeneous ghosts); ghosts re-applied by `bcx`.  *alpha_out
+// = 0 when dv carries no information.
+int galerkin_step(SbmState& s, int op, double* x, const double* kap, const double* rhs, const double* dv,
+                  const BcArgs& bcx, double* alpha_out) {
+  const Dims& d = s.d;
+  const double sc = 1.0 / (s.P.dx * s.P.dx);
+  double num = 0.0, den = 0.0;
+  double* scr = (double*)scratch(1, (size_t)d.N() * sizeof(double) + sizeof(double));
+  if (!scr) return ELSIM_ERR_ALLOC;
+  double* ddot = scr + d.N();
+  ELSIM_TRY(launch_dot_resid(op, d, x, kap, nullptr, sc, rhs, dv, &num));
+  ELSIM_TRY(launch_matvec_f64(op, d, dv, kap, nullptr, sc, scr, ddot, true));
+  ELSIM_HIP(hipMemcpyAsync(&den, ddot, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  const double alpha = (den < 0.0 && std::isfinite(num / den)) ? num / den : 0.0;
+  if (alpha != 0.0) {
+    k_ws_axpy<<<col_grid(d), dim3(TX, TY, 1), 0, stream()>>>(d, x, dv, alpha);
+    ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(bc(s, x, bcx));
+  }
+  *alpha_out = alpha;
+  return ELSIM_OK;
+}
+
 int ensure_ws_buffers(SbmState& s) {
   const size_t N = (size_t)s.d.N(), G = (size_t)s.d.G();
   ELSIM_TRY(dev_alloc(s.nn_x, N * NN_IN_CH * sizeof(uint16_t)));
@@ -612,6 +646,8 @@ int ensure_ws_buffers(SbmState& s) {
   ELSIM_TRY(dev_alloc(s.ws_dir, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir_e, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_r0, N * sizeof(double)));
+  ELSIM_TRY(dev_alloc(s.outer_dp, G * sizeof(double)));
+  ELSIM_TRY(dev_alloc(s.outer_de, G * sizeof(double)));
   return ELSIM_OK;
 }
 
@@ -760,6 +796,14 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       s.rec_x = nullptr;
       s.rec_start = nullptr;
     }
+    const bool outer_acc = P.ws_mode != ELSIM_WS_PLAIN && nlp >= 1;
+    if (outer_acc) {
+      // later coupling iterations: a Galerkin step along the last phP
+      // correction (successive corrections of the linearly converging
+      // coupling loop are nearly parallel)
+      double a_p = 0.0;
+      ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHP, s.g(s.php), s.g(s.kapp), s.g(s.rhsp), s.g(s.outer_dp), pbc0, &a_p));
+    }
     ELSIM_TRY(solve(s, ELSIM_OP_PHP, mode, &r));
     z.it_php += r.iters;
     z.relres_php = r.relres;
@@ -773,20 +817,33 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
                                      ws.partials);
     ELSIM_CHECK_LAUNCH();
     ELSIM_TRY(fin_sbm(s, R_CVGP));
+    if (P.ws_mode != ELSIM_WS_PLAIN) {
+      ELSIM_TRY(ensure_ws_buffers(s));
+      k_diff_interior<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.outer_dp));
+      ELSIM_CHECK_LAUNCH();
+    }
+    BcArgs ebc = mirror_bc();
+    ebc.lo = 1;
+    ebc.lo_vp = &s.S()->bve;
+    if (outer_acc) {
+      double a_e = 0.0;
+      ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHE, s.g(s.phe), s.g(s.pcl), s.g(s.rhse), s.g(s.outer_de), ebc, &a_e));
+    }
 
     ELSIM_TRY(solve(s, ELSIM_OP_PHE, mode, &r));
     z.it_phe += r.iters;
     z.relres_phe = r.relres;
     z.n_reliable += r.n_reliable;
     if (r.status) status = r.status;
-    BcArgs ebc = mirror_bc();
-    ebc.lo = 1;
-    ebc.lo_vp = &s.S()->bve;
     ELSIM_TRY(bc(s, s.g(s.phe), ebc));
     k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.phe), (const double*)s.cg.x0.p, s.g(s.pse), R_CVGL, s.S(), P,
                                      ws.partials);
     ELSIM_CHECK_LAUNCH();
     ELSIM_TRY(fin_sbm(s, R_CVGL));
+    if (P.ws_mode != ELSIM_WS_PLAIN) {
+      k_diff_interior<<<grid, blk, 0, stv>>>(d, s.g(s.phe), (const double*)s.cg.x0.p, s.g(s.outer_de));
+      ELSIM_CHECK_LAUNCH();
+    }
     ELSIM_TRY(sync_scal(s));
     cvgp = s.hs->cvgp;
     cvgl = s.hs->cvgl;
