@@ -292,16 +292,19 @@ __global__ void k_pool_bwd(int n1, int n2, int n3, int C, const float* __restric
 }
 
 // fine view [v][c] = coarse[parent][c] (bf16 copy); n* are the fine dims.
+// One thread per fine voxel and 8-channel chunk (16-byte copies; channel
+// counts, strides and offsets are multiples of 8).
 __global__ void k_up(int n1, int n2, int n3, int C, const uint16_t* __restrict__ coarse, uint16_t* __restrict__ fine,
                      View fv) {
-  const int64_t n = (int64_t)n1 * n2 * n3 * C;
+  const int nq = C / 8;
+  const int64_t n = (int64_t)n1 * n2 * n3 * nq;
+  const int m1 = n1 / 2, m2 = n2 / 2;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    const int64_t vf = e / C;
+    const int q = (int)(e % nq);
+    const int64_t vf = e / nq;
     const int x = (int)(vf % n1), y = (int)((vf / n1) % n2), z = (int)(vf / ((int64_t)n1 * n2));
-    const int m1 = n1 / 2, m2 = n2 / 2;
     const int64_t vc = x / 2 + (int64_t)m1 * (y / 2 + (int64_t)m2 * (z / 2));
-    fine[vf * fv.cs + fv.co + c] = coarse[vc * C + c];
+    *(uint4*)(fine + vf * fv.cs + fv.co + 8 * q) = *(const uint4*)(coarse + vc * C + 8 * q);
   }
 }
 
@@ -576,11 +579,11 @@ struct NnNet {
     ELSIM_TRY(conv<0>(r1, r2, r3, (uint16_t*)p3.p, 64, View{64, 0}, WT(L_E3), B + b_off[L_E3], 1, 64, e3.p,
                       View{64, 0}, s));
     // decoder
-    k_up<<<nblk_cap(N2 * 64), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
+    k_up<<<nblk_cap(N2 * 8), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
     ELSIM_TRY(conv<0>(q1, q2, q3, C2, 128, View{128, 0}, WT(L_D2), B + b_off[L_D2], 1, 64, d2.p, View{64, 0}, s));
-    k_up<<<nblk_cap(N1 * 64), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
+    k_up<<<nblk_cap(N1 * 8), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
     ELSIM_TRY(conv<0>(m1, m2, m3, C1, 96, View{96, 0}, WT(L_D1), B + b_off[L_D1], 1, 32, d1.p, View{32, 0}, s));
-    k_up<<<nblk_cap(N0 * 32), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
+    k_up<<<nblk_cap(N0 * 4), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
     ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, View{48, 0}, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s));
     const int ob = nblk(N0);
     if (target && ob > kMaxRedBlocks) {

@@ -164,6 +164,20 @@ __device__ __forceinline__ void for_col(const Dims& d, const ColIdx& ci, F&& f) 
   for (int k = ci.k0; k <= ci.k1; k++, c += sz, n += nz) f(c, n, k);
 }
 
+// Same with U items per thread (fewer blocks, so fewer reduction partials).
+#define FLAT_LOOP_U(U, n, LOADS, BODY)                                                  \
+  {                                                                                      \
+    const int64_t base_ = (int64_t)blockIdx.x * (FLAT_T * (U)) + threadIdx.x;           \
+    _Pragma("unroll") for (int u = 0; u < (U); u++) {                                    \
+      const int64_t q = base_ + u * FLAT_T;                                              \
+      if (q < (n)) { LOADS }                                                             \
+    }                                                                                    \
+    _Pragma("unroll") for (int u = 0; u < (U); u++) {                                    \
+      const int64_t q = base_ + u * FLAT_T;                                              \
+      if (q < (n)) { BODY }                                                              \
+    }                                                                                    \
+  }
+
 // Flat loop over [0, n): block b owns the contiguous tile
 // [b*FLAT_T*FLAT_U, (b+1)*FLAT_T*FLAT_U); a load phase then a compute/store
 // phase, so the FLAT_U loads of a thread are in flight together.
@@ -451,14 +465,17 @@ __global__ __launch_bounds__(256) void k_mx_start(int64_t G, const double* __res
 }
 
 // y += alpha p; r -= alpha w; rz' = r.(dinv r); decide on a reliable update.
+// UPD_U items per thread: half the blocks of the other flat kernels, so the
+// finalising kernel that follows every iteration reads half the partials.
+constexpr int UPD_U = 8;
 __global__ __launch_bounds__(256) void k_mx_update(int64_t G, float* __restrict__ y32, const float* __restrict__ p32,
                                                    float* __restrict__ r32, const float* __restrict__ w32,
                                                    const float* __restrict__ dinv, CgDev* cg, double* partials) {
   if (cg->done || cg->need_reliable) return;
   const float alp = (float)cg->alpha;
   double acc[2] = {0.0, 0.0};
-  float yv[FLAT_U], pv[FLAT_U], rv[FLAT_U], wv[FLAT_U], dv[FLAT_U];
-  FLAT_LOOP(G, { yv[u] = y32[q]; pv[u] = p32[q]; rv[u] = r32[q]; wv[u] = w32[q]; dv[u] = dinv[q]; }, {
+  float yv[UPD_U], pv[UPD_U], rv[UPD_U], wv[UPD_U], dv[UPD_U];
+  FLAT_LOOP_U(UPD_U, G, { yv[u] = y32[q]; pv[u] = p32[q]; rv[u] = r32[q]; wv[u] = w32[q]; dv[u] = dinv[q]; }, {
     y32[q] = yv[u] + alp * pv[u];
     const float r = rv[u] - alp * wv[u];
     r32[q] = r;
@@ -685,6 +702,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   const int nb64 = stencil_ntiles<double>(d), nb32 = stencil_ntiles<float>(d);
   const int64_t G = d.G();
   const int fb = flat_blocks(G);
+  const int fbu = (int)((G + (int64_t)FLAT_T * UPD_U - 1) / ((int64_t)FLAT_T * UPD_U));
   if (nb64 > kMaxRedBlocks || nb32 > kMaxRedBlocks || fb > kMaxRedBlocks) {
     set_error("grid too large for the reduction workspace");
     return ELSIM_ERR_ARG;
@@ -781,10 +799,10 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
         }));
         fin1(FIN_ALPHA_RZ, nb32);
         ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] {
-          k_mx_update<<<fb, FLAT_T, 0, s>>>(G, (float*)w.y32.p, (const float*)w.p32.p, (float*)w.r32.p,
-                                            (const float*)w.w32.p, dinv, cg, P);
+          k_mx_update<<<fbu, FLAT_T, 0, s>>>(G, (float*)w.y32.p, (const float*)w.p32.p, (float*)w.r32.p,
+                                             (const float*)w.w32.p, dinv, cg, P);
         }));
-        fin2(FIN_MX_UPDATE, fb);
+        fin2(FIN_MX_UPDATE, fbu);
         ELSIM_TRY(timed(t, ELSIM_PROF_DIR32, s, [&] {
           k_mx_dir<<<fb, FLAT_T, 0, s>>>(G, (float*)w.p32.p, (const float*)w.r32.p, dinv, cg);
         }));
