@@ -82,8 +82,9 @@ def matvec_roofline(n, reps=20):
 # Algorithmic bytes per interior voxel of the CG kernels (operands read and
 # written once).  Operator: p, kappa in, Ap out (the CnE operator also reads
 # psE: 4/8 B more).  Update: y/x, p, r, w, dinv in; y/x, r out.  Direction:
-# r, p, dinv in; p out.
-KERNEL_BYTES = {"op32": 12, "upd32": 28, "dir32": 16, "op64": 24, "upd64": 56, "dir64": 32}
+# r, p, dinv in; p out.  The mixed solver's dinv is BF16 (2 B), the FP64
+# solver's FP64.
+KERNEL_BYTES = {"op32": 12, "upd32": 26, "dir32": 14, "op64": 24, "upd64": 56, "dir64": 32}
 KERNEL_NAMES = {"op32": "k_matvec<float> (7-point operator + p.Ap)", "upd32": "k_mx_update (y += a p, r -= a w, r.z)",
                 "dir32": "k_mx_dir (p = z + b p)", "op64": "k_matvec<double>", "upd64": "k_pcg64_update",
                 "dir64": "k_pcg64_dir"}

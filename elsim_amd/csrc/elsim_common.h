@@ -171,6 +171,21 @@ __device__ __forceinline__ double lane_shift(double v) {
   return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
 
+// BF16 storage (upper half of an IEEE single, round to nearest even).  The
+// mixed solver keeps its Jacobi inverse in BF16: a preconditioner only has to
+// be a fixed symmetric definite approximation of A^-1, and BF16 keeps the
+// exponent range of the inverse diagonal (it spans ~1e-1 .. 1e7 here).
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+  union { uint32_t u; float f; } c;
+  c.u = (uint32_t)h << 16;
+  return c.f;
+}
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+  union { uint32_t u; float f; } c;
+  c.f = f;
+  return (uint16_t)((c.u + 0x7fffu + ((c.u >> 16) & 1u)) >> 16);
+}
+
 // Flat launch over n items: each block owns one contiguous tile of
 // FLAT_T * FLAT_U items (one trip, no grid-stride loop: on MI355X this keeps
 // the most bytes in flight, ~6.3 TB/s for a copy against ~4.5 TB/s with a

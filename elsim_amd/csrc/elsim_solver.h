@@ -35,7 +35,7 @@ struct CgWork {
   DevBuf res, wmx, prj, x0, r0;      // FP64: N, N, G, N, N
   DevBuf e64;                        // FP64 G (correction, ghosted for the stencil)
   DevBuf y32, r32, w32, p32;         // FP32: N, N, N, G
-  DevBuf kap32, pse32, dinv32;       // FP32: G, G, N
+  DevBuf kap32, pse32, dinv32;       // FP32: G, G; BF16 Jacobi inverse: G
   DevBuf dinv64;                     // FP64: N (ELSIM_MODE_FP64_PCG)
   DevBuf cg;                         // CgDev
   CgDev* host = nullptr;             // pinned mirror
