@@ -36,6 +36,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per ghosted element moved by the FP32 CG kernels inside the CG
+# loop at 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs over the
+# launches that ran, scaled by the counts of a device copy of known size
+# (FETCH x 2.0, WRITE x 1.0); profiles/cg_kernels_256_pmc.md.  The flat
+# kernels sweep the whole ghosted box, (n+2)^3 elements.
+PMC_BYTES_PER_GHOSTED = {"upd32": (150970.8 * 2 + 134713.9) * 1024 / 258 ** 3,
+                         "dir32": (83871.2 * 2 + 67084.0) * 1024 / 258 ** 3}
+# the phP operator covers the interior: bytes per interior voxel, same runs
+PMC_BYTES_PER_VOXEL_OP32 = (72022.2 * 2 + 67632.4) * 1024 / 256 ** 3
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -274,11 +284,15 @@ def main():
     # dominant kernel of the fast path: largest share of the CG iteration
     dom = max(kf, key=lambda k: kf[k]["us"]) if kf else None
     if dom:
+        traffic = PMC_BYTES_PER_GHOSTED[dom] * (n + 2) ** 3 if dom in PMC_BYTES_PER_GHOSTED else None
         roof = {"bound": "hbm", "achieved": kernels[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": kernels[dom]["GBps"] / HBM_PEAK_GBS, "traffic": None, "kernel": kernels[dom]["kernel"],
+                "frac": kernels[dom]["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per launch, profiles/cg_kernels_256_pmc.md",
+                "kernel": kernels[dom]["kernel"],
                 "kernel_us": kernels[dom]["us"], "bytes_per_launch": KERNEL_BYTES[dom] * N,
                 "stencil": {"kernel": kernels["op32"]["kernel"], "us": kernels["op32"]["us"],
                             "GBps": kernels["op32"]["GBps"], "frac": kernels["op32"]["GBps"] / HBM_PEAK_GBS,
+                            "traffic": PMC_BYTES_PER_VOXEL_OP32 * N, "bytes_per_launch": KERNEL_BYTES["op32"] * N,
                             "standalone_cold_us": t_mv * 1e6}}
     else:
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
