@@ -594,8 +594,15 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
     if (op == FIN_TRUE_RELIABLE) skip = cg->done || !cg->need_reliable;
   }
   if (skip) return;
-  double tot[NV];
-  sum_partials<NV>(partials, nb, tot);
+  // the operations of the two-value reductions read tot[1]; with NV = 1 it
+  // stays zero (those operations are only launched with NV = 2)
+  double tot[2] = {0.0, 0.0};
+  {
+    double t[NV];
+    sum_partials<NV>(partials, nb, t);
+#pragma unroll
+    for (int q = 0; q < NV; q++) tot[q] = t[q];
+  }
   if (linear_tid() != 0) return;
   switch (op) {
     case FIN_DOT: *dot = tot[0]; break;
