@@ -97,6 +97,8 @@ _SIGS = {
     "elsim_redist3d_xyz": (I, [IP, IP, IP, IP, P, P, DP]),
     "elsim_dev_redist3d_f32": (I, [I, I, I, I, P, P, C.c_float]),
     "elsim_dev_redist3d_f64": (I, [I, I, I, I, P, P, D]),
+    "elsim_ls_frontend": (I, [IP, IP, IP, P, IP, IP, IP, IP, DP, IP, P, P]),
+    "elsim_dev_ls_frontend": (I, [I, I, I, P, I, I, I, I, D, I, P, P]),
     "elsim_accdiff_phase": (I, [IP, IP, IP, P, IP, IP, P]),
     "elsim_dev_accdiff_phase": (I, [I, I, I, P, I, I, P]),
     "elsim_label_connected": (I, [IP, IP, IP, P, IP, P, P]),

@@ -10,7 +10,8 @@
 // reference loop over 0..p+1 does.  f ping-pongs between two buffers; the
 // reference's extra Gf and nf arrays and the nf->f copy disappear.
 // FP64 (bit-identical to the reference) and FP32 (f in FP32, sign as int8:
-// 9 bytes per voxel per sweep) variants.
+// 9 bytes per voxel per sweep) variants.  Also the geometry front-end that
+// prepares the sign function (LevelSet_3D.f90:84-144).
 #include "elsim_common.h"
 
 namespace elsim {
@@ -106,6 +107,120 @@ int redist(int lit, const Dims& d, const S* sg, T* f, T dt) {
   return ELSIM_OK;
 }
 
+// ------------------------------------------------- geometry front-end --
+// LevelSet_3D's preparation of the sign function (LS:84-144), one frame:
+// reorient the voxel image TmpA(yo,xo,zo) (LS:84-87), zero two cylindrical
+// tunnels (LS:89-108), stack the thickness copies behind a separator of sep
+// voxels (LS:76, 110-121), NoGrdBc3D, n_ac Allen-Cahn iterations
+// (LS:123-135), sign function (LS:138-143).
+
+struct FrontGeom {
+  int yo, xo, zo, sep;
+  double tr;
+};
+
+// Writes v into the interior cell (i,j,k) and into every ghost cell whose
+// clamped interior cell it is: the state NoGrdBc3D leaves (LS:164-190).
+__device__ __forceinline__ int mirror_set(int i, int n, int (&out)[3]) {
+  int c = 0;
+  out[c++] = i;
+  if (i == 1) out[c++] = 0;
+  if (i == n) out[c++] = n + 1;
+  return c;
+}
+__device__ __forceinline__ void put_mirrored(const Dims& d, double* __restrict__ f, int i, int j, int k, double v) {
+  int I[3], J[3], K[3];
+  const int ni = mirror_set(i, d.n1, I), nj = mirror_set(j, d.n2, J), nk = mirror_set(k, d.n3, K);
+  for (int c = 0; c < nk; c++)
+    for (int b = 0; b < nj; b++)
+      for (int a = 0; a < ni; a++) f[d.gi(I[a], J[b], K[c])] = v;
+}
+
+// Inside one of the two tunnels of the loop's cell (i, k) (LS:94-105), with
+// the reference's axes at (1, pz) and (py, 1).
+__device__ __forceinline__ bool in_tunnel(const Dims& d, int i, int k, double tr) {
+  const double a = (double)i - 1.0, b = (double)k - (double)d.n3;
+  if (sqrt(a * a + b * b) < tr) return true;
+  const double c = (double)i - (double)d.n1, e = (double)k - 1.0;
+  return sqrt(c * c + e * e) < tr;
+}
+
+// dsF(i,j,k) with its mirrored ghosts.  The tunnel loop (LS:91-108) runs
+// j over 1..px although TmpB's second extent is zo, so it zeroes the element
+// at the column-major offset (i-1) + yo*(j-1) + yo*zo*(k-1); the TmpB element
+// (i, jb, kb) is therefore zero if any loop cell (j0, k0) with
+// (j0-1) + zo*(k0-1) = (jb-1) + zo*(kb-1) lies in a tunnel.
+__global__ __launch_bounds__(256) void k_front_build(Dims d, FrontGeom g, const double* __restrict__ tmpa,
+                                                     double* __restrict__ dsf) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.N()) return;
+  const int i = (int)(q % d.n1) + 1, j = (int)((q / d.n1) % d.n2) + 1, k = (int)(q / ((int64_t)d.n1 * d.n2)) + 1;
+  double v = 0.0;
+  if (j > g.sep) {
+    const int r = (j - g.sep - 1) % g.zo, blk = (j - g.sep - 1) / g.zo;
+    const int jb = (blk & 1) ? g.zo - r : r + 1;
+    // TmpB(i, jb, k) = TmpA(i, k, zo - jb + 1) (LS:86)
+    v = tmpa[(int64_t)(i - 1) + (int64_t)g.yo * ((int64_t)(k - 1) + (int64_t)g.xo * (g.zo - jb))];
+    const int64_t m = (int64_t)(jb - 1) + (int64_t)g.zo * (k - 1);
+    const int64_t lo = m - (d.n2 - 1) > 0 ? (m - (d.n2 - 1) + g.zo - 1) / g.zo : 0;
+    const int64_t hi = min((int64_t)(d.n3 - 1), m / g.zo);
+    for (int64_t k0 = lo; k0 <= hi; k0++)
+      if (in_tunnel(d, i, (int)k0 + 1, g.tr)) {
+        v = 0.0;
+        break;
+      }
+  }
+  put_mirrored(d, dsf, i, j, k, v);
+}
+
+// One Allen-Cahn iteration (LS:127-133) in the reference's evaluation order
+// (REAL(4) literals 0.64 and 0.05), new value and its mirrored ghosts to dn.
+__global__ __launch_bounds__(256) void k_allen_cahn(Dims d, const double* __restrict__ f, double* __restrict__ dn) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.N()) return;
+  const int i = (int)(q % d.n1) + 1, j = (int)((q / d.n1) % d.n2) + 1, k = (int)(q / ((int64_t)d.n1 * d.n2)) + 1;
+  const int64_t c = d.gi(i, j, k), sx = d.g1(), sz = d.g12();
+  const double x = f[c];
+  const double lap = f[c + 1] + f[c - 1] + f[c + sx] + f[c - sx] + f[c + sz] + f[c - sz] - 6 * x;
+  const double sg = 2 * x * (1.0 - x) * (1.0 - 2 * x) - (double)0.64f * lap;
+  put_mirrored(d, dn, i, j, k, x - (double)0.05f * sg);
+}
+
+// sgF = +1 where dsF > 0.5, else -1, over the ghosted box (LS:138-143).
+__global__ __launch_bounds__(256) void k_front_sign(int64_t G, const double* __restrict__ f, double* __restrict__ sg) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < G) sg[q] = f[q] > 0.5 ? 1.0 : -1.0;
+}
+
+bool front_ok(int yo, int xo, int zo, int py, int px, int pz, int sep) {
+  if (yo <= 0 || xo <= 0 || zo <= 0 || !dims_ok(py, px, pz) || py > yo || pz > xo || sep < 0 || px <= sep ||
+      (px - sep) % zo != 0)
+    return false;
+  // the tunnel loop's last offset must stay inside TmpB (yo*zo*xo elements)
+  const int64_t last = (int64_t)(py - 1) + (int64_t)yo * ((int64_t)(px - 1) + (int64_t)zo * (pz - 1));
+  return last < (int64_t)yo * zo * xo;
+}
+
+// Front-end on device arrays: tmpa (yo*xo*zo), sgf and dsf ghosted; the
+// smoothing ping-pongs between dsf and sgf, the sign is written last.
+int front_end(const Dims& d, const FrontGeom& g, int n_ac, const double* tmpa, double* sgf, double* dsf) {
+  hipStream_t s = stream();
+  const int nb = (int)((d.N() + 255) / 256);
+  k_front_build<<<nb, 256, 0, s>>>(d, g, tmpa, dsf);
+  double* a = dsf;
+  double* b = sgf;
+  for (int it = 0; it < n_ac; it++) {
+    k_allen_cahn<<<nb, 256, 0, s>>>(d, a, b);
+    double* t = a;
+    a = b;
+    b = t;
+  }
+  if (a != dsf) ELSIM_HIP(hipMemcpyAsync(dsf, a, (size_t)d.G() * 8, hipMemcpyDeviceToDevice, s));
+  k_front_sign<<<(int)((d.G() + 255) / 256), 256, 0, s>>>(d.G(), dsf, sgf);
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+
 }  // namespace
 }  // namespace elsim
 
@@ -173,6 +288,34 @@ int elsim_nogrdbc3d(const int* py, const int* px, const int* pz, double* f) {
   k_mirror<<<flat_blocks(d.G()), FLAT_T, 0, s>>>(d, df);
   ELSIM_CHECK_LAUNCH();
   ELSIM_HIP(hipMemcpyAsync(f, df, G, hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  return ELSIM_OK;
+}
+
+int elsim_dev_ls_frontend(int yo, int xo, int zo, const double* tmpa, int py, int px, int pz, int sep, double tr,
+                          int n_ac, double* sgf, double* dsf) {
+  if (!tmpa || !sgf || !dsf || n_ac < 0 || !front_ok(yo, xo, zo, py, px, pz, sep)) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  return front_end(Dims{py, px, pz}, FrontGeom{yo, xo, zo, sep, tr}, n_ac, tmpa, sgf, dsf);
+}
+
+int elsim_ls_frontend(const int* yo, const int* xo, const int* zo, const double* tmpa, const int* py, const int* px,
+                      const int* pz, const int* sep, const double* tr, const int* n_ac, double* sgf, double* dsf) {
+  if (!yo || !xo || !zo || !tmpa || !py || !px || !pz || !sep || !tr || !n_ac || !sgf || !dsf || *n_ac < 0 ||
+      !front_ok(*yo, *xo, *zo, *py, *px, *pz, *sep))
+    return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  const Dims d{*py, *px, *pz};
+  const size_t G = (size_t)d.G() * 8, T = (size_t)*yo * *xo * *zo * 8;
+  double* dt = (double*)scratch(2, T);
+  double* dsg = (double*)scratch(3, G);
+  double* dds = (double*)scratch(4, G);
+  if (!dt || !dsg || !dds) return ELSIM_ERR_ALLOC;
+  hipStream_t s = stream();
+  ELSIM_HIP(hipMemcpyAsync(dt, tmpa, T, hipMemcpyHostToDevice, s));
+  ELSIM_TRY(front_end(d, FrontGeom{*yo, *xo, *zo, *sep, *tr}, *n_ac, dt, dsg, dds));
+  ELSIM_HIP(hipMemcpyAsync(sgf, dsg, G, hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipMemcpyAsync(dsf, dds, G, hipMemcpyDeviceToHost, s));
   ELSIM_HIP(hipStreamSynchronize(s));
   return ELSIM_OK;
 }

@@ -8,7 +8,8 @@
 !! and the subroutines carved out of the inline blocks of the SBM and AD
 !! programs (SBM:397-840, AccDiff_3D.f90:103-243, RM_UCNCT_3D.f90:79-82):
 !!   SBM_Init / SBM_Step / SBM_Get / SBM_Set / SBM_Free, AccDiff_Phase,
-!!   Label_Connected, RM_Threshold.
+!!   Label_Connected, RM_Threshold; and LS_FrontEnd, the inline geometry
+!!   front-end of LevelSet_3D's frame loop (LevelSet_3D.f90:84-144).
 !! A program that linked the reference subroutines links these instead.
 
 SUBROUTINE NoGrdBc3D(py, px, pz, f)
@@ -45,6 +46,19 @@ SUBROUTINE ReDist3D_XYZ_R4(lit, py, px, pz, sg, f)
    dt = 1.0e-1
    CALL elsim_check(elsim_redist3d_xyz(lit, py, px, pz, sg, f, dt), 'ReDist3D_XYZ')
 END SUBROUTINE ReDist3D_XYZ_R4
+
+!! LevelSet_3D.f90:84-144 for one tunnel radius tr: sgF and the smoothed dsF
+!! on the ghosted grid; the frame loop then sets dsF = 0 and calls
+!! ReDist3D_XYZ(175, py, px, pz, sgF, dsF).
+SUBROUTINE LS_FrontEnd(yo, xo, zo, TmpA, py, px, pz, sep, tr, nac, sgF, dsF)
+   USE elsim_gpu
+   IMPLICIT NONE
+   INTEGER, INTENT(IN) :: yo, xo, zo, py, px, pz, sep, nac
+   REAL(KIND=8), INTENT(IN), DIMENSION(yo,xo,zo) :: TmpA
+   REAL(KIND=8), INTENT(IN) :: tr
+   REAL(KIND=8), INTENT(OUT), DIMENSION(0:py+1,0:px+1,0:pz+1) :: sgF, dsF
+   CALL elsim_check(elsim_ls_frontend(yo, xo, zo, TmpA, py, px, pz, sep, tr, nac, sgF, dsF), 'LS_FrontEnd')
+END SUBROUTINE LS_FrontEnd
 
 SUBROUTINE Tbl_Chk_Pln_3DA(py, px, pz, sz, Cord, Vals, Cn, fn)
    USE elsim_gpu

@@ -55,6 +55,13 @@ MODULE elsim_gpu
          REAL(c_double), INTENT(IN) :: sg(*), dt
          REAL(c_double), INTENT(INOUT) :: f(*)
       END FUNCTION
+      INTEGER(c_int) FUNCTION elsim_ls_frontend(yo, xo, zo, tmpa, py, px, pz, sep, tr, n_ac, sgf, dsf) &
+         BIND(C, name="elsim_ls_frontend")
+         IMPORT :: c_int, c_double
+         INTEGER(c_int), INTENT(IN) :: yo, xo, zo, py, px, pz, sep, n_ac
+         REAL(c_double), INTENT(IN) :: tmpa(*), tr
+         REAL(c_double), INTENT(OUT) :: sgf(*), dsf(*)
+      END FUNCTION
       INTEGER(c_int) FUNCTION elsim_tbl_chk_pln_3da(py, px, pz, sz, cord, vals, cn, fn) &
          BIND(C, name="elsim_tbl_chk_pln_3da")
          IMPORT :: c_int, c_double

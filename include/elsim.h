@@ -181,6 +181,26 @@ int elsim_dev_redist3d_f32(int lit, int py, int px, int pz, const int8_t* sg,
                            float* f, float dt);
 int elsim_dev_redist3d_f64(int lit, int py, int px, int pz, const double* sg,
                            double* f, double dt);
+/* Replaces the geometry front-end of LevelSet_3D's frame loop (LS:84-144),
+ * the inline block between READ(20) of the voxel image and the call of
+ * ReDist3D_XYZ: reorient tmpa(yo,xo,zo) into TmpB(yo,zo,xo) (LS:84-87), zero
+ * the two cylindrical tunnels of radius tr around the d2 lines at
+ * (i,k) = (1,pz) and (py,1) (LS:89-108; the loop's TmpB(i,j,k) with j up to
+ * px beyond TmpB's second extent zo is resolved, as Fortran does without
+ * bounds checks, by column-major offset), stack (px-sep)/zo thickness copies
+ * behind a separator of sep voxels, alternately as is and mirrored in d2
+ * (LS:76, 110-121), NoGrdBc3D, n_ac Allen-Cahn smoothing iterations
+ * (LS:123-135) and the sign function (LS:138-143).  The reference: yo, xo,
+ * zo = 500, 520, 170; py, px, pz = 93, 50 + 4*170, 160; sep = 50; n_ac = 50;
+ * tr = TuRd(n) as stored (REAL(4) literals).  Outputs ghosted FP64:
+ * sgf = +-1, dsf = the smoothed field (the reference then zeroes dsF and calls
+ * ReDist3D_XYZ(175, ..., sgF, dsF)).  ELSIM_ERR_ARG unless py <= yo,
+ * pz <= xo, (px - sep) % zo == 0 and the tunnel loop stays inside TmpB. */
+int elsim_ls_frontend(const int* yo, const int* xo, const int* zo, const double* tmpa,
+                      const int* py, const int* px, const int* pz, const int* sep,
+                      const double* tr, const int* n_ac, double* sgf, double* dsf);
+int elsim_dev_ls_frontend(int yo, int xo, int zo, const double* tmpa, int py, int px,
+                          int pz, int sep, double tr, int n_ac, double* sgf, double* dsf);
 
 /* ------------------------------------------------------------------------ */
 /* Connectivity (AD:68-245, RM:79-82,128-130)                               */

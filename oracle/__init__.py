@@ -79,6 +79,21 @@ def redist3d_f32(lit, sg8, f, dt=np.float32(0.1)):
     return unflat(x, f.shape)
 
 
+def ls_frontend(tmpa, grid, sep, tr, n_ac=50):
+    """LevelSet_3D's geometry front-end (LS:84-144) on the voxel image
+    tmpa(yo, xo, zo): returns the ghosted sign field and the smoothed phase
+    field on grid = (py, px, pz)."""
+    yo, xo, zo = tmpa.shape
+    py, px, pz = grid
+    g = (py + 2) * (px + 2) * (pz + 2)
+    sg, ds = np.zeros(g), np.zeros(g)
+    rc = lib().or_ls_frontend(yo, xo, zo, p(flat(tmpa)), py, px, pz, sep, C.c_double(tr), n_ac, p(sg), p(ds))
+    if rc != 0:
+        raise ValueError("front-end geometry does not fit the voxel image")
+    gs = (py + 2, px + 2, pz + 2)
+    return unflat(sg, gs), unflat(ds, gs)
+
+
 def _ref(path):
     if not os.path.exists(path):
         return None

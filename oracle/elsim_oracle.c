@@ -18,6 +18,8 @@
  *   SBM:624-838  outer loop + current control         (or_sbm_step)
  *   SBM:987-1028 Tbl_Chk_Pln_3DA                      (or_table)
  *   SBM:1031-1090 BcMpiYZ boundary semantics on one rank (or_bc_yz: mirror)
+ *   LS:84-144    geometry front-end: reorient, tunnels, stacking, Allen-Cahn, sign
+ *                                                     (or_ls_frontend)
  *   LS:164-322   NoGrdBc3D, ReDist3D_XYZ, AbsGrd3DUW_XYZ
  *   AD:83-243    accelerated-diffusion connectivity   (or_accdiff_phase)
  *   plus a breadth-first 6-connectivity labeller      (or_label_connected)
@@ -214,6 +216,85 @@ void or_redist3d_f32(int lit, int n1, int n2, int n3, const int8_t* sg, float* f
 
 /* ======================================================================= */
 /* Accelerated-diffusion connectivity (AD:83-243) and labelling             */
+/* ======================================================================= */
+/* Geometry front-end of LevelSet_3D (LS:84-144)                           */
+/* ======================================================================= */
+
+/* One frame of the LevelSet_3D main loop up to the reinitialisation, in the
+ * reference's order: reorient the voxel image TmpA(yo,xo,zo) into
+ * TmpB(yo,zo,xo) (LS:84-87), zero two cylindrical tunnels (LS:89-108), stack
+ * the thickness copies behind a separator of sep voxels (LS:76, 110-121),
+ * NoGrdBc3D, n_ac Allen-Cahn iterations (LS:123-135) and the sign function
+ * (LS:138-143).  The reference's grid is py = 93, px = 50 + 4*170, pz = 160
+ * and its tunnel axes sit at (i,k) = (1,160) and (93,1), i.e. (1,pz) and
+ * (py,1).  The tunnel loop runs j over 1..px although TmpB's second extent is
+ * zo; Fortran does not check bounds, so the element it zeroes is the one at
+ * the column-major offset (i-1) + yo*(j-1) + yo*zo*(k-1), which this
+ * restatement writes explicitly.  sgf, dsf: ghosted (py+2)(px+2)(pz+2).
+ * Returns 0, or -1 if the tunnel loop would leave TmpB or the stacking does
+ * not fit the grid. */
+int or_ls_frontend(int yo, int xo, int zo, const double* tmpa, int py, int px, int pz, int sep, double tr,
+                   int n_ac, double* sgf, double* dsf) {
+  const int n1 = py, n2 = px, n3 = pz;
+  const size_t ntb = (size_t)yo * zo * xo;
+#define TA(i, j, k) tmpa[(size_t)((i)-1) + (size_t)yo * ((size_t)((j)-1) + (size_t)xo * (size_t)((k)-1))]
+#define TBO(i, j, k) ((size_t)((i)-1) + (size_t)yo * ((size_t)((j)-1) + (size_t)zo * (size_t)((k)-1)))
+  if (py > yo || pz > xo || sep < 0 || px <= sep || (px - sep) % zo != 0 || TBO(py, px, pz) >= ntb) return -1;
+  double* tb = (double*)malloc(ntb * sizeof(double));
+  if (!tb) return -1;
+  /* LS:84-87 */
+  for (int k = 1; k <= zo; k++)
+    for (int j = 1; j <= xo; j++)
+      for (int i = 1; i <= yo; i++) tb[TBO(i, k, j)] = TA(i, j, zo - k + 1);
+  /* LS:89-108 */
+  for (int k = 1; k <= pz; k++)
+    for (int j = 1; j <= px; j++)
+      for (int i = 1; i <= py; i++) {
+        const double a = (double)i - 1.0, b = (double)k - (double)pz;
+        double rad = sqrt(a * a + b * b);
+        if (rad < tr) tb[TBO(i, j, k)] = 0.0;
+        const double c = (double)i - (double)py, e = (double)k - 1.0;
+        rad = sqrt(c * c + e * e);
+        if (rad < tr) tb[TBO(i, j, k)] = 0.0;
+      }
+  /* LS:76, 110-121: block 0 as is, block 1 mirrored in d2, blocks 2, 3 copies of 0, 1 */
+  memset(dsf, 0, (size_t)(n1 + 2) * (n2 + 2) * (n3 + 2) * sizeof(double));
+  for (int k = 1; k <= pz; k++)
+    for (int j = sep + 1; j <= px; j++) {
+      const int r = (j - sep - 1) % zo, blk = (j - sep - 1) / zo;
+      const int jb = (blk % 2 == 0) ? r + 1 : zo - r;
+      for (int i = 1; i <= py; i++) dsf[GI(i, j, k)] = tb[TBO(i, jb, k)];
+    }
+  free(tb);
+#undef TA
+#undef TBO
+  or_nogrdbc3d(py, px, pz, dsf);
+  /* LS:123-135: sgF is the reference's work array of the smoothing */
+  const double c64 = R4(0.64), c05 = R4(0.05);
+  for (int it = 0; it < n_ac; it++) {
+    for (int k = 1; k <= pz; k++)
+      for (int j = 1; j <= px; j++)
+        for (int i = 1; i <= py; i++) {
+          const size_t q = GI(i, j, k);
+          const double d = dsf[q];
+          const double lap = dsf[GI(i + 1, j, k)] + dsf[GI(i - 1, j, k)] + dsf[GI(i, j + 1, k)] +
+                             dsf[GI(i, j - 1, k)] + dsf[GI(i, j, k + 1)] + dsf[GI(i, j, k - 1)] - 6 * d;
+          sgf[q] = 2 * d * (1.0 - d) * (1.0 - 2 * d) - c64 * lap;
+        }
+    for (int k = 1; k <= pz; k++)
+      for (int j = 1; j <= px; j++)
+        for (int i = 1; i <= py; i++) {
+          const size_t q = GI(i, j, k);
+          dsf[q] = dsf[q] - c05 * sgf[q];
+        }
+    or_nogrdbc3d(py, px, pz, dsf);
+  }
+  /* LS:138-143 */
+  const size_t G = (size_t)(n1 + 2) * (n2 + 2) * (n3 + 2);
+  for (size_t q = 0; q < G; q++) sgf[q] = dsf[q] > 0.5 ? 1.0 : -1.0;
+  return 0;
+}
+
 /* ======================================================================= */
 
 /* One phase of AccDiff_3D.  rdin interior FP64; cn interior FP64 output. */
