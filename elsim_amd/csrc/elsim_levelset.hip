@@ -2,12 +2,12 @@
 // NoGrdBc3D (LevelSet_3D.f90:164-190), AbsGrd3DUW_XYZ (LevelSet_3D.f90:265-322)
 // and ReDist3D_XYZ (LevelSet_3D.f90:194-261; RM_UCNCT_3D.f90:209-276).
 //
-// One sweep is one kernel over the ghosted box: a cell computes the Godunov
-// upwind |grad f| of its clamped interior cell (the reference evaluates the
-// gradient on the interior and then mirrors it into the ghosts with
-// NoGrdBc3D, so a ghost uses its mirror's gradient) and applies the update
-// with the sign-flip guard to itself, ghosts included, exactly as the
-// reference loop over 0..p+1 does.  f ping-pongs between two buffers; the
+// One sweep is one kernel: every interior cell computes the Godunov upwind
+// |grad f| (the reference evaluates the gradient on the interior and then
+// mirrors it into the ghosts with NoGrdBc3D, so a ghost uses its mirror's
+// gradient) and applies the update with the sign-flip guard to itself and to
+// the ghost cells that mirror it, each with its own f and sign, exactly as
+// the reference loop over 0..p+1 does.  f ping-pongs between two buffers; the
 // reference's extra Gf and nf arrays and the nf->f copy disappear.
 // FP64 (bit-identical to the reference) and FP32 (f in FP32, sign as int8:
 // 9 bytes per voxel per sweep) variants.  Also the geometry front-end that
@@ -47,24 +47,130 @@ __device__ __forceinline__ T grad_at(const Dims& d, const T* __restrict__ f, con
   return godunov<T>(f[c], f[c - 1], f[c + 1], f[c - sx], f[c + sx], f[c - sz], f[c + sz], s > T(0));
 }
 
+// The index i itself (first) and the ghost indices that mirror it in a
+// dimension of n interior cells (0 if i == 1, n + 1 if i == n).
+__device__ __forceinline__ int mirror_set(int i, int n, int (&out)[3]) {
+  int c = 0;
+  out[c++] = i;
+  if (i == 1) out[c++] = 0;
+  if (i == n) out[c++] = n + 1;
+  return c;
+}
+
+// One sweep on the register-blocked tile of the CG operator (256 d1 voxels x
+// SJ rows x SKC planes per block): each thread keeps its rows of f for the
+// planes k-1, k, k+1 in registers, takes its d1 neighbours from the adjacent
+// lanes (DPP) and loads only the rows above and below the tile and the
+// wave-edge columns.  It updates its interior cell and the ghost cells across
+// the d2 and d3 faces whose clamped interior cell it is (k_sweep_ghosts does
+// the d1 faces, where the ghost work would split the lanes of a wave), each
+// ghost with its own f and sign and the gradient of that interior cell -- the
+// reference's rule for the ghost layer (LS:236-250 over 0..p+1).  Same
+// arithmetic as grad_at.
+template <typename T>
+__device__ __forceinline__ T sweep_update(T fo, T s, T g, T dt) {
+  T v = fo + s * (T(1) - g) * dt;
+  if (!(v * fo > T(0))) v = fo + dt * T(1.0e-7) * s;
+  return v;
+}
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_sweep(Dims d, const S* __restrict__ sg, const T* __restrict__ f,
-                                               T* __restrict__ nf, T dt) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-  if (i > d.n1 + 1 || j > d.n2 + 1) return;
-  const int k0 = blockIdx.z * 8, k1 = min(k0 + 7, d.n3 + 1);
-  const int ic = min(max(i, 1), d.n1), jc = min(max(j, 1), d.n2);
-  const T tiny = T(1.0e-7);
-  for (int k = k0; k <= k1; k++) {
-    const int kc = min(max(k, 1), d.n3);
-    const T g = grad_at<T, S>(d, f, sg, d.gi(ic, jc, kc));
-    const int64_t q = d.gi(i, j, k);
-    const T fo = f[q], s = sgn_val(sg, q);
-    T v = fo + s * (T(1) - g) * dt;
-    if (!(v * fo > T(0))) v = fo + dt * tiny * s;
-    nf[q] = v;
+__global__ __launch_bounds__(256) void k_sweep_rb(Dims d, const S* __restrict__ sg, const T* __restrict__ f,
+                                                  T* __restrict__ nf, T dt) {
+  constexpr int SJ = Rows<T>::SJ;
+  const StIdx st = stencil_idx<SJ>(d, blockIdx.x);
+  const int64_t sz = d.g12();
+  const int i = st.i;
+  const bool col_ok = i <= d.n1;
+  int64_t rc[SJ];
+#pragma unroll
+  for (int t = 0; t < SJ; t++) rc[t] = d.gi(st.ic, min(st.j0 + t, d.n2 + 1), st.k0);
+  const int64_t clo = d.gi(st.ic, st.j0 - 1, st.k0), chi = d.gi(st.ic, min(st.j0 + SJ, d.n2 + 1), st.k0);
+  const bool ld_left = st.lane == 0 && col_ok, ld_right = st.lane == 63 && col_ok;
+  T fc[SJ], fm[SJ];
+#pragma unroll
+  for (int t = 0; t < SJ; t++) {
+    fc[t] = f[rc[t]];
+    fm[t] = f[rc[t] - sz];
+  }
+  int64_t off = 0;
+  for (int k = st.k0; k <= st.k1; k++, off += sz) {
+    T fp[SJ], sv[SJ];
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      fp[t] = f[rc[t] + off + sz];
+      sv[t] = sgn_val(sg, rc[t] + off);
+    }
+    const T fjlo = f[clo + off], fjhi = f[chi + off];
+    T fl[SJ], fr[SJ];
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      fl[t] = lane_shift<true>(fc[t]);
+      fr[t] = lane_shift<false>(fc[t]);
+    }
+    if (ld_left) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) fl[t] = f[rc[t] + off - 1];
+    }
+    if (ld_right) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) fr[t] = f[rc[t] + off + 1];
+    }
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      const int j = st.j0 + t;
+      if (!col_ok || j > d.n2) continue;
+      const T x = fc[t], s = sv[t];
+      const T fjm = t == 0 ? fjlo : fc[t - 1];
+      const T fjp = t == SJ - 1 ? fjhi : fc[t + 1];
+      const T g = s == T(0) ? T(0) : godunov<T>(x, fl[t], fr[t], fjm, fjp, fm[t], fp[t], s > T(0));
+      const int64_t c = rc[t] + off;
+      nf[c] = sweep_update<T>(x, s, g, dt);
+      // ghosts across d2 / d3 faces (uniform per row and plane); those across
+      // the d1 faces are k_sweep_ghosts'
+      if (j == 1 || j == d.n2 || k == 1 || k == d.n3) {
+        int J[3], K[3];
+        const int nj = mirror_set(j, d.n2, J), nk = mirror_set(k, d.n3, K);
+        for (int cc = 0; cc < nk; cc++)
+          for (int b = 0; b < nj; b++) {
+            if ((b | cc) == 0) continue;
+            const int64_t q = d.gi(i, J[b], K[cc]);
+            nf[q] = sweep_update<T>(f[q], sgn_val(sg, q), g, dt);
+          }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      fm[t] = fc[t];
+      fc[t] = fp[t];
+    }
   }
 }
+
+// The ghost cells across the two d1 faces of one sweep: one thread per
+// interior cell with i = 1 or i = n1 recomputes that cell's gradient (same
+// arithmetic as the tile kernel) and updates the ghost cells (0 or n1+1, J,
+// K) that mirror it, each with its own f and sign; k_sweep_rb updates the
+// rest of the ghost layer.  Reads the same input f; writes ghosts only.
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_sweep_ghosts(Dims d, const S* __restrict__ sg, const T* __restrict__ f,
+                                                      T* __restrict__ nf, T dt) {
+  const int64_t nf1 = d.n1 > 1 ? 2 : 1, face = (int64_t)d.n2 * d.n3;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nf1 * face) return;
+  const int i = q < face ? 1 : d.n1;
+  q %= face;
+  const int j = (int)(q % d.n2) + 1, k = (int)(q / d.n2) + 1;
+  const T g = grad_at<T, S>(d, f, sg, d.gi(i, j, k));
+  int I[3], J[3], K[3];
+  const int ni = mirror_set(i, d.n1, I), nj = mirror_set(j, d.n2, J), nk = mirror_set(k, d.n3, K);
+  for (int cc = 0; cc < nk; cc++)
+    for (int b = 0; b < nj; b++)
+      for (int a = 1; a < ni; a++) {
+        const int64_t qq = d.gi(I[a], J[b], K[cc]);
+        nf[qq] = sweep_update<T>(f[qq], sgn_val(sg, qq), g, dt);
+      }
+}
+inline int64_t d1_face_cells(const Dims& d) { return (d.n1 > 1 ? 2 : 1) * (int64_t)d.n2 * d.n3; }
 
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_absgrd(Dims d, const S* __restrict__ sg, const T* __restrict__ f,
@@ -85,8 +191,6 @@ __global__ void k_mirror(Dims d, double* __restrict__ f) {
   }
 }
 
-dim3 sweep_grid(const Dims& d) { return dim3((d.n1 + 2 + 63) / 64, (d.n2 + 2 + 3) / 4, (d.n3 + 2 + 7) / 8); }
-
 template <typename T, typename S>
 int redist(int lit, const Dims& d, const S* sg, T* f, T dt) {
   ELSIM_TRY(ensure_init());
@@ -96,8 +200,10 @@ int redist(int lit, const Dims& d, const S* sg, T* f, T dt) {
   hipStream_t s = stream();
   T* a = f;
   T* b = tmp;
+  const int nt = stencil_ntiles<T>(d);
   for (int m = 0; m < lit; m++) {
-    k_sweep<T, S><<<sweep_grid(d), dim3(64, 4, 1), 0, s>>>(d, sg, a, b, dt);
+    k_sweep_rb<T, S><<<nt, 256, 0, s>>>(d, sg, a, b, dt);
+    k_sweep_ghosts<T, S><<<(int)((d1_face_cells(d) + 255) / 256), 256, 0, s>>>(d, sg, a, b, dt);
     T* t = a;
     a = b;
     b = t;
@@ -121,13 +227,6 @@ struct FrontGeom {
 
 // Writes v into the interior cell (i,j,k) and into every ghost cell whose
 // clamped interior cell it is: the state NoGrdBc3D leaves (LS:164-190).
-__device__ __forceinline__ int mirror_set(int i, int n, int (&out)[3]) {
-  int c = 0;
-  out[c++] = i;
-  if (i == 1) out[c++] = 0;
-  if (i == n) out[c++] = n + 1;
-  return c;
-}
 __device__ __forceinline__ void put_mirrored(const Dims& d, double* __restrict__ f, int i, int j, int k, double v) {
   int I[3], J[3], K[3];
   const int ni = mirror_set(i, d.n1, I), nj = mirror_set(j, d.n2, J), nk = mirror_set(k, d.n3, K);
