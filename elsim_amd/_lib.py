@@ -98,6 +98,10 @@ _SIGS = {
     "elsim_dev_redist3d_f32": (I, [I, I, I, I, P, P, C.c_float]),
     "elsim_dev_redist3d_f64": (I, [I, I, I, I, P, P, D]),
     "elsim_ls_frontend": (I, [IP, IP, IP, P, IP, IP, IP, IP, DP, IP, P, P]),
+    "elsim_refine2_sdf": (I, [IP, IP, IP, P, P]),
+    "elsim_dev_refine2_sdf": (I, [I, I, I, P, P]),
+    "elsim_rm_phase": (I, [IP, IP, IP, P, P, IP, P]),
+    "elsim_dev_rm_phase": (I, [I, I, I, P, P, I, P]),
     "elsim_dev_ls_frontend": (I, [I, I, I, P, I, I, I, I, D, I, P, P]),
     "elsim_accdiff_phase": (I, [IP, IP, IP, P, IP, IP, P]),
     "elsim_dev_accdiff_phase": (I, [I, I, I, P, I, I, P]),

@@ -212,6 +212,24 @@ int label(const Dims& d, const double* rdin, int phase, int32_t* L, int32_t* con
   return ELSIM_OK;
 }
 
+// RM_UCNCT_3D's sign block for one phase (RM:64-96 solid, RM:115-136
+// electrolyte): psA = 0.5(1+tanh(RdIn/1.0)) (1 - psA for the electrolyte),
+// zeroed where the connectivity Cn < 1e-4f, NoGrdBc3D, then sgF = +1 where
+// the value > 0.5, else -1, over the ghosted box: a ghost takes the sign of
+// its clamped interior cell.
+template <int PHASE>
+__global__ __launch_bounds__(256) void k_rm_sign(Dims d, const double* __restrict__ rdin,
+                                                 const double* __restrict__ cn, double* __restrict__ sgf) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.G()) return;
+  const int i = (int)(q % d.g1()), j = (int)((q / d.g1()) % (d.n2 + 2)), k = (int)(q / d.g12());
+  const int64_t n = d.ni(min(max(i, 1), d.n1), min(max(j, 1), d.n2), min(max(k, 1), d.n3));
+  double ps = 0.5 * (1.0 + tanh(rdin[n] / 1.0));
+  if (PHASE == 1) ps = 1.0 - ps;
+  if (cn[n] < kThrSnap) ps = 0.0;
+  sgf[q] = ps > 0.5 ? 1.0 : -1.0;
+}
+
 }  // namespace
 }  // namespace elsim
 
@@ -283,6 +301,38 @@ int elsim_rm_threshold(const int* py, const int* px, const int* pz, const double
   k_rm<<<flat_blocks(N), FLAT_T, 0, s>>>(N, a, b, c);
   ELSIM_CHECK_LAUNCH();
   ELSIM_HIP(hipMemcpyAsync(psi_out, c, N * 8, hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  return ELSIM_OK;
+}
+
+int elsim_dev_rm_phase(int py, int px, int pz, const double* rdin, const double* cn, int phase, double* sgf) {
+  if (!dims_ok(py, px, pz) || !rdin || !cn || !sgf || (phase != 0 && phase != 1)) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  const Dims d{py, px, pz};
+  const int nb = (int)((d.G() + 255) / 256);
+  if (phase == 0)
+    k_rm_sign<0><<<nb, 256, 0, stream()>>>(d, rdin, cn, sgf);
+  else
+    k_rm_sign<1><<<nb, 256, 0, stream()>>>(d, rdin, cn, sgf);
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+
+int elsim_rm_phase(const int* py, const int* px, const int* pz, const double* rdin, const double* cn,
+                   const int* phase, double* sgf) {
+  if (!py || !px || !pz || !rdin || !cn || !phase || !sgf || !dims_ok(*py, *px, *pz)) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  const Dims d{*py, *px, *pz};
+  const size_t N = (size_t)d.N() * 8, G = (size_t)d.G() * 8;
+  double* a = (double*)scratch(3, N);
+  double* b = (double*)scratch(4, N);
+  double* c = (double*)scratch(6, G);
+  if (!a || !b || !c) return ELSIM_ERR_ALLOC;
+  hipStream_t s = stream();
+  ELSIM_HIP(hipMemcpyAsync(a, rdin, N, hipMemcpyHostToDevice, s));
+  ELSIM_HIP(hipMemcpyAsync(b, cn, N, hipMemcpyHostToDevice, s));
+  ELSIM_TRY(elsim_dev_rm_phase(*py, *px, *pz, a, b, *phase, c));
+  ELSIM_HIP(hipMemcpyAsync(sgf, c, G, hipMemcpyDeviceToHost, s));
   ELSIM_HIP(hipStreamSynchronize(s));
   return ELSIM_OK;
 }

@@ -291,6 +291,45 @@ __global__ __launch_bounds__(256) void k_front_sign(int64_t G, const double* __r
   if (q < G) sg[q] = f[q] > 0.5 ? 1.0 : -1.0;
 }
 
+// --------------------------------------------------- 2x refinement --
+// The step between RM_UCNCT_3D and the SBM program that the reference does
+// not contain (its SBM reads FP32 distance functions on a grid twice as fine,
+// SBM:138-152, 181-190): fine cell I of 2n lies at coarse coordinate
+// (I + 0.5)/2, i.e. the two fine cells of coarse cell c at c -+ 0.25.  The
+// value is the trilinear interpolation of the eight surrounding coarse cells
+// (weights 1/4 and 3/4, exact in binary; coarse indices clamped to 1..n,
+// the Neumann rule of the ghost layer), times 2 (distances in fine voxels),
+// rounded to FP32.  Interpolation order: d1, then d2, then d3.
+__device__ __forceinline__ void refine_axis(int I, int n, int& c0, int& c1, double& w0, double& w1) {
+  // fine I (1-based) -> coarse cells c0 <= c1 and weights
+  const int c = (I + 1) / 2;                  // coarse cell containing fine I
+  const bool lo = (I & 1) != 0;               // first of the two fine cells: at c - 0.25
+  c0 = lo ? c - 1 : c;
+  c1 = c0 + 1;
+  w0 = lo ? 0.25 : 0.75;
+  w1 = lo ? 0.75 : 0.25;
+  c0 = min(max(c0, 1), n);
+  c1 = min(max(c1, 1), n);
+}
+__global__ __launch_bounds__(256) void k_refine2(Dims d, const double* __restrict__ sdf, float* __restrict__ out) {
+  const int64_t m1 = 2 * (int64_t)d.n1, m2 = 2 * (int64_t)d.n2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m1 * m2 * 2 * (int64_t)d.n3) return;
+  const int I = (int)(q % m1) + 1, J = (int)((q / m1) % m2) + 1, K = (int)(q / (m1 * m2)) + 1;
+  int i0, i1, j0, j1, k0, k1;
+  double a0, a1, b0, b1, c0, c1;
+  refine_axis(I, d.n1, i0, i1, a0, a1);
+  refine_axis(J, d.n2, j0, j1, b0, b1);
+  refine_axis(K, d.n3, k0, k1, c0, c1);
+  auto at = [&](int i, int j, int k) { return sdf[d.ni(i, j, k)]; };
+  const double x00 = a0 * at(i0, j0, k0) + a1 * at(i1, j0, k0);
+  const double x10 = a0 * at(i0, j1, k0) + a1 * at(i1, j1, k0);
+  const double x01 = a0 * at(i0, j0, k1) + a1 * at(i1, j0, k1);
+  const double x11 = a0 * at(i0, j1, k1) + a1 * at(i1, j1, k1);
+  const double y0 = b0 * x00 + b1 * x10, y1 = b0 * x01 + b1 * x11;
+  out[q] = (float)(2.0 * (c0 * y0 + c1 * y1));
+}
+
 bool front_ok(int yo, int xo, int zo, int py, int px, int pz, int sep) {
   if (yo <= 0 || xo <= 0 || zo <= 0 || !dims_ok(py, px, pz) || py > yo || pz > xo || sep < 0 || px <= sep ||
       (px - sep) % zo != 0)
@@ -387,6 +426,33 @@ int elsim_nogrdbc3d(const int* py, const int* px, const int* pz, double* f) {
   k_mirror<<<flat_blocks(d.G()), FLAT_T, 0, s>>>(d, df);
   ELSIM_CHECK_LAUNCH();
   ELSIM_HIP(hipMemcpyAsync(f, df, G, hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  return ELSIM_OK;
+}
+
+int elsim_dev_refine2_sdf(int py, int px, int pz, const double* sdf, float* out) {
+  if (!dims_ok(py, px, pz) || !dims_ok(2 * py, 2 * px, 2 * pz) || !sdf || !out) return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  const Dims d{py, px, pz};
+  const int64_t M = 8 * d.N();
+  k_refine2<<<(int)((M + 255) / 256), 256, 0, stream()>>>(d, sdf, out);
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+
+int elsim_refine2_sdf(const int* py, const int* px, const int* pz, const double* sdf, float* out) {
+  if (!py || !px || !pz || !sdf || !out || !dims_ok(*py, *px, *pz) || !dims_ok(2 * *py, 2 * *px, 2 * *pz))
+    return ELSIM_ERR_ARG;
+  ELSIM_TRY(ensure_init());
+  const Dims d{*py, *px, *pz};
+  const size_t N = (size_t)d.N();
+  double* a = (double*)scratch(3, N * 8);
+  float* b = (float*)scratch(4, 8 * N * 4);
+  if (!a || !b) return ELSIM_ERR_ALLOC;
+  hipStream_t s = stream();
+  ELSIM_HIP(hipMemcpyAsync(a, sdf, N * 8, hipMemcpyHostToDevice, s));
+  ELSIM_TRY(elsim_dev_refine2_sdf(*py, *px, *pz, a, b));
+  ELSIM_HIP(hipMemcpyAsync(out, b, 8 * N * 4, hipMemcpyDeviceToHost, s));
   ELSIM_HIP(hipStreamSynchronize(s));
   return ELSIM_OK;
 }

@@ -8,8 +8,11 @@
 !! and the subroutines carved out of the inline blocks of the SBM and AD
 !! programs (SBM:397-840, AccDiff_3D.f90:103-243, RM_UCNCT_3D.f90:79-82):
 !!   SBM_Init / SBM_Step / SBM_Get / SBM_Set / SBM_Free, AccDiff_Phase,
-!!   Label_Connected, RM_Threshold; and LS_FrontEnd, the inline geometry
-!!   front-end of LevelSet_3D's frame loop (LevelSet_3D.f90:84-144).
+!!   Label_Connected, RM_Threshold; LS_FrontEnd, the inline geometry
+!!   front-end of LevelSet_3D's frame loop (LevelSet_3D.f90:84-144); RM_Phase,
+!!   the sign block of one phase of RM_UCNCT_3D (RM_UCNCT_3D.f90:64-96,
+!!   115-136); and Refine2_SDF, the 2x refinement to FP32 that the reference
+!!   runs between RM_UCNCT_3D and the SBM program but does not contain.
 !! A program that linked the reference subroutines links these instead.
 
 SUBROUTINE NoGrdBc3D(py, px, pz, f)
@@ -59,6 +62,29 @@ SUBROUTINE LS_FrontEnd(yo, xo, zo, TmpA, py, px, pz, sep, tr, nac, sgF, dsF)
    REAL(KIND=8), INTENT(OUT), DIMENSION(0:py+1,0:px+1,0:pz+1) :: sgF, dsF
    CALL elsim_check(elsim_ls_frontend(yo, xo, zo, TmpA, py, px, pz, sep, tr, nac, sgF, dsF), 'LS_FrontEnd')
 END SUBROUTINE LS_FrontEnd
+
+!! RM_UCNCT_3D.f90:64-96 (phase 0, solid) / 115-136 (phase 1, electrolyte):
+!! sgF from the L24Z distance function and that phase's connectivity; the
+!! program then sets dsF = 0 and calls ReDist3D_XYZ_R4(175, ...).
+SUBROUTINE RM_Phase(py, px, pz, RdIn, Cn, phase, sgF)
+   USE elsim_gpu
+   IMPLICIT NONE
+   INTEGER, INTENT(IN) :: py, px, pz, phase
+   REAL(KIND=8), INTENT(IN), DIMENSION(py,px,pz) :: RdIn, Cn
+   REAL(KIND=8), INTENT(OUT), DIMENSION(0:py+1,0:px+1,0:pz+1) :: sgF
+   CALL elsim_check(elsim_rm_phase(py, px, pz, RdIn, Cn, phase, sgF), 'RM_Phase')
+END SUBROUTINE RM_Phase
+
+!! GRP_LS / POR_LS (py x px x pz, FP64) -> the FP32 distance function on the
+!! 2py x 2px x 2pz grid that the SBM program reads (SBM:138-152).
+SUBROUTINE Refine2_SDF(py, px, pz, Sdf, Out)
+   USE elsim_gpu
+   IMPLICIT NONE
+   INTEGER, INTENT(IN) :: py, px, pz
+   REAL(KIND=8), INTENT(IN), DIMENSION(py,px,pz) :: Sdf
+   REAL(KIND=4), INTENT(OUT), DIMENSION(2*py,2*px,2*pz) :: Out
+   CALL elsim_check(elsim_refine2_sdf(py, px, pz, Sdf, Out), 'Refine2_SDF')
+END SUBROUTINE Refine2_SDF
 
 SUBROUTINE Tbl_Chk_Pln_3DA(py, px, pz, sz, Cord, Vals, Cn, fn)
    USE elsim_gpu

@@ -62,6 +62,18 @@ MODULE elsim_gpu
          REAL(c_double), INTENT(IN) :: tmpa(*), tr
          REAL(c_double), INTENT(OUT) :: sgf(*), dsf(*)
       END FUNCTION
+      INTEGER(c_int) FUNCTION elsim_rm_phase(py, px, pz, rdin, cn, phase, sgf) BIND(C, name="elsim_rm_phase")
+         IMPORT :: c_int, c_double
+         INTEGER(c_int), INTENT(IN) :: py, px, pz, phase
+         REAL(c_double), INTENT(IN) :: rdin(*), cn(*)
+         REAL(c_double), INTENT(OUT) :: sgf(*)
+      END FUNCTION
+      INTEGER(c_int) FUNCTION elsim_refine2_sdf(py, px, pz, sdf, out) BIND(C, name="elsim_refine2_sdf")
+         IMPORT :: c_int, c_double, c_float
+         INTEGER(c_int), INTENT(IN) :: py, px, pz
+         REAL(c_double), INTENT(IN) :: sdf(*)
+         REAL(c_float), INTENT(OUT) :: out(*)
+      END FUNCTION
       INTEGER(c_int) FUNCTION elsim_tbl_chk_pln_3da(py, px, pz, sz, cord, vals, cn, fn) &
          BIND(C, name="elsim_tbl_chk_pln_3da")
          IMPORT :: c_int, c_double

@@ -181,6 +181,16 @@ int elsim_dev_redist3d_f32(int lit, int py, int px, int pz, const int8_t* sg,
                            float* f, float dt);
 int elsim_dev_redist3d_f64(int lit, int py, int px, int pz, const double* sg,
                            double* f, double dt);
+/* The step the reference does not contain between RM_UCNCT_3D (distance
+ * functions on py x px x pz, FP64, RM:99-107) and the SBM program (FP32
+ * distance functions on a grid twice as fine, SBM:138-152, 181-190): no
+ * reference interface exists.  out (2py x 2px x 2pz, FP32, interior,
+ * column-major) = 2 x the trilinear interpolation of sdf (interior FP64) at
+ * the fine cell centres (the two fine cells of coarse cell c at c -+ 0.25,
+ * weights 1/4, 3/4; coarse indices clamped to the box, the Neumann rule),
+ * i.e. the distance in fine voxels. */
+int elsim_refine2_sdf(const int* py, const int* px, const int* pz, const double* sdf, float* out);
+int elsim_dev_refine2_sdf(int py, int px, int pz, const double* sdf, float* out);
 /* Replaces the geometry front-end of LevelSet_3D's frame loop (LS:84-144),
  * the inline block between READ(20) of the voxel image and the call of
  * ReDist3D_XYZ: reorient tmpa(yo,xo,zo) into TmpB(yo,zo,xo) (LS:84-87), zero
@@ -228,6 +238,17 @@ int elsim_dev_label_connected(int py, int px, int pz, const double* rdin, int ph
 /* RM:79-82 / RM:128-130: psi_out = (cn < 1e-4f) ? 0 : psi_in (interior). */
 int elsim_rm_threshold(const int* py, const int* px, const int* pz, const double* cn,
                        const double* psi_in, double* psi_out);
+/* Replaces RM_UCNCT_3D's sign block of one phase (RM:64-96 solid, phase 0;
+ * RM:115-136 electrolyte, phase 1): psA = 0.5(1+tanh(rdin/1.0)) (1 - psA for
+ * the electrolyte), zeroed where cn < 1e-4f, NoGrdBc3D, sgf = +1 where > 0.5
+ * else -1 over the ghosted box.  rdin, cn interior FP64 (the L24Z distance
+ * function and the AccDiff connectivity of that phase); sgf ghosted FP64.
+ * The program then sets dsF = 0 and calls ReDist3D_XYZ(175, ..., sgF, dsF)
+ * with its REAL(4) time step to write GRP_LS / POR_LS. */
+int elsim_rm_phase(const int* py, const int* px, const int* pz, const double* rdin,
+                   const double* cn, const int* phase, double* sgf);
+int elsim_dev_rm_phase(int py, int px, int pz, const double* rdin, const double* cn, int phase,
+                       double* sgf);
 
 /* ------------------------------------------------------------------------ */
 /* SBM solver core (SBM:143-838)                                             */

@@ -94,6 +94,22 @@ def ls_frontend(tmpa, grid, sep, tr, n_ac=50):
     return unflat(sg, gs), unflat(ds, gs)
 
 
+def rm_phase(rdin, cn, phase):
+    """RM_UCNCT_3D's sign block of one phase (RM:64-96, 115-136)."""
+    n1, n2, n3 = rdin.shape
+    sg = np.zeros((n1 + 2) * (n2 + 2) * (n3 + 2))
+    lib().or_rm_phase(n1, n2, n3, p(flat(rdin)), p(flat(cn)), phase, p(sg))
+    return unflat(sg, (n1 + 2, n2 + 2, n3 + 2))
+
+
+def refine2(sdf):
+    """The 2x refinement to FP32 between RM_UCNCT_3D and the SBM program."""
+    n1, n2, n3 = sdf.shape
+    out = np.zeros(8 * n1 * n2 * n3, np.float32)
+    lib().or_refine2(n1, n2, n3, p(flat(sdf)), p(out))
+    return unflat(out, (2 * n1, 2 * n2, 2 * n3))
+
+
 def _ref(path):
     if not os.path.exists(path):
         return None

@@ -18,6 +18,8 @@
  *   SBM:624-838  outer loop + current control         (or_sbm_step)
  *   SBM:987-1028 Tbl_Chk_Pln_3DA                      (or_table)
  *   SBM:1031-1090 BcMpiYZ boundary semantics on one rank (or_bc_yz: mirror)
+ *   RM:64-136    sign block of one RM phase            (or_rm_phase)
+ *                the 2x refinement to the SBM grid (not in the reference) (or_refine2)
  *   LS:84-144    geometry front-end: reorient, tunnels, stacking, Allen-Cahn, sign
  *                                                     (or_ls_frontend)
  *   LS:164-322   NoGrdBc3D, ReDist3D_XYZ, AbsGrd3DUW_XYZ
@@ -216,6 +218,63 @@ void or_redist3d_f32(int lit, int n1, int n2, int n3, const int8_t* sg, float* f
 
 /* ======================================================================= */
 /* Accelerated-diffusion connectivity (AD:83-243) and labelling             */
+/* ======================================================================= */
+/* RM_UCNCT_3D sign block (RM:64-96, 115-136) and the 2x refinement         */
+/* ======================================================================= */
+
+/* One phase of RM_UCNCT_3D before its reinitialisation: psA = 0.5(1 +
+ * tanh(RdIn/1.0)) (RM:66-67; 1 - psA for the electrolyte, RM:115), psB = 0
+ * where Cn < 1e-4 (REAL(4), RM:80, 128), dsF = psB, NoGrdBc3D, sgF = +-1 by
+ * dsF > 0.5 (RM:92-96, 133-137).  sgf ghosted. */
+void or_rm_phase(int n1, int n2, int n3, const double* rdin, const double* cn, int phase, double* sgf) {
+  double* ds = (double*)calloc((size_t)(n1 + 2) * (n2 + 2) * (n3 + 2), sizeof(double));
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) {
+        const size_t n = NI(i, j, k);
+        double ps = 0.5 * (1.0 + tanh(rdin[n] / 1.0));
+        if (phase == 1) ps = 1.0 - ps;
+        if (cn[n] < R4(1.0e-4)) ps = 0.0;
+        ds[GI(i, j, k)] = ps;
+      }
+  or_nogrdbc3d(n1, n2, n3, ds);
+  const size_t G = (size_t)(n1 + 2) * (n2 + 2) * (n3 + 2);
+  for (size_t q = 0; q < G; q++) sgf[q] = ds[q] > 0.5 ? 1.0 : -1.0;
+  free(ds);
+}
+
+/* The 2x refinement between RM_UCNCT_3D and the SBM program (not in the
+ * reference): fine cell I of 2n at coarse coordinate (I + 0.5)/2, trilinear
+ * from the eight surrounding coarse cells (weights 1/4, 3/4, indices clamped
+ * to 1..n), interpolated along d1, then d2, then d3, times 2, to FP32. */
+static void refine_axis(int I, int n, int* c0, int* c1, double* w0, double* w1) {
+  const int c = (I + 1) / 2, lo = I & 1;
+  int a = lo ? c - 1 : c, b = a + 1;
+  *w0 = lo ? 0.25 : 0.75;
+  *w1 = lo ? 0.75 : 0.25;
+  *c0 = a < 1 ? 1 : (a > n ? n : a);
+  *c1 = b < 1 ? 1 : (b > n ? n : b);
+}
+void or_refine2(int n1, int n2, int n3, const double* sdf, float* out) {
+  const int m1 = 2 * n1, m2 = 2 * n2, m3 = 2 * n3;
+  for (int K = 1; K <= m3; K++)
+    for (int J = 1; J <= m2; J++)
+      for (int I = 1; I <= m1; I++) {
+        int i0, i1, j0, j1, k0, k1;
+        double a0, a1, b0, b1, c0, c1;
+        refine_axis(I, n1, &i0, &i1, &a0, &a1);
+        refine_axis(J, n2, &j0, &j1, &b0, &b1);
+        refine_axis(K, n3, &k0, &k1, &c0, &c1);
+        const double x00 = a0 * sdf[NI(i0, j0, k0)] + a1 * sdf[NI(i1, j0, k0)];
+        const double x10 = a0 * sdf[NI(i0, j1, k0)] + a1 * sdf[NI(i1, j1, k0)];
+        const double x01 = a0 * sdf[NI(i0, j0, k1)] + a1 * sdf[NI(i1, j0, k1)];
+        const double x11 = a0 * sdf[NI(i0, j1, k1)] + a1 * sdf[NI(i1, j1, k1)];
+        const double y0 = b0 * x00 + b1 * x10, y1 = b0 * x01 + b1 * x11;
+        out[(size_t)(I - 1) + (size_t)m1 * ((size_t)(J - 1) + (size_t)m2 * (size_t)(K - 1))] =
+            (float)(2.0 * (c0 * y0 + c1 * y1));
+      }
+}
+
 /* ======================================================================= */
 /* Geometry front-end of LevelSet_3D (LS:84-144)                           */
 /* ======================================================================= */
