@@ -118,6 +118,7 @@ _SIGS = {
     "elsim_sbm_set_params": (I, [C.POINTER(C.c_int64), C.POINTER(SbmParams)]),
     "elsim_sbm_step": (I, [C.POINTER(C.c_int64), IP, C.POINTER(StepStats)]),
     "elsim_sbm_ch_update": (I, [C.POINTER(C.c_int64)]),
+    "elsim_sbm_time_control": (I, [DP, DP, DP, DP, DP, DP]),
     "elsim_sbm_matvec": (I, [C.POINTER(C.c_int64), IP, P, P]),
     "elsim_dev_matvec_f64": (I, [I, I, I, I, P, P, P, D, P, P]),
     "elsim_dev_matvec_stored_f64": (I, [I, I, I, I, P, P, P, D, P]),

@@ -1,5 +1,6 @@
 // elsim_runtime.hip -- device selection, the library stream, error state,
 // device memory and timers of libelsim_hip.so.
+#include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
@@ -223,6 +224,41 @@ void elsim_sbm_default_params(elsim_sbm_params* p) {
   p->max_it_p = 30000;
   p->max_it_l = 100000;
   p->max_outer = 1000;
+}
+
+// The time-loop control of the SBM program (SBM:358-383) for the next step:
+// the voltage scan rate in five tiers of the current error, the time step in
+// three tiers of the lithiation fraction, each tier scaling the rate again.
+// Literals are default REAL in the reference: comparisons use their REAL(4)
+// values and dt is the REAL(4) product, promoted.
+int elsim_sbm_time_control(const double* xrf, const double* scrnt, const double* trgi, const double* vsr0,
+                           double* dt, double* vsr) {
+  if (!xrf || !scrnt || !trgi || !vsr0 || !dt || !vsr) return ELSIM_ERR_ARG;
+  const double dc = std::fabs(*trgi - *scrnt), at = std::fabs(*trgi);
+  double v;
+  if (dc < (double)0.07f * at)
+    v = *vsr0 * 0.5;
+  else if (dc < (double)0.20f * at)
+    v = *vsr0;
+  else if (dc < (double)0.40f * at)
+    v = *vsr0 * 3.0;
+  else if (dc < (double)0.75f * at)
+    v = *vsr0 * 5.0;
+  else
+    v = *vsr0 * 8.0;
+  const float base = 1.05625e-02f * 0.125f;
+  if (*xrf < (double)0.12f) {
+    *dt = (double)(base * 0.75f);
+    v = v * 2;
+  } else if (*xrf < (double)0.20f) {
+    *dt = (double)(base * 0.5f);
+    v = v * 10;
+  } else {
+    *dt = (double)(base * 0.5f);
+    v = v * 15;
+  }
+  *vsr = v;
+  return ELSIM_OK;
 }
 
 }  // extern "C"

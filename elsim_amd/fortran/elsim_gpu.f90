@@ -138,6 +138,17 @@ MODULE elsim_gpu
          REAL(c_double), INTENT(OUT) :: tpp, tpe, xrf, bve
          INTEGER(c_int), INTENT(OUT) :: iter
       END FUNCTION
+      INTEGER(c_int) FUNCTION elsim_sbm_set_params(handle, prm) BIND(C, name="elsim_sbm_set_params")
+         IMPORT :: c_int, c_int64_t, elsim_sbm_params
+         INTEGER(c_int64_t), INTENT(IN) :: handle
+         TYPE(elsim_sbm_params), INTENT(IN) :: prm
+      END FUNCTION
+      INTEGER(c_int) FUNCTION elsim_sbm_time_control(xrf, scrnt, trgi, vsr0, dt, vsr) &
+         BIND(C, name="elsim_sbm_time_control")
+         IMPORT :: c_int, c_double
+         REAL(c_double), INTENT(IN) :: xrf, scrnt, trgi, vsr0
+         REAL(c_double), INTENT(OUT) :: dt, vsr
+      END FUNCTION
    END INTERFACE
 
 CONTAINS

@@ -22,6 +22,17 @@ def reference_params():
     return p
 
 
+def time_control(xrf, scrnt, trgi, vsr0):
+    """dt and Vsr of the next step from the reference's time-loop control
+    (SBM:358-383): Vsr0 scaled by five tiers of |trgI - sCrnt| / |trgI|, dt and
+    a further factor by three tiers of Xrf."""
+    dt, vsr = C.c_double(), C.c_double()
+    d = C.c_double
+    check(lib().elsim_sbm_time_control(C.byref(d(xrf)), C.byref(d(scrnt)), C.byref(d(trgi)), C.byref(d(vsr0)),
+                                       C.byref(dt), C.byref(vsr)), "time_control")
+    return dt.value, vsr.value
+
+
 def synthetic_params(**kw):
     """Nondimensional parameters of the synthetic electrochemical step (h = 1
     voxel): psi = 0.5(1+tanh(d/2)), alpha = 0.5, F/RT = 1, dt = 1e-3, every CG to

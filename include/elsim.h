@@ -67,7 +67,7 @@ extern "C" {
  * elsim_amd/sbm.py (ctypes) and elsim_amd/fortran/elsim_gpu.f90 (bind(C)). */
 typedef struct elsim_sbm_params {
   double dx;          /* SBM:90 */
-  double dt;          /* SBM:91 (fixed; the adaptive dt of SBM:374-383 is host time-loop control) */
+  double dt;          /* SBM:91; per step from elsim_sbm_time_control in the reference's loop (SBM:374-383) */
   double zeta;        /* SBM:94  psi = 0.5(1+tanh(d/zeta)) */
   double psp_off;     /* SBM:247 */
   double pse_off;     /* SBM:258 */
@@ -289,6 +289,14 @@ int elsim_sbm_set_params(const int64_t* handle, const elsim_sbm_params* prm);
 /* One electrochemical step SBM:397-840 (CH -> CnE solve -> outer
  * {BV -> phP -> phE} -> current control).  mode = ELSIM_MODE_*. */
 int elsim_sbm_step(const int64_t* handle, const int* mode, elsim_step_stats* stats);
+/* The time-loop control of the SBM program before each step (SBM:358-383):
+ * from the lithiation fraction xrf, the last step's current scrnt, the target
+ * current trgi and the base scan rate vsr0 (the reference's Vsr0, SBM:112),
+ * the time step dt and scan rate vsr of the next step, with the reference's
+ * REAL(4) literals.  Host logic; the caller puts dt and vsr into the step's
+ * parameters (elsim_sbm_set_params). */
+int elsim_sbm_time_control(const double* xrf, const double* scrnt, const double* trgi, const double* vsr0,
+                           double* dt, double* vsr);
 /* The Cahn-Hilliard stage alone (SBM:397-443). */
 int elsim_sbm_ch_update(const int64_t* handle);
 /* Homogeneous operator of the state's current coefficients applied to a

@@ -94,6 +94,14 @@ def ls_frontend(tmpa, grid, sep, tr, n_ac=50):
     return unflat(sg, gs), unflat(ds, gs)
 
 
+def time_control(xrf, scrnt, trgi, vsr0):
+    """The SBM program's dt and Vsr for the next step (SBM:358-383)."""
+    dt, vsr = C.c_double(), C.c_double()
+    lib().or_time_control(C.c_double(xrf), C.c_double(scrnt), C.c_double(trgi), C.c_double(vsr0), C.byref(dt),
+                          C.byref(vsr))
+    return dt.value, vsr.value
+
+
 def rm_phase(rdin, cn, phase):
     """RM_UCNCT_3D's sign block of one phase (RM:64-96, 115-136)."""
     n1, n2, n3 = rdin.shape
@@ -235,6 +243,10 @@ class OracleSbm:
 
     def ch_update(self):
         lib().or_ch_update(self.h)
+
+    def set_params(self, params):
+        self.params = params
+        lib().or_sbm_set_params(self.h, C.byref(params))
 
     def scalars(self):
         v = [C.c_double() for _ in range(4)]

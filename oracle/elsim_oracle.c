@@ -18,6 +18,7 @@
  *   SBM:624-838  outer loop + current control         (or_sbm_step)
  *   SBM:987-1028 Tbl_Chk_Pln_3DA                      (or_table)
  *   SBM:1031-1090 BcMpiYZ boundary semantics on one rank (or_bc_yz: mirror)
+ *   SBM:358-383  time-loop control: dt, Vsr            (or_time_control)
  *   RM:64-136    sign block of one RM phase            (or_rm_phase)
  *                the 2x refinement to the SBM grid (not in the reference) (or_refine2)
  *   LS:84-144    geometry front-end: reorient, tunnels, stacking, Allen-Cahn, sign
@@ -218,6 +219,39 @@ void or_redist3d_f32(int lit, int n1, int n2, int n3, const int8_t* sg, float* f
 
 /* ======================================================================= */
 /* Accelerated-diffusion connectivity (AD:83-243) and labelling             */
+/* ======================================================================= */
+/* Time-loop control (SBM:358-383)                                          */
+/* ======================================================================= */
+
+/* Vsr by the current error, then dt and a further factor by Xrf; default-REAL
+ * literals (0.07, 0.20, 0.40, 0.75, 0.12, 0.20 and the dt product) in REAL(4). */
+void or_time_control(double xrf, double scrnt, double trgi, double vsr0, double* dt, double* vsr) {
+  const double dcrnt = fabs(trgi - scrnt);
+  double v;
+  if (dcrnt < R4(0.07) * fabs(trgi))
+    v = vsr0 * 0.5;
+  else if (dcrnt < R4(0.20) * fabs(trgi))
+    v = vsr0;
+  else if (dcrnt < R4(0.40) * fabs(trgi))
+    v = vsr0 * 3.0;
+  else if (dcrnt < R4(0.75) * fabs(trgi))
+    v = vsr0 * 5.0;
+  else
+    v = vsr0 * 8.0;
+  volatile float a = 1.05625e-02f, b = 0.125f, c = 0.75f, h = 0.5f;
+  if (xrf < R4(0.12)) {
+    *dt = (double)((float)(a * b) * c);
+    v = v * 2;
+  } else if (xrf < R4(0.20)) {
+    *dt = (double)((float)(a * b) * h);
+    v = v * 10;
+  } else {
+    *dt = (double)((float)(a * b) * h);
+    v = v * 15;
+  }
+  *vsr = v;
+}
+
 /* ======================================================================= */
 /* RM_UCNCT_3D sign block (RM:64-96, 115-136) and the 2x refinement         */
 /* ======================================================================= */
@@ -885,6 +919,7 @@ int or_sbm_set(void* h, int f, const double* in) {
   if (!p) return -1;
   size_t n = g ? (size_t)(s->n1 + 2) * (s->n2 + 2) * (s->n3 + 2) : (size_t)s->n1 * s->n2 * s->n3;
   memcpy(p, in, n * sizeof(double));
+  if (f == ELSIM_F_CNP) s->xrf = sum_gw(s, s->cnp, s->psp) / s->tpp; /* SBM:300-303 */
   return 0;
 }
 

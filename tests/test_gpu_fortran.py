@@ -21,7 +21,11 @@ def namelist(path, run, prm):
     with open(path, "w") as f:
         f.write("&run\n")
         for k, v in run.items():
-            f.write(f"  {k} = {v!r},\n" if isinstance(v, str) else f"  {k} = {v},\n")
+            if isinstance(v, bool):
+                v = ".TRUE." if v else ".FALSE."
+            elif isinstance(v, str):
+                v = repr(v)
+            f.write(f"  {k} = {v},\n")
         f.write("/\n&params\n")
         for k, v in prm.as_dict().items():
             f.write(f"  prm%{k} = {v!r},\n")
@@ -56,3 +60,96 @@ def test_sbm_run_matches_oracle(tmp_path):
         assert e <= 1e-6, (ff, e)
     line = open(tmp_path / "GBQ105TmFx.txt").read().split()
     assert len(line) == 7 and int(line[1]) == steps
+
+
+def _r4(x):
+    return float(np.float32(x))
+
+
+def oracle_time_loop(o, prm, nsteps, cr=6.0):
+    """The reference's time loop (SBM:341-877) over the oracle's step, with the
+    oracle's time-loop control: the TmFx lines and the frames it writes."""
+    f = np.float32
+    sc = o.scalars()
+    xrf, bve = sc["xrf"], sc["bve"]
+    trgi = _r4(0.95) * sc["tpp"] * prm.rho / (3600 / cr)
+    vsr0, scrnt, cnt, k, tm = prm.vsr, trgi, 1, 0, 0.0
+    lines, frames, stop = [], {}, None
+
+    def frame(fr):
+        frames[fr] = {fid: o.get(fid)[1:-1, 1:-1, 1:-1] for fid in (F_CNP, F_CNE, F_PHP, F_PHE)}
+
+    while xrf < _r4(0.971):
+        if k >= nsteps:
+            lines.append((cnt, k, tm, xrf, scrnt, trgi, bve))
+            frame(cnt)
+            stop = "cap"
+            break
+        k += 1
+        prm.dt, prm.vsr = O.time_control(xrf, scrnt, trgi, vsr0)
+        o.set_params(prm)
+        st = o.step()
+        sc = o.scalars()
+        xrf, bve, scrnt = sc["xrf"], sc["bve"], st.scrnt
+        tm += prm.dt
+        if xrf > float(f(f(0.02) + f(f(cnt - 1) * f(0.005)))):
+            if cnt % 10 == 1:
+                frame(cnt)
+            lines.append((cnt, k, tm, xrf, scrnt, trgi, bve))
+            cnt += 1
+        if bve >= prm.bvp + _r4(0.005) or xrf > _r4(0.97):
+            lines.append((cnt, k, tm, xrf, scrnt, trgi, bve))
+            frame(cnt)
+            stop = "test"
+            break
+    return lines, frames, stop
+
+
+@pytest.mark.parametrize("case", ["cadence_and_cap", "stop_test"])
+def test_sbm_run_adaptive_matches_oracle(tmp_path, case):
+    """sbm_run with adaptive = T against the same loop over the oracle: the
+    TmFx lines (frame, iteration, time, Xrf, current, target, BvE) and every
+    frame's fields."""
+    assert os.path.exists(EXE), "Fortran driver not built"
+    n = 20
+    sdf, _, _ = E.sphere_pack(n, 3, 4.0, 8.0)
+    prm = E.synthetic_params(vsr=0.02)
+    tab = E.tables()
+    if case == "cadence_and_cap":
+        cnp, nsteps = E.initial_cnp(n, 2), 3
+    else:  # starts just above the stop test's Xrf > 0.97
+        cnp, nsteps = E.initial_cnp(n, 2, mean=0.9703, amp=1e-4), 5
+    fileio.write_record(tmp_path / "ds.dat", sdf, np.float32)
+    fileio.write_record(tmp_path / "dl.dat", -sdf, np.float32)
+    fileio.write_record(tmp_path / "tab.dat", tab.reshape((101, 8), order="F"), np.float64)
+    fileio.write_record(tmp_path / "cnp.dat", cnp, np.float64)
+    namelist(tmp_path / "run.nml", dict(n1=n, n2=n, n3=n, nsteps=nsteps, mode=E.MODE_MIXED,
+                                        sdf_s=str(tmp_path / "ds.dat"), sdf_l=str(tmp_path / "dl.dat"),
+                                        cnp_file=str(tmp_path / "cnp.dat"), tables_file=str(tmp_path / "tab.dat"),
+                                        outdir=str(tmp_path), outpf="GBQ105", adaptive=True), prm)
+    r = subprocess.run([EXE, str(tmp_path / "run.nml")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = O.OracleSbm(sdf, -sdf, E.synthetic_params(vsr=0.02), tab)
+    o.set(F_CNP, cnp)
+    lines, frames, stop = oracle_time_loop(o, o.params, nsteps)
+    assert stop == ("cap" if case == "cadence_and_cap" else "test")
+    # list-directed records of 7 items (the compiler may wrap a record over lines)
+    tok = open(tmp_path / "GBQ105TmFx.txt").read().split()
+    assert len(tok) % 7 == 0, tok
+    got = [tok[i:i + 7] for i in range(0, len(tok), 7)]
+    assert len(got) == len(lines), (got, lines)
+    for g, w in zip(got, lines):
+        assert (int(g[0]), int(g[1])) == w[:2]
+        v = [float(x) for x in g[2:]]
+        assert v[0] == pytest.approx(w[2], rel=1e-14)  # time: a sum of the control's dt values
+        assert v[3] == pytest.approx(w[5], rel=1e-12)  # target current from tPP
+        assert abs(v[1] - w[3]) <= 1e-8, (g, w)  # Xrf
+        assert abs(v[2] - w[4]) <= 1e-6 * max(abs(w[4]), abs(w[5])), (g, w)  # current, against the target's scale
+        assert abs(v[4] - w[6]) <= 1e-12, (g, w)  # BvE moves by +-Vsr dt per step
+    for fr, fields in frames.items():
+        for ff, fid in (("CP", F_CNP), ("CE", F_CNE), ("PP", F_PHP), ("PE", F_PHE)):
+            got_f = fileio.read_record(tmp_path / fileio.output_name(ff, 0, fr), (n, n, n), np.float32)
+            ref = fields[fid]
+            e = np.linalg.norm(got_f - ref) / np.linalg.norm(ref)
+            assert e <= 1e-6, (fr, ff, e)
+    o.close()
