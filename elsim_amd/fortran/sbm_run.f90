@@ -4,7 +4,7 @@
 !!
 !! Usage: sbm_run <namelist file>
 !!   &run n1=, n2=, n3=, nsteps=, mode=, sdf_s='', sdf_l='', cnp_file='',
-!!        tables_file='', outdir='.', outpf='GBQ105', adaptive=F, cr=6.0 /
+!!        tables_file='', outdir='.', outpf='GBQ105', adaptive=F, cr=6.0, asmb=F /
 !!   &params prm%dx=..., prm%dt=..., ... /   (defaults: the reference values)
 !! adaptive = F: nsteps steps at the fixed prm%dt and prm%vsr, then one output
 !!   frame and one TmFx line.
@@ -17,8 +17,12 @@
 !! Inputs are Fortran unformatted records as the reference reads them:
 !! FP32 SDFs (SBM:149-152, 181-184), the FP64 ghosted CnP (optional) and the
 !! tables as one FP64 (101,8) record.  Outputs: one FP32 record per field
-!! named as OutPut3D_A does (SBM:892-919) and the TmFx text file of ReC_TmFX
-!! (SBM:922-943).
+!! named as OutPut3D_A does (SBM:892-919) -- the volume fractions PS and PL
+!! once before the loop (SBM:347-348), CP, CE, PP, PE per frame -- and the TmFx
+!! text file of ReC_TmFX (SBM:922-943).  Names are <FF>1000<1000+frame>.dat,
+!! rank 0's of the MPI program; with asmb = T they are <FF>ASMB<1000+frame>.dat,
+!! the names the rank stitcher writes (SBMES_3D_Cmbn_MPI_v2.f90:113-129) and the
+!! MATLAB viewers read: one GPU holds the global array, so no stitching is needed.
 PROGRAM sbm_run
    USE elsim_gpu
    IMPLICIT NONE
@@ -33,14 +37,15 @@ PROGRAM sbm_run
    REAL(KIND=8) :: tpp, tpe, xrf, bve, tm, cr, vsr0, scrnt
    INTEGER(c_int) :: it
    INTEGER :: cnt
-   LOGICAL :: adaptive
-   NAMELIST /run/ n1, n2, n3, nsteps, mode, sdf_s, sdf_l, cnp_file, tables_file, outdir, outpf, adaptive, cr
+   LOGICAL :: adaptive, asmb
+   CHARACTER(LEN=4) :: rk
+   NAMELIST /run/ n1, n2, n3, nsteps, mode, sdf_s, sdf_l, cnp_file, tables_file, outdir, outpf, adaptive, cr, asmb
    NAMELIST /params/ prm
 
    CALL elsim_sbm_default_params(prm)
    n1 = 0; n2 = 0; n3 = 0; nsteps = 1; mode = ELSIM_MODE_FP64
    sdf_s = ''; sdf_l = ''; cnp_file = ''; tables_file = ''; outdir = '.'; outpf = 'GBQ105'
-   adaptive = .FALSE.; cr = 6.0
+   adaptive = .FALSE.; cr = 6.0; asmb = .FALSE.
    IF (COMMAND_ARGUMENT_COUNT() < 1) THEN
       WRITE (0, *) 'usage: sbm_run <namelist file>'
       ERROR STOP 1
@@ -72,6 +77,13 @@ PROGRAM sbm_run
       CALL SBM_Set(h, ELSIM_F_CNP, Fld)
    END IF
 
+   IF (asmb) THEN
+      rk = 'ASMB'
+   ELSE
+      WRITE (rk, "(I4)") 1000
+   END IF
+   CALL write_field(ELSIM_F_PSP, 'PS', 1)
+   CALL write_field(ELSIM_F_PSE, 'PL', 1)
    tm = 0.0d0
    IF (adaptive) THEN
       CALL time_loop()
@@ -142,15 +154,14 @@ CONTAINS
       CALL write_field(ELSIM_F_PHE, 'PE', fr)
    END SUBROUTINE write_frame
 
-   !! OutPut3D_A(fr, ..., prfn=FF//'1000', AB=2): SNGL record named
-   !! FF//I4(1000+rank)//I4(1000+frame)//'.dat'
+   !! OutPut3D_A(fr, ..., prfn=FF//rk, AB=2): SNGL record named
+   !! FF//rk//I4(1000+frame)//'.dat', rk = I4(1000+rank) or 'ASMB'
    SUBROUTINE write_field(fid, ff, fr)
       INTEGER, INTENT(IN) :: fid, fr
       CHARACTER(LEN=2), INTENT(IN) :: ff
-      CHARACTER(LEN=4) :: rk, fs
+      CHARACTER(LEN=4) :: fs
       INTEGER :: v
       CALL SBM_Get(h, fid, Fld)
-      WRITE (rk, "(I4)") 1000
       WRITE (fs, "(I4)") 1000 + fr
       OPEN (NEWUNIT=v, FILE=TRIM(outdir)//'/'//ff//rk//fs//'.dat', FORM='UNFORMATTED', STATUS='REPLACE', &
          ACTION='WRITE')

@@ -60,6 +60,10 @@ def test_sbm_run_matches_oracle(tmp_path):
         assert e <= 1e-6, (ff, e)
     line = open(tmp_path / "GBQ105TmFx.txt").read().split()
     assert len(line) == 7 and int(line[1]) == steps
+    # the volume fractions, once before the loop (SBM:347-348)
+    for ff, fid in (("PS", 0), ("PL", 1)):
+        got = fileio.read_record(tmp_path / fileio.output_name(ff, 0, 1), (n, n, n), np.float32)
+        assert np.allclose(got, o.get(fid)[1:-1, 1:-1, 1:-1], rtol=1e-6, atol=1e-7), ff
 
 
 def _r4(x):
@@ -126,7 +130,8 @@ def test_sbm_run_adaptive_matches_oracle(tmp_path, case):
     namelist(tmp_path / "run.nml", dict(n1=n, n2=n, n3=n, nsteps=nsteps, mode=E.MODE_MIXED,
                                         sdf_s=str(tmp_path / "ds.dat"), sdf_l=str(tmp_path / "dl.dat"),
                                         cnp_file=str(tmp_path / "cnp.dat"), tables_file=str(tmp_path / "tab.dat"),
-                                        outdir=str(tmp_path), outpf="GBQ105", adaptive=True), prm)
+                                        outdir=str(tmp_path), outpf="GBQ105", adaptive=True,
+                                        asmb=case == "stop_test"), prm)
     r = subprocess.run([EXE, str(tmp_path / "run.nml")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     o = O.OracleSbm(sdf, -sdf, E.synthetic_params(vsr=0.02), tab)
@@ -148,8 +153,14 @@ def test_sbm_run_adaptive_matches_oracle(tmp_path, case):
         assert abs(v[4] - w[6]) <= 1e-12, (g, w)  # BvE moves by +-Vsr dt per step
     for fr, fields in frames.items():
         for ff, fid in (("CP", F_CNP), ("CE", F_CNE), ("PP", F_PHP), ("PE", F_PHE)):
-            got_f = fileio.read_record(tmp_path / fileio.output_name(ff, 0, fr), (n, n, n), np.float32)
+            name = f"{ff}ASMB{1000 + fr}.dat" if case == "stop_test" else fileio.output_name(ff, 0, fr)
+            got_f = fileio.read_record(tmp_path / name, (n, n, n), np.float32)
             ref = fields[fid]
             e = np.linalg.norm(got_f - ref) / np.linalg.norm(ref)
             assert e <= 1e-6, (fr, ff, e)
+    names = sorted(os.listdir(tmp_path))
+    rk = "ASMB" if case == "stop_test" else "1000"
+    assert f"PS{rk}1001.dat" in names and f"PL{rk}1001.dat" in names, names
+    if rk == "ASMB":
+        assert not any(x.startswith(("CP1000", "PS1000")) for x in names), names
     o.close()
