@@ -41,10 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # launches that ran, scaled by the counts of a device copy of known size
 # (FETCH x 2.0, WRITE x 1.0); profiles/cg_kernels_256_pmc.md.  The flat
 # kernels sweep the whole ghosted box, (n+2)^3 elements.
-PMC_BYTES_PER_GHOSTED = {"upd32": (150970.8 * 2 + 134713.9) * 1024 / 258 ** 3,
-                         "dir32": (83871.2 * 2 + 67084.0) * 1024 / 258 ** 3}
+PMC_BYTES_PER_GHOSTED = {"upd32": (83891.2 * 2 + 67354.2) * 1024 / 258 ** 3,
+                         "dir32": (117425.8 * 2 + 134885.4) * 1024 / 258 ** 3}
 # the phP operator covers the interior: bytes per interior voxel, same runs
-PMC_BYTES_PER_VOXEL_OP32 = (72022.2 * 2 + 67632.4) * 1024 / 256 ** 3
+PMC_BYTES_PER_VOXEL_OP32 = (72105.4 * 2 + 67737.6) * 1024 / 256 ** 3
 
 
 def log(*a):
@@ -91,12 +91,13 @@ def matvec_roofline(n, reps=20):
 
 # Algorithmic bytes per interior voxel of the CG kernels (operands read and
 # written once).  Operator: p, kappa in, Ap out (the CnE operator also reads
-# psE: 4/8 B more).  Update: y/x, p, r, w, dinv in; y/x, r out.  Direction:
-# r, p, dinv in; p out.  The mixed solver's dinv is BF16 (2 B), the FP64
-# solver's FP64.
-KERNEL_BYTES = {"op32": 12, "upd32": 26, "dir32": 14, "op64": 24, "upd64": 56, "dir64": 32}
-KERNEL_NAMES = {"op32": "k_matvec<float> (7-point operator + p.Ap)", "upd32": "k_mx_update (y += a p, r -= a w, r.z)",
-                "dir32": "k_mx_dir (p = z + b p)", "op64": "k_matvec<double>", "upd64": "k_pcg64_update",
+# psE: 4/8 B more).  FP64 update: x, p, r, w, dinv in; x, r out; FP64
+# direction: r, p, dinv in; p out.  The mixed solver moves the solution
+# update into the direction kernel: update r, w, dinv in, r out; direction y,
+# r, p, dinv in, y, p out.  Its dinv is BF16 (2 B), the FP64 solver's FP64.
+KERNEL_BYTES = {"op32": 12, "upd32": 14, "dir32": 22, "op64": 24, "upd64": 56, "dir64": 32}
+KERNEL_NAMES = {"op32": "k_matvec<float> (7-point operator + p.Ap)", "upd32": "k_mx_update (r -= a w, r.z)",
+                "dir32": "k_mx_dir (y += a p, p = z + b p)", "op64": "k_matvec<double>", "upd64": "k_pcg64_update",
                 "dir64": "k_pcg64_dir"}
 
 
@@ -284,7 +285,10 @@ def main():
     # dominant kernel of the fast path: largest share of the CG iteration
     dom = max(kf, key=lambda k: kf[k]["us"]) if kf else None
     if dom:
-        traffic = PMC_BYTES_PER_GHOSTED[dom] * (n + 2) ** 3 if dom in PMC_BYTES_PER_GHOSTED else None
+        if dom == "op32":
+            traffic = PMC_BYTES_PER_VOXEL_OP32 * N
+        else:
+            traffic = PMC_BYTES_PER_GHOSTED[dom] * (n + 2) ** 3 if dom in PMC_BYTES_PER_GHOSTED else None
         roof = {"bound": "hbm", "achieved": kernels[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": kernels[dom]["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                 "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per launch, profiles/cg_kernels_256_pmc.md",

@@ -27,6 +27,7 @@ struct CgDev {
   int need_reliable;
   int n_reliable;
   int stop_rule;
+  int pending_y;   // mixed: this iteration's y += alpha p is still to be applied (by k_mx_dir)
 };
 
 // Device buffers of one grid's solves (allocated once per grid).

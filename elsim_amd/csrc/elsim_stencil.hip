@@ -494,29 +494,27 @@ __device__ __forceinline__ BfPair bf16x2(uint32_t w) {
   return BfPair{bf16_to_f32((uint16_t)(w & 0xffffu)), bf16_to_f32((uint16_t)(w >> 16))};
 }
 
-// y += alpha p; r -= alpha w; rz' = r.(dinv r); decide on a reliable update.
-// UPD_P pairs per thread (2048 elements per block): half the blocks of the
-// other flat kernels, so the finalising kernel that follows every iteration
-// reads half the partials.  G2 = number of element pairs.
-constexpr int UPD_P = 4;
+// r -= alpha w; rz' = r.(dinv r); decide on a reliable update.  The
+// solution update y += alpha p of the same iteration is left to k_mx_dir,
+// which reads p anyway (4 B per element less than doing it here).
+// UPD_P pairs per thread (4096 elements per block): a quarter of the blocks of
+// the other flat kernels, so the finalising kernel that follows every
+// iteration reads a quarter of the partials (2, 4, 8 and 16 pairs measured at
+// 256^3: 8 is the fastest).  G2 = number of element pairs.
+constexpr int UPD_P = 8;
 constexpr int UPD_U = 2 * UPD_P;
-__global__ __launch_bounds__(256) void k_mx_update(int64_t G2, float2* __restrict__ y32, const float2* __restrict__ p32,
-                                                   float2* __restrict__ r32, const float2* __restrict__ w32,
+__global__ __launch_bounds__(256) void k_mx_update(int64_t G2, float2* __restrict__ r32, const float2* __restrict__ w32,
                                                    const uint32_t* __restrict__ dinv, CgDev* cg, double* partials) {
   if (cg->done || cg->need_reliable) return;
   const float alp = (float)cg->alpha;
   double acc[2] = {0.0, 0.0};
-  float2 yv[UPD_P], pv[UPD_P], rv[UPD_P], wv[UPD_P];
+  float2 rv[UPD_P], wv[UPD_P];
   uint32_t dv[UPD_P];
-  FLAT_LOOP_U(UPD_P, G2, { yv[u] = y32[q]; pv[u] = p32[q]; rv[u] = r32[q]; wv[u] = w32[q]; dv[u] = dinv[q]; }, {
+  FLAT_LOOP_U(UPD_P, G2, { rv[u] = r32[q]; wv[u] = w32[q]; dv[u] = dinv[q]; }, {
     const BfPair dd = bf16x2(dv[u]);
-    float2 yn;
     float2 rn;
-    yn.x = yv[u].x + alp * pv[u].x;
-    yn.y = yv[u].y + alp * pv[u].y;
     rn.x = rv[u].x - alp * wv[u].x;
     rn.y = rv[u].y - alp * wv[u].y;
-    y32[q] = yn;
     r32[q] = rn;
     acc[0] += (double)rn.x * (double)(dd.a * rn.x);
     acc[0] += (double)rn.y * (double)(dd.b * rn.y);
@@ -538,21 +536,50 @@ __global__ __launch_bounds__(256) void k_mx_fold(int64_t G, double* __restrict__
   })
 }
 
-// p = dinv r + beta p (element pairs, DIR_P per thread).
+// y += alpha p (the solution update of the iteration whose residual update
+// ran, flagged by pending_y), then p = dinv r + beta p unless the iteration
+// ended the batch (converged, or a reliable update is due).  Element pairs,
+// DIR_P per thread.  The flags are uniform over the grid, so each launch takes
+// one of the three loops.
 constexpr int DIR_P = 4;
-__global__ __launch_bounds__(256) void k_mx_dir(int64_t G2, float2* __restrict__ p32, const float2* __restrict__ r32,
-                                                const uint32_t* __restrict__ dinv, const CgDev* cg) {
-  if (cg->done || cg->need_reliable) return;
-  const float bet = (float)cg->beta;
-  float2 rv[DIR_P], pv[DIR_P];
-  uint32_t dv[DIR_P];
-  FLAT_LOOP_U(DIR_P, G2, { rv[u] = r32[q]; pv[u] = p32[q]; dv[u] = dinv[q]; }, {
-    const BfPair dd = bf16x2(dv[u]);
-    float2 pn;
-    pn.x = dd.a * rv[u].x + bet * pv[u].x;
-    pn.y = dd.b * rv[u].y + bet * pv[u].y;
-    p32[q] = pn;
-  })
+__global__ __launch_bounds__(256) void k_mx_dir(int64_t G2, float2* __restrict__ y32, float2* __restrict__ p32,
+                                                const float2* __restrict__ r32, const uint32_t* __restrict__ dinv,
+                                                const CgDev* cg) {
+  const bool upd_y = cg->pending_y != 0, upd_p = !(cg->done || cg->need_reliable);
+  const float alp = (float)cg->alpha, bet = (float)cg->beta;
+  if (upd_y && upd_p) {
+    float2 yv[DIR_P], rv[DIR_P], pv[DIR_P];
+    uint32_t dv[DIR_P];
+    FLAT_LOOP_U(DIR_P, G2, { yv[u] = y32[q]; rv[u] = r32[q]; pv[u] = p32[q]; dv[u] = dinv[q]; }, {
+      const BfPair dd = bf16x2(dv[u]);
+      float2 yn;
+      float2 pn;
+      yn.x = yv[u].x + alp * pv[u].x;
+      yn.y = yv[u].y + alp * pv[u].y;
+      pn.x = dd.a * rv[u].x + bet * pv[u].x;
+      pn.y = dd.b * rv[u].y + bet * pv[u].y;
+      y32[q] = yn;
+      p32[q] = pn;
+    })
+  } else if (upd_y) {
+    float2 yv[DIR_P], pv[DIR_P];
+    FLAT_LOOP_U(DIR_P, G2, { yv[u] = y32[q]; pv[u] = p32[q]; }, {
+      float2 yn;
+      yn.x = yv[u].x + alp * pv[u].x;
+      yn.y = yv[u].y + alp * pv[u].y;
+      y32[q] = yn;
+    })
+  } else if (upd_p) {
+    float2 rv[DIR_P], pv[DIR_P];
+    uint32_t dv[DIR_P];
+    FLAT_LOOP_U(DIR_P, G2, { rv[u] = r32[q]; pv[u] = p32[q]; dv[u] = dinv[q]; }, {
+      const BfPair dd = bf16x2(dv[u]);
+      float2 pn;
+      pn.x = dd.a * rv[u].x + bet * pv[u].x;
+      pn.y = dd.b * rv[u].y + bet * pv[u].y;
+      p32[q] = pn;
+    })
+  }
 }
 
 // x += e (ghosts of e are zero).
@@ -593,6 +620,9 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
     if (op == FIN_CG64 || op == FIN_PCG64) skip = cg->done;
     if (op == FIN_TRUE_RELIABLE) skip = cg->done || !cg->need_reliable;
   }
+  // mixed path: the iteration whose alpha this is will update r, and its
+  // y += alpha p falls to k_mx_dir; an iteration skipped here updates nothing
+  if (cg && op == FIN_ALPHA_RZ && linear_tid() == 0) cg->pending_y = skip ? 0 : 1;
   if (skip) return;
   // the operations of the two-value reductions read tot[1]; with NV = 1 it
   // stays zero (those operations are only launched with NV = 2)
@@ -641,6 +671,7 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
     }
     case FIN_TRUE_FINAL: cg->rr_true = tot[0]; break;
     case FIN_TRUE_RELIABLE:
+      cg->pending_y = 0;  // y was folded into e64 after the last iteration's k_mx_dir
       cg->rr_true = tot[0];
       cg->n_reliable += 1;
       cg->need_reliable = 0;
@@ -857,12 +888,13 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
         }));
         fin1(FIN_ALPHA_RZ, nb32);
         ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] {
-          k_mx_update<<<fbu, FLAT_T, 0, s>>>(G2, (float2*)w.y32.p, (const float2*)w.p32.p, (float2*)w.r32.p,
-                                             (const float2*)w.w32.p, (const uint32_t*)dinv, cg, P);
+          k_mx_update<<<fbu, FLAT_T, 0, s>>>(G2, (float2*)w.r32.p, (const float2*)w.w32.p, (const uint32_t*)dinv,
+                                             cg, P);
         }));
         fin2(FIN_MX_UPDATE, fbu);
         ELSIM_TRY(timed(t, ELSIM_PROF_DIR32, s, [&] {
-          k_mx_dir<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)w.p32.p, (const float2*)w.r32.p, (const uint32_t*)dinv, cg);
+          k_mx_dir<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)w.y32.p, (float2*)w.p32.p, (const float2*)w.r32.p,
+                                          (const uint32_t*)dinv, cg);
         }));
       }
       ELSIM_CHECK_LAUNCH();
@@ -879,7 +911,8 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
         k_true_resid<OP, false><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, (float*)w.r32.p, dinv, cg,
                                                      P);
         fin2(FIN_TRUE_RELIABLE, nb64);
-        k_mx_dir<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)w.p32.p, (const float2*)w.r32.p, (const uint32_t*)dinv, cg);
+        k_mx_dir<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)w.y32.p, (float2*)w.p32.p, (const float2*)w.r32.p,
+                                        (const uint32_t*)dinv, cg);
         ELSIM_CHECK_LAUNCH();
         ELSIM_TRY(poll(w, s));
         if (w.host->done) break;

@@ -22,7 +22,7 @@ from elsim_amd import nn as NN
 from elsim_amd._lib import F_CNP, KernelProf
 
 # algorithmic bytes per voxel (the mixed solver keeps its Jacobi inverse in BF16)
-KERNEL_BYTES = {"op32": 12, "upd32": 26, "dir32": 14}
+KERNEL_BYTES = {"op32": 12, "upd32": 14, "dir32": 22}
 
 
 def log(*a):
