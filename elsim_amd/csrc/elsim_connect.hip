@@ -37,46 +37,129 @@ __device__ __forceinline__ double ps_int(const double* psi, int64_t q) {
   return PHASE == 1 ? 1.0 - psi[q] : psi[q];
 }
 
-// One AD iteration: cn (interior) -> cn_new (interior).
+// One AD iteration: cn (interior) -> cn_new (interior), on stencil tiles of
+// 256 d1 voxels x AD_SJ rows x up to 16 planes per block:
+// each thread keeps its rows of psi and Cn for the planes k-1, k, k+1 in
+// registers, d1 neighbours come from the adjacent lanes (DPP), and only the
+// rows beside the tile and the wave-edge lanes load neighbours from memory.
+// Registers hold psi as stored; a neighbour inside the box is taken as this
+// phase's value (1 - psi for the electrolyte), one in the ghost layer as
+// stored.  Cn has no ghost layer: its loads are clamped into the box, and a
+// neighbour across a face is replaced by the face rule (mirror, or bval on
+// the Dirichlet face) before use.  Per-voxel arithmetic and its order are
+// those of the reference loop body (AD:103-148, 175-222).
+template <int PHASE>
+__device__ __forceinline__ double ph(double raw) {
+  return PHASE == 1 ? 1.0 - raw : raw;
+}
+
+// 4 rows per thread: 877 us per iteration at 512^3 against 946 with 2 rows,
+// 892 with 3 and 888 with 8; marching odd plane tiles downward (as the CG
+// operator does) measured slower here (915 us with 4 rows).
+constexpr int AD_SJ = 4;
+
 template <int PHASE>
 __global__ __launch_bounds__(256) void k_ad_iter(Dims d, const double* __restrict__ psi, const double* __restrict__ cn,
                                                  double* __restrict__ cnn, double bval, int snap) {
-  const int i = blockIdx.x * 64 + threadIdx.x + 1, j = blockIdx.y * 4 + threadIdx.y + 1;
-  if (i > d.n1 || j > d.n2) return;
-  const int k0 = blockIdx.z * KC + 1, k1 = min(k0 + KC - 1, d.n3);
-  const int64_t sx = d.g1(), sz = d.g12(), nx = d.n1, nz = (int64_t)d.n1 * d.n2;
+  constexpr int SJ = AD_SJ;
+  const StIdx st = stencil_idx<SJ>(d, blockIdx.x);
+  const int i = st.i;
+  const bool col_ok = i <= d.n1;
+  const int icn = min(st.ic, d.n1);  // Cn column (lanes past the edge read the last one)
+  const int kstart = st.k0;
+  const int64_t sz = d.g12(), nz = (int64_t)d.n1 * d.n2;
+  int64_t rq[SJ], rn[SJ];
+#pragma unroll
+  for (int t = 0; t < SJ; t++) {
+    rq[t] = d.gi(st.ic, min(st.j0 + t, d.n2 + 1), kstart);
+    rn[t] = d.ni(icn, min(st.j0 + t, d.n2), kstart);
+  }
+  const int64_t qlo = d.gi(st.ic, st.j0 - 1, kstart), qhi = d.gi(st.ic, min(st.j0 + SJ, d.n2 + 1), kstart);
+  const int64_t nlo = d.ni(icn, max(st.j0 - 1, 1), kstart), nhi = d.ni(icn, min(st.j0 + SJ, d.n2), kstart);
+  const bool ld_left = st.lane == 0 && col_ok, ld_right = st.lane == 63 && col_ok;
+  const bool has_behind = kstart > 1;  // plane k0 - 1 is in the box
+  double pc[SJ], pb[SJ], cc[SJ], cb[SJ];  // planes k and k - 1
+#pragma unroll
+  for (int t = 0; t < SJ; t++) {
+    pc[t] = psi[rq[t]];
+    pb[t] = psi[rq[t] - sz];
+    cc[t] = cn[rn[t]];
+    cb[t] = has_behind ? cn[rn[t] - nz] : cc[t];
+  }
   const double dt = 0.0625, idh2 = 1.0;
-  for (int k = k0; k <= k1; k++) {
-    const int64_t q = d.gi(i, j, k), n = d.ni(i, j, k);
-    const double p = ps_int<PHASE>(psi, q);
-    const double c = cn[n];
-    double v = c;
-    if (p >= kThrUpd) {
-      // neighbour psi: interior values of this phase, ghosts as stored
-      const double pyp = i < d.n1 ? ps_int<PHASE>(psi, q + 1) : psi[q + 1];
-      const double pym = i > 1 ? ps_int<PHASE>(psi, q - 1) : psi[q - 1];
-      const double pxp = j < d.n2 ? ps_int<PHASE>(psi, q + sx) : psi[q + sx];
-      const double pxm = j > 1 ? ps_int<PHASE>(psi, q - sx) : psi[q - sx];
-      const double pzp = k < d.n3 ? ps_int<PHASE>(psi, q + sz) : psi[q + sz];
-      const double pzm = k > 1 ? ps_int<PHASE>(psi, q - sz) : psi[q - sz];
-      // neighbour Cn: mirror on Neumann faces, bval on the Dirichlet face
-      const double cyp = i < d.n1 ? cn[n + 1] : c;
-      const double cym = i > 1 ? cn[n - 1] : c;
-      const double cxp = j < d.n2 ? cn[n + nx] : (PHASE == 0 ? bval : c);
-      const double cxm = j > 1 ? cn[n - nx] : (PHASE == 1 ? bval : c);
-      const double czp = k < d.n3 ? cn[n + nz] : c;
-      const double czm = k > 1 ? cn[n - nz] : c;
-      double s = (pyp + p) * (cyp - c);
-      s = s - (p + pym) * (c - cym);
-      s = s + (pxp + p) * (cxp - c);
-      s = s - (p + pxm) * (c - cxm);
-      s = s + (pzp + p) * (czp - c);
-      s = s - (p + pzm) * (c - czm);
-      const double div = 0.5 * idh2 / p * s;
-      v = c + dt * div;
+  int64_t oq = 0, on = 0;
+  for (int k = st.k0; k <= st.k1; k++, oq += sz, on += nz) {
+    double pa[SJ], ca[SJ];  // plane k + 1
+    const bool has_ahead = k < d.n3;
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      pa[t] = psi[rq[t] + oq + sz];
+      ca[t] = has_ahead ? cn[rn[t] + on + nz] : cc[t];
     }
-    if (snap && v > kThrSnap && p > kThrUpd) v = 1.0;
-    cnn[n] = v;
+    const double pjlo = psi[qlo + oq], pjhi = psi[qhi + oq], cjlo = cn[nlo + on], cjhi = cn[nhi + on];
+    double pl[SJ], pr[SJ], cl[SJ], cr[SJ];
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      pl[t] = lane_shift<true>(pc[t]);
+      pr[t] = lane_shift<false>(pc[t]);
+      cl[t] = lane_shift<true>(cc[t]);
+      cr[t] = lane_shift<false>(cc[t]);
+    }
+    if (ld_left) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        pl[t] = psi[rq[t] + oq - 1];
+        cl[t] = i > 1 ? cn[rn[t] + on - 1] : cc[t];
+      }
+    }
+    if (ld_right) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        pr[t] = psi[rq[t] + oq + 1];
+        cr[t] = i < d.n1 ? cn[rn[t] + on + 1] : cc[t];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      const int j = st.j0 + t;
+      if (!col_ok || j > d.n2) continue;
+      const double p = ph<PHASE>(pc[t]);
+      const double c = cc[t];
+      double v = c;
+      if (p >= kThrUpd) {
+        const double pyp = i < d.n1 ? ph<PHASE>(pr[t]) : pr[t];
+        const double pym = i > 1 ? ph<PHASE>(pl[t]) : pl[t];
+        const double pxr = t == SJ - 1 ? pjhi : pc[t + 1], pxl = t == 0 ? pjlo : pc[t - 1];
+        const double pxp = j < d.n2 ? ph<PHASE>(pxr) : pxr;
+        const double pxm = j > 1 ? ph<PHASE>(pxl) : pxl;
+        const double pzp = k < d.n3 ? ph<PHASE>(pa[t]) : pa[t];
+        const double pzm = k > 1 ? ph<PHASE>(pb[t]) : pb[t];
+        const double cyp = i < d.n1 ? cr[t] : c;
+        const double cym = i > 1 ? cl[t] : c;
+        const double cxr = t == SJ - 1 ? cjhi : cc[t + 1], cxl = t == 0 ? cjlo : cc[t - 1];
+        const double cxp = j < d.n2 ? cxr : (PHASE == 0 ? bval : c);
+        const double cxm = j > 1 ? cxl : (PHASE == 1 ? bval : c);
+        const double czp = k < d.n3 ? ca[t] : c;
+        const double czm = k > 1 ? cb[t] : c;
+        double s = (pyp + p) * (cyp - c);
+        s = s - (p + pym) * (c - cym);
+        s = s + (pxp + p) * (cxp - c);
+        s = s - (p + pxm) * (c - cxm);
+        s = s + (pzp + p) * (czp - c);
+        s = s - (p + pzm) * (c - czm);
+        const double div = 0.5 * idh2 / p * s;
+        v = c + dt * div;
+      }
+      if (snap && v > kThrSnap && p > kThrUpd) v = 1.0;
+      cnn[rn[t] + on] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      pb[t] = pc[t];
+      pc[t] = pa[t];
+      cb[t] = cc[t];
+      cc[t] = ca[t];
+    }
   }
 }
 
@@ -169,16 +252,17 @@ int accdiff(const Dims& d, const double* rdin, int phase, int term, double* cn) 
   if (!psi || !tmp) return ELSIM_ERR_ALLOC;
   k_ad_psi<<<flat_blocks(d.G()), FLAT_T, 0, s>>>(d, rdin, psi);
   ELSIM_HIP(hipMemsetAsync(cn, 0, (size_t)d.N() * 8, s));
-  const dim3 grid((d.n1 + 63) / 64, (d.n2 + 3) / 4, (d.n3 + KC - 1) / KC), blk(64, 4, 1);
+  const TileGrid tg = stencil_tiles(d, AD_SJ);
+  const int nt = tg.gx * tg.gy * tg.gz;
   double* a = cn;
   double* b = tmp;
   for (int it = 2; it <= term; it++) {
     const double bval = it == 2 ? 0.0 : 1.0;
     const int snap = (it % 5) == 1;
     if (phase == 0)
-      k_ad_iter<0><<<grid, blk, 0, s>>>(d, psi, a, b, bval, snap);
+      k_ad_iter<0><<<nt, 256, 0, s>>>(d, psi, a, b, bval, snap);
     else
-      k_ad_iter<1><<<grid, blk, 0, s>>>(d, psi, a, b, bval, snap);
+      k_ad_iter<1><<<nt, 256, 0, s>>>(d, psi, a, b, bval, snap);
     double* t = a;
     a = b;
     b = t;

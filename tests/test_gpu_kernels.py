@@ -153,9 +153,11 @@ def ad_case(shape, seed, frac=0.55):
 
 
 @pytest.mark.parametrize("phase", [0, 1])
-def test_accdiff_bit_exact(gpu, oracle_lib, phase):
+@pytest.mark.parametrize("shape", [(24, 30, 20), (65, 17, 33), (260, 5, 17)])
+def test_accdiff_bit_exact(gpu, oracle_lib, phase, shape):
+    """Odd sizes: a d1 extent past one wave and past one 256-wide tile, an odd
+    d2 extent (a half row tile) and d3 extents that end inside a plane tile."""
     O = oracle_lib
-    shape = (24, 30, 20)
     rd = ad_case(shape, 11)
     term = 151
     ref = O.accdiff_phase(rd, phase, term)
