@@ -79,6 +79,9 @@ def build_fortran(lib=LIB):
     exe = os.path.join(LIBDIR, "sbm_run")
     _run([FLANG, "-o", exe, objs[2], "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip",
           "-Wl,-rpath,$ORIGIN"])
+    _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "geom_run"),
+          os.path.join(FORTRAN, "geom_run.f90"), "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip",
+          "-Wl,-rpath,$ORIGIN"])
     # the rank stitcher is file I/O only: no GPU library
     _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "sbm_asmb"),
           os.path.join(FORTRAN, "sbm_asmb.f90")])

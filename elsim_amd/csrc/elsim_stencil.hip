@@ -539,9 +539,10 @@ __global__ __launch_bounds__(256) void k_mx_fold(int64_t G, double* __restrict__
 // y += alpha p (the solution update of the iteration whose residual update
 // ran, flagged by pending_y), then p = dinv r + beta p unless the iteration
 // ended the batch (converged, or a reliable update is due).  Element pairs,
-// DIR_P per thread.  The flags are uniform over the grid, so each launch takes
+// DIR_P per thread (2 measured 5 % faster than 4 or 8 at 512^3 from HBM, and
+// equal at 256^3).  The flags are uniform over the grid, so each launch takes
 // one of the three loops.
-constexpr int DIR_P = 4;
+constexpr int DIR_P = 2;
 __global__ __launch_bounds__(256) void k_mx_dir(int64_t G2, float2* __restrict__ y32, float2* __restrict__ p32,
                                                 const float2* __restrict__ r32, const uint32_t* __restrict__ dinv,
                                                 const CgDev* cg) {

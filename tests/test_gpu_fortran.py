@@ -1,7 +1,8 @@
-"""The Fortran host path end to end: sbm_run (Fortran, iso_c_binding) reads
-the reference's record formats, steps on the GPU and writes OutPut3D_A-style
-FP32 records, compared with the FP64 oracle; plus the drop-in ReDist3D_XYZ
-called from Fortran."""
+"""The Fortran host path end to end on the GPU, against the oracle:
+sbm_run (iso_c_binding) reads the reference's record formats, runs the SBM
+time loop (fixed or adaptive dt) and writes OutPut3D_A-style FP32 records and
+the TmFx file; geom_run runs the geometry chain LevelSet_3D -> AccDiff_3D ->
+RM_UCNCT_3D -> 2x refinement through the drop-in subroutines."""
 import os
 import subprocess
 
@@ -164,3 +165,36 @@ def test_sbm_run_adaptive_matches_oracle(tmp_path, case):
     if rk == "ASMB":
         assert not any(x.startswith(("CP1000", "PS1000")) for x in names), names
     o.close()
+
+
+def test_geom_run_matches_oracle(tmp_path):
+    """geom_run (Fortran: LS_FrontEnd, ReDist3D_XYZ, AccDiff_Phase, RM_Phase,
+    ReDist3D_XYZ_R4, Refine2_SDF) writes every record of the geometry chain
+    bit-exact to the oracle's chain, under the reference's file names."""
+    from elsim_amd import frontend as F
+    from elsim_amd import pipeline as P
+    exe = os.path.join(os.path.dirname(E.__file__), "lib", "geom_run")
+    assert os.path.exists(exe), "Fortran geometry driver not built"
+    image, (py, px, pz), sep, tr, fr, term = (20, 30, 8), (16, 4 + 4 * 8, 20), 4, 4.5, 5, 301
+    sdf, _, _ = E.sphere_pack(image, 5, 2.0, 4.0, target=0.5, hardcore=0.85, connect=False)
+    tmpa = (sdf > 0).astype(np.float64)
+    np.savetxt(tmp_path / "img.txt", tmpa.ravel(order="F"), fmt="%.1f")
+    with open(tmp_path / "g.nml", "w") as f:
+        f.write(f"&geom image='{tmp_path / 'img.txt'}', yo={image[0]}, xo={image[1]}, zo={image[2]}, py={py}, "
+                f"px={px}, pz={pz}, sep={sep}, nac=50, fr={fr}, tr={tr!r}, lit=175, term={term}, "
+                f"outdir='{tmp_path}' /\n")
+    r = subprocess.run([exe, str(tmp_path / "g.nml")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    shape, g, frm = (py, px, pz), f"{py}x{px}x{pz}", f"{100 + fr}"
+    sg, _ = O.ls_frontend(tmpa, shape, sep, tr, F.REF_NAC)
+    l24 = O.redist3d(175, sg, np.zeros_like(sg), 0.1)[1:-1, 1:-1, 1:-1]
+    rd = lambda name, dt=np.float64, s=shape: fileio.read_record(tmp_path / name, s, dt)  # noqa: E731
+    assert np.array_equal(rd(f"L24Z{1000 + fr}.dat"), l24)
+    for ph, pre in ((0, "SOD"), (1, "LIQ")):
+        cn = O.accdiff_phase(l24, ph, term)
+        assert np.array_equal(rd(f"{pre}_3D_CN_{g}_Z_II_{frm}.dat"), cn), pre
+        ls = O.redist3d(175, O.rm_phase(l24, cn, ph), np.zeros_like(sg), P.RM_DT)[1:-1, 1:-1, 1:-1]
+        lsname, fine = ("GRP_LS", "DS") if ph == 0 else ("POR_LS", "DL")
+        assert np.array_equal(rd(f"{lsname}_{g}_Z_II_{frm}.dat"), ls), lsname
+        got = rd(f"GrMic_II_{fine}_{2 * py}x{2 * px}x{2 * pz}_B{frm}.dat", np.float32, (2 * py, 2 * px, 2 * pz))
+        assert np.array_equal(got, O.refine2(ls)), fine
