@@ -82,6 +82,11 @@ def build_fortran(lib=LIB):
     _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "geom_run"),
           os.path.join(FORTRAN, "geom_run.f90"), "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip",
           "-Wl,-rpath,$ORIGIN"])
+    # the test program of the drop-ins that no driver calls
+    chk = os.path.join(ROOT, "tests", "fortran", "dropin_check.f90")
+    if os.path.exists(chk):
+        _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "dropin_check"), chk,
+              "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip", "-Wl,-rpath,$ORIGIN"])
     # the rank stitcher is file I/O only: no GPU library
     _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "sbm_asmb"),
           os.path.join(FORTRAN, "sbm_asmb.f90")])

@@ -198,3 +198,39 @@ def test_geom_run_matches_oracle(tmp_path):
         assert np.array_equal(rd(f"{lsname}_{g}_Z_II_{frm}.dat"), ls), lsname
         got = rd(f"GrMic_II_{fine}_{2 * py}x{2 * px}x{2 * pz}_B{frm}.dat", np.float32, (2 * py, 2 * px, 2 * pz))
         assert np.array_equal(got, O.refine2(ls)), fine
+
+
+@pytest.mark.parametrize("phase", [0, 1])
+def test_dropins_from_fortran(tmp_path, phase):
+    """NoGrdBc3D, AbsGrd3DUW_XYZ, Tbl_Chk_Pln_3DA, Label_Connected and
+    RM_Threshold called from Fortran (tests/fortran/dropin_check.f90) with the
+    reference's argument lists: bit-exact against the oracle."""
+    exe = os.path.join(os.path.dirname(E.__file__), "lib", "dropin_check")
+    assert os.path.exists(exe), "Fortran drop-in check program not built"
+    py, px, pz, sz = 13, 9, 7, 101
+    rng = np.random.default_rng(40 + phase)
+    g = (py + 2, px + 2, pz + 2)
+    f = rng.standard_normal(g)
+    sg = rng.choice([-1.0, 0.0, 1.0], size=g, p=[0.45, 0.1, 0.45])
+    c = rng.uniform(-0.2, 1.2, (py, px, pz))
+    c[0, 0, 0] = float(np.float32(1e-4))
+    psa = rng.uniform(0, 1, (py, px, pz))
+    rdin, _, _ = E.sphere_pack((py, px, pz), 9 + phase, 1.5, 3.0, target=0.5, hardcore=0.85, connect=False, dcut=8.0)
+    rdin = rdin.astype(np.float64)
+    cord = np.linspace(0.0, 1.0, sz)
+    vals = rng.standard_normal(sz)
+    for name, a in (("f", f), ("sg", sg), ("c", c), ("psa", psa), ("rd", rdin), ("cord", cord), ("vals", vals)):
+        fileio.write_record(tmp_path / f"{name}.dat", a, np.float64)
+    r = subprocess.run([exe, str(tmp_path), str(py), str(px), str(pz), str(sz), str(phase)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rd = lambda name, s, dt=np.float64: fileio.read_record(tmp_path / name, s, dt)  # noqa: E731
+    n = (py, px, pz)
+    assert np.array_equal(rd("gr_out.dat", n), O.absgrd3duw(sg, f))
+    assert np.array_equal(rd("f_out.dat", g), O.nogrdbc3d(f))
+    fn = np.array([O.table(x, cord, vals) for x in c.ravel(order="F")]).reshape(n, order="F")
+    assert np.array_equal(rd("fn_out.dat", n), fn)
+    assert np.array_equal(rd("psb_out.dat", n), np.where(c < np.float32(1e-4), 0.0, psa))
+    lab, con = O.label_connected(rdin, phase)
+    assert np.array_equal(rd("lab_out.dat", n, np.int32), lab)
+    assert np.array_equal(rd("con_out.dat", n, np.int32), con)
