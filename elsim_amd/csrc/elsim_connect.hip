@@ -58,11 +58,53 @@ __device__ __forceinline__ double ph(double raw) {
 // operator does) measured slower here (915 us with 4 rows).
 constexpr int AD_SJ = 4;
 
+// Skipping unchanged tiles.  Each block records in a flag word per tile
+// whether any value of its tile changed (bit 0) and whether any value could
+// still be snapped, psi > 0.4f and 1e-4f < Cn != 1 (bit 1).  When the
+// previous iteration did not snap and had the Dirichlet value of this one, a
+// tile whose values and whose six neighbour tiles' values did not change in
+// the previous iteration gets the same update values again -- the map is
+// applied to the same inputs -- and, if this iteration snaps, keeps them
+// unless one can be snapped.  The output buffer already holds those values
+// (it was the previous iteration's input), so such a block only rewrites its
+// flag word: unchanged, snappable as before.
+__device__ __forceinline__ int ad_tile_id(const TileGrid& t, int ti, int tj, int tk) {
+  return tj + t.gy * (ti + t.gx * tk);
+}
+
 template <int PHASE>
 __global__ __launch_bounds__(256) void k_ad_iter(Dims d, const double* __restrict__ psi, const double* __restrict__ cn,
-                                                 double* __restrict__ cnn, double bval, int snap) {
+                                                 double* __restrict__ cnn, double bval, int snap,
+                                                 const int* __restrict__ chg_r, int* __restrict__ chg_w,
+                                                 int may_skip) {
+  constexpr int CHANGED = 1, SNAPPABLE = 2;
   constexpr int SJ = AD_SJ;
   const StIdx st = stencil_idx<SJ>(d, blockIdx.x);
+  // this block's tile in the tile grid (the renumbering of stencil_idx)
+  const TileGrid tgr = stencil_tiles(d, SJ);
+  int tb = blockIdx.x;
+  {
+    const int per = (tgr.gx * tgr.gy * tgr.gz) / 8;
+    if (tb < per * 8) tb = (tb % 8) * per + tb / 8;
+  }
+  const int tj = tb % tgr.gy, ti = (tb / tgr.gy) % tgr.gx, tk = tb / (tgr.gx * tgr.gy);
+  const int my_tile = ad_tile_id(tgr, ti, tj, tk);
+  if (may_skip) {
+    const int own = chg_r[my_tile];
+    int any = own & CHANGED;
+    if (ti > 0) any |= chg_r[ad_tile_id(tgr, ti - 1, tj, tk)] & CHANGED;
+    if (ti + 1 < tgr.gx) any |= chg_r[ad_tile_id(tgr, ti + 1, tj, tk)] & CHANGED;
+    if (tj > 0) any |= chg_r[ad_tile_id(tgr, ti, tj - 1, tk)] & CHANGED;
+    if (tj + 1 < tgr.gy) any |= chg_r[ad_tile_id(tgr, ti, tj + 1, tk)] & CHANGED;
+    if (tk > 0) any |= chg_r[ad_tile_id(tgr, ti, tj, tk - 1)] & CHANGED;
+    if (tk + 1 < tgr.gz) any |= chg_r[ad_tile_id(tgr, ti, tj, tk + 1)] & CHANGED;
+    if (snap) any |= own & SNAPPABLE;
+    if (!any) {
+      if (threadIdx.x == 0) chg_w[my_tile] = own & SNAPPABLE;
+      return;
+    }
+  }
+  int changed = 0, snappable = 0;
   const int i = st.i;
   const bool col_ok = i <= d.n1;
   const int icn = min(st.ic, d.n1);  // Cn column (lanes past the edge read the last one)
@@ -151,6 +193,8 @@ __global__ __launch_bounds__(256) void k_ad_iter(Dims d, const double* __restric
         v = c + dt * div;
       }
       if (snap && v > kThrSnap && p > kThrUpd) v = 1.0;
+      changed |= __double_as_longlong(v) != __double_as_longlong(c);
+      snappable |= p > kThrUpd && v > kThrSnap && v != 1.0;
       cnn[rn[t] + on] = v;
     }
 #pragma unroll
@@ -161,6 +205,9 @@ __global__ __launch_bounds__(256) void k_ad_iter(Dims d, const double* __restric
       cc[t] = ca[t];
     }
   }
+  changed = __syncthreads_or(changed);
+  snappable = __syncthreads_or(snappable);
+  if (threadIdx.x == 0) chg_w[my_tile] = (changed ? CHANGED : 0) | (snappable ? SNAPPABLE : 0);
 }
 
 __device__ __forceinline__ int uf_find(const int* __restrict__ L, int x) {
@@ -254,15 +301,22 @@ int accdiff(const Dims& d, const double* rdin, int phase, int term, double* cn) 
   ELSIM_HIP(hipMemsetAsync(cn, 0, (size_t)d.N() * 8, s));
   const TileGrid tg = stencil_tiles(d, AD_SJ);
   const int nt = tg.gx * tg.gy * tg.gz;
+  int* chg = (int*)scratch(7, (size_t)2 * nt * sizeof(int));
+  if (!chg) return ELSIM_ERR_ALLOC;
   double* a = cn;
   double* b = tmp;
   for (int it = 2; it <= term; it++) {
     const double bval = it == 2 ? 0.0 : 1.0;
     const int snap = (it % 5) == 1;
+    // the previous iteration did not snap and had this Dirichlet value
+    // (iterations 3 and on use 1)
+    const int may_skip = it >= 4 && (it % 5) != 2;
+    const int* chg_r = chg + ((it + 1) & 1) * nt;  // written by iteration it - 1
+    int* chg_w = chg + (it & 1) * nt;
     if (phase == 0)
-      k_ad_iter<0><<<nt, 256, 0, s>>>(d, psi, a, b, bval, snap);
+      k_ad_iter<0><<<nt, 256, 0, s>>>(d, psi, a, b, bval, snap, chg_r, chg_w, may_skip);
     else
-      k_ad_iter<1><<<nt, 256, 0, s>>>(d, psi, a, b, bval, snap);
+      k_ad_iter<1><<<nt, 256, 0, s>>>(d, psi, a, b, bval, snap, chg_r, chg_w, may_skip);
     double* t = a;
     a = b;
     b = t;

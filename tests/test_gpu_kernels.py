@@ -153,10 +153,12 @@ def ad_case(shape, seed, frac=0.55):
 
 
 @pytest.mark.parametrize("phase", [0, 1])
-@pytest.mark.parametrize("shape", [(24, 30, 20), (65, 17, 33), (260, 5, 17)])
+@pytest.mark.parametrize("shape", [(24, 30, 20), (65, 17, 33), (260, 5, 17), (300, 40, 36)])
 def test_accdiff_bit_exact(gpu, oracle_lib, phase, shape):
     """Odd sizes: a d1 extent past one wave and past one 256-wide tile, an odd
-    d2 extent (a half row tile) and d3 extents that end inside a plane tile."""
+    d2 extent (a half row tile) and d3 extents that end inside a plane tile;
+    the last case has tiles in every direction that start, stop and resume
+    changing while the front crosses them (the skipping of unchanged tiles)."""
     O = oracle_lib
     rd = ad_case(shape, 11)
     term = 151
