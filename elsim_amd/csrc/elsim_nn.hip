@@ -124,10 +124,104 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
   }
 }
 
+// The same convolution for 16-channel slabs (input channels not a multiple
+// of 32) and at most 32 output channels: each wave loads the weight
+// fragments of all 27 taps of a slab into registers at once (27 x NT loads in
+// flight, issued before the slab is staged), instead of one load per tap
+// whose latency the tap's few MFMAs cannot hide.  Same MFMAs in the same order
+// as k_conv3, so the same results.
+template <int COUT, int MT, int MODE>
+__global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
+                                                   View iv, const uint16_t* __restrict__ wt,
+                                                   const float* __restrict__ bias, int relu, void* __restrict__ out,
+                                                   View ov) {
+  constexpr int NT = COUT / 16, CS = 16;
+  static_assert(NT <= 2, "weight fragments of 27 taps for at most two output tiles");
+  constexpr int HX = CONV_TX + 2, HY = CONV_TY + 2, HZ = MT + 2, HN = HX * HY * HZ;
+  constexpr int RS = CS + LDS_PAD;
+  __shared__ __align__(16) uint16_t lds[HN * RS];
+  const int tx = (n1 + CONV_TX - 1) / CONV_TX, ty = (n2 + CONV_TY - 1) / CONV_TY;
+  const int b = blockIdx.x;
+  const int x0 = (b % tx) * CONV_TX, y0 = ((b / tx) % ty) * CONV_TY, z0 = (b / (tx * ty)) * MT;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int K = 27 * cin;
+  f4v acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; m++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[m][t] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  for (int slab = 0; slab < cin; slab += CS) {
+    s4v bw[27][NT];
+#pragma unroll
+    for (int tap = 0; tap < 27; tap++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) bw[tap][t] = *(const s4v*)(wt + (int64_t)(t * 16 + lr) * K + tap * cin + slab + 4 * lq);
+    __syncthreads();
+    for (int it = tid; it < HN * 2; it += 256) {
+      const int h = it >> 1, q = it & 1;
+      const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3) {
+        const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
+        v = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
+      }
+      *(uint4*)(lds + h * RS + 8 * q) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < 27; tap++) {
+      const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
+      s4v a[MT];
+#pragma unroll
+      for (int m = 0; m < MT; m++) {
+        const int row = wave * MT + m, y = row % CONV_TY, z = row / CONV_TY;
+        const int h = ((z + dz) * HY + (y + dy)) * HX + (lr + dx);
+        a[m] = *(const s4v*)(lds + h * RS + 4 * lq);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bw[tap][t], acc[m][t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int row = wave * MT + m, y = y0 + row % CONV_TY, z = z0 + row / CONV_TY;
+    if (y >= n2 || z >= n3) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int x = x0 + 4 * lq + i;
+      if (x >= n1) continue;
+      const int64_t vox = x + (int64_t)n1 * (y + (int64_t)n2 * z);
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const int co = t * 16 + lr;
+        if (MODE == 0) {
+          float v = acc[m][t][i] + bias[co];
+          if (relu) v = fmaxf(v, 0.f);
+          ((uint16_t*)out)[vox * ov.cs + ov.co + co] = f2bf(v);
+        } else {
+          ((float*)out)[vox * ov.cs + ov.co + co] = acc[m][t][i];
+        }
+      }
+    }
+  }
+}
+
 template <int MODE, int COUT, int MT>
 static void conv_launch(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, const uint16_t* w,
                         const float* bias, int relu, void* out, View ov, hipStream_t s) {
   const int nb = ((n1 + CONV_TX - 1) / CONV_TX) * ((n2 + CONV_TY - 1) / CONV_TY) * ((n3 + MT - 1) / MT);
+  if constexpr (COUT <= 32) {
+    if (cin % 32 != 0) {
+      k_conv3_w27<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov);
+      return;
+    }
+  }
   k_conv3<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov);
 }
 
