@@ -80,14 +80,10 @@ __global__ __launch_bounds__(256) void k_ad_iter(Dims d, const double* __restric
   constexpr int CHANGED = 1, SNAPPABLE = 2;
   constexpr int SJ = AD_SJ;
   const StIdx st = stencil_idx<SJ>(d, blockIdx.x);
-  // this block's tile in the tile grid (the renumbering of stencil_idx)
+  // this block's tile in the tile grid
   const TileGrid tgr = stencil_tiles(d, SJ);
-  int tb = blockIdx.x;
-  {
-    const int per = (tgr.gx * tgr.gy * tgr.gz) / 8;
-    if (tb < per * 8) tb = (tb % 8) * per + tb / 8;
-  }
-  const int tj = tb % tgr.gy, ti = (tb / tgr.gy) % tgr.gx, tk = tb / (tgr.gx * tgr.gy);
+  const TileXyz tc = stencil_tile(d, SJ, blockIdx.x);
+  const int ti = tc.ti, tj = tc.tj, tk = tc.tk;
   const int my_tile = ad_tile_id(tgr, ti, tj, tk);
   if (may_skip) {
     const int own = chg_r[my_tile];
@@ -116,8 +112,9 @@ __global__ __launch_bounds__(256) void k_ad_iter(Dims d, const double* __restric
     rq[t] = d.gi(st.ic, min(st.j0 + t, d.n2 + 1), kstart);
     rn[t] = d.ni(icn, min(st.j0 + t, d.n2), kstart);
   }
-  const int64_t qlo = d.gi(st.ic, st.j0 - 1, kstart), qhi = d.gi(st.ic, min(st.j0 + SJ, d.n2 + 1), kstart);
-  const int64_t nlo = d.ni(icn, max(st.j0 - 1, 1), kstart), nhi = d.ni(icn, min(st.j0 + SJ, d.n2), kstart);
+  const int64_t qlo = d.gi(st.ic, min(st.j0 - 1, d.n2 + 1), kstart),
+                qhi = d.gi(st.ic, min(st.j0 + SJ, d.n2 + 1), kstart);
+  const int64_t nlo = d.ni(icn, min(max(st.j0 - 1, 1), d.n2), kstart), nhi = d.ni(icn, min(st.j0 + SJ, d.n2), kstart);
   const bool ld_left = st.lane == 0 && col_ok, ld_right = st.lane == 63 && col_ok;
   const bool has_behind = kstart > 1;  // plane k0 - 1 is in the box
   double pc[SJ], pb[SJ], cc[SJ], cb[SJ];  // planes k and k - 1

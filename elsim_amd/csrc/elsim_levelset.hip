@@ -102,7 +102,8 @@ __global__ __launch_bounds__(256) void k_sweep_rb(Dims d, const S* __restrict__ 
   int64_t rc[SJ];
 #pragma unroll
   for (int t = 0; t < SJ; t++) rc[t] = d.gi(st.ic, min(st.j0 + t, d.n2 + 1), st.k0);
-  const int64_t clo = d.gi(st.ic, st.j0 - 1, st.k0), chi = d.gi(st.ic, min(st.j0 + SJ, d.n2 + 1), st.k0);
+  const int64_t clo = d.gi(st.ic, min(st.j0 - 1, d.n2 + 1), st.k0),
+                chi = d.gi(st.ic, min(st.j0 + SJ, d.n2 + 1), st.k0);
   const bool ld_left = st.lane == 0 && col_ok, ld_right = st.lane == 63 && col_ok;
   T fc[SJ], fm[SJ];
 #pragma unroll

@@ -78,7 +78,8 @@ __device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __
   int64_t rc[SJ];
 #pragma unroll
   for (int t = 0; t < SJ; t++) rc[t] = d.gi(s.ic, min(s.j0 + t, d.n2 + 1), kstart);
-  const int64_t clo = d.gi(s.ic, s.j0 - 1, kstart), chi = d.gi(s.ic, min(s.j0 + SJ, d.n2 + 1), kstart);
+  const int64_t clo = d.gi(s.ic, min(s.j0 - 1, d.n2 + 1), kstart),
+                chi = d.gi(s.ic, min(s.j0 + SJ, d.n2 + 1), kstart);
   const bool ld_left = s.lane == 0 && (i > 1 || !HOM);
   const bool ld_right = s.lane == 63 && (i < d.n1 || (!HOM && i == d.n1));
   const T half = T(0.5);
