@@ -428,6 +428,13 @@ __global__ __launch_bounds__(256) void k_ws_axpy(Dims d, double* __restrict__ x,
   each(d, ci, [&](int64_t c, int64_t, int) { x[c] += alpha * dv[c]; });
 }
 
+// x(interior) += dv - shift
+__global__ __launch_bounds__(256) void k_ws_axpy_shift(Dims d, double* __restrict__ x, const double* __restrict__ dv,
+                                                       double shift) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t c, int64_t, int) { x[c] += dv[c] - shift; });
+}
+
 struct SbmState {
   Dims d{0, 0, 0};
   elsim_sbm_params P{};
@@ -725,6 +732,16 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
     k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.ws_dir), 1.0);
     ELSIM_CHECK_LAUNCH();
     ELSIM_TRY(bc(s, s.g(s.php), pbc0));
+    // phi_E: only the solves' part of its last change.  The rest of that
+    // change is the current control's uniform shift (SBM:831), which the
+    // previous step applied at its end (dv of that step, still in the
+    // scalars) and which this step applies again with its own sign.
+    k_ws_axpy_shift<<<grid, blk, 0, stv>>>(d, s.g(s.phe), s.g(s.ws_dir_e), s.hs->dv * P.dt);
+    ELSIM_CHECK_LAUNCH();
+    BcArgs ebc0 = mirror_bc();
+    ebc0.lo = 1;
+    ebc0.lo_vp = &s.S()->bve;
+    ELSIM_TRY(bc(s, s.g(s.phe), ebc0));
     z.ws_used = 1;
   }
 
