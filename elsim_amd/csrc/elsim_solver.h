@@ -33,7 +33,7 @@ struct CgDev {
 // Device buffers of one grid's solves (allocated once per grid).
 struct CgWork {
   Dims d{0, 0, 0};
-  DevBuf res, wmx, prj, x0, r0;      // FP64: N, N, G, N, N
+  DevBuf res, wmx, prj, x0, r0;      // FP64, G each (x0: the solve's start)
   DevBuf e64;                        // FP64 G (correction, ghosted for the stencil)
   DevBuf y32, r32, w32, p32;         // FP32: N, N, N, G
   DevBuf kap32, pse32, dinv32;       // FP32: G, G; BF16 Jacobi inverse: G

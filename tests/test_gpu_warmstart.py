@@ -39,6 +39,8 @@ def test_learned_warm_start_saves_iterations(gpu):
     learned, fl = run(E.WS_LEARNED, net)
     it = {k: sum(st.it_php for st in v[2:]) for k, v in (("plain", plain), ("extrap", extrap), ("learned", learned))}
     print("phP CG iterations, steps 3..%d:" % STEPS, it)
+    print("coupling iterations per step:", {k: [st.n_outer for st in v] for k, v in
+                                            (("plain", plain), ("extrap", extrap), ("learned", learned))})
     for st in learned[2:]:
         assert st.ws_used == 2 and st.ws_energy_gain >= 0.0
     for v in (plain, extrap, learned):
