@@ -448,6 +448,9 @@ struct SbmState {
   // learned warm start: phi_P of the previous step, the network, its input,
   // output and the candidate field
   DevBuf php_prev, phe_prev, nn_x, nn_pred, ws_cand, ws_dir, ws_dir_e, ws_r0, outer_dp, outer_de;
+  // phi_P's change over the step before last (valid when ws_p_hist)
+  DevBuf ws_dir2;
+  bool ws_p_hist = false;
   uint16_t* rec_x = nullptr;  // armed recording of the next step's network input
   double* rec_start = nullptr;
   int nprev = 0;
@@ -455,7 +458,7 @@ struct SbmState {
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
                     &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0, &outer_dp,
-                    &outer_de};
+                    &outer_de, &ws_dir2};
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
     if (hs) (void)hipHostFree(hs);
@@ -651,6 +654,7 @@ int ensure_ws_buffers(SbmState& s) {
   ELSIM_TRY(dev_alloc(s.nn_pred, N * sizeof(float)));
   ELSIM_TRY(dev_alloc(s.ws_cand, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir, G * sizeof(double)));
+  ELSIM_TRY(dev_alloc(s.ws_dir2, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir_e, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_r0, N * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.outer_dp, G * sizeof(double)));
@@ -678,12 +682,16 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   //    transient)
   const bool ws_learned = P.ws_mode == ELSIM_WS_LEARNED && (s.ws_net != nullptr || s.rec_x != nullptr);
   const bool ws_on = (P.ws_mode == ELSIM_WS_EXTRAP || ws_learned) && s.nprev >= 2;
+  bool ws_quad = false;  // the change over the step before last is known too
   if (ws_on) {
     ELSIM_TRY(ensure_ws_buffers(s));
+    std::swap(s.ws_dir, s.ws_dir2);
     k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.php), s.g(s.php_prev), s.g(s.ws_dir));
     k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.phe), s.g(s.phe_prev), s.g(s.ws_dir_e));
     ELSIM_CHECK_LAUNCH();
+    ws_quad = s.ws_p_hist;
   }
+  s.ws_p_hist = ws_on;
   ELSIM_HIP(hipMemcpyAsync(s.php_prev.p, s.php.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
   ELSIM_HIP(hipMemcpyAsync(s.phe_prev.p, s.phe.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
   s.nprev++;
@@ -764,6 +772,17 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       // the mirror ghosts of phP are already exact (last BC application),
       // so re-mirroring them changes no value
       ELSIM_TRY(bc(s, s.g(s.php), only_hi));
+    }
+    if (nlp == 0 && ws_on && ws_quad) {
+      // second-order term: a Galerkin step along the second difference
+      // d1 - d2 of phi_P's last two changes (step 1 is quadratic
+      // extrapolation, 0 keeps the linear one), A-norm optimal, so never
+      // worse than the start it corrects.  Before the learned correction,
+      // which then sees the residual that this step leaves
+      k_diff_interior<<<grid, blk, 0, stv>>>(d, s.g(s.ws_dir), s.g(s.ws_dir2), s.g(s.ws_cand));
+      ELSIM_CHECK_LAUNCH();
+      double a2 = 0.0;
+      ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHP, s.g(s.php), s.g(s.kapp), s.g(s.rhsp), s.g(s.ws_cand), pbc0, &a2));
     }
     if (nlp == 0 && ws_learned && ws_on) {
       // learned correction of the extrapolated phi_P: the network maps

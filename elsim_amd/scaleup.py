@@ -7,7 +7,7 @@ electrochemical steps with the mixed-precision solver and the learned warm
 start.  Reports per-step iterations and times, the device memory footprint
 and the in-loop HBM rates of the CG kernels.
 
-    python -m elsim_amd.scaleup --n 512 --steps 10 --report scaleup.json
+    python -m elsim_amd.scaleup --n 512 --steps 20 --report scaleup.json
 """
 import argparse
 import ctypes as C
@@ -67,7 +67,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--ws", choices=["plain", "extrap", "learned"], default="learned")
     ap.add_argument("--weights", default=None)
     ap.add_argument("--report", default=None)
@@ -118,6 +118,7 @@ def main(argv=None):
                       "tunnel_radius": 3, "seed": a.seed, "warm_start": a.ws, "mode": "fp32 jacobi-pcg + fp64 reliable"},
            "generate_s": t_gen, "cleanup": cstats, "steps": steps,
            "s_per_step_after_first": float(np.mean([x["wall_s"] for x in steps[1:]])) if len(steps) > 1 else None,
+           "s_per_step_last_half": float(np.mean([x["wall_s"] for x in steps[len(steps) // 2:]])) if steps else None,
            "device_memory_bytes": used - base, "device_memory_total": total, "kernels_in_loop": kern}
     log(json.dumps({k: v for k, v in rep.items() if k != "steps"}))
     if a.report:
