@@ -285,8 +285,13 @@ def main():
     for k, v in list(kf.items()) + list(k64.items()):
         kernels[k] = {"kernel": KERNEL_NAMES[k], "us": v["us"], "bytes_per_voxel": KERNEL_BYTES[k],
                       "GBps": KERNEL_BYTES[k] * N / (v["us"] * 1e-6) / 1e9, "launches_timed": v["launches"]}
-    # dominant kernel of the fast path: largest share of the CG iteration
+    # dominant kernel of the fast path: largest share of the CG iteration.
+    # The operator and the direction kernel take the same time to within a
+    # few tenths of a microsecond, so a tie within 2 % goes to the operator,
+    # the one further from its roofline (the conservative figure).
     dom = max(kf, key=lambda k: kf[k]["us"]) if kf else None
+    if dom and "op32" in kf and kf["op32"]["us"] >= 0.98 * kf[dom]["us"]:
+        dom = "op32"
     if dom:
         if dom == "op32":
             traffic = PMC_BYTES_PER_VOXEL_OP32 * N

@@ -421,6 +421,14 @@ __global__ __launch_bounds__(256) void k_diff_interior(Dims d, const double* __r
   each(d, ci, [&](int64_t c, int64_t, int) { out[c] = a[c] - b[c]; });
 }
 
+// out(interior) = a - 2 b + c (ghosts of out untouched).
+__global__ __launch_bounds__(256) void k_diff3_interior(Dims d, const double* __restrict__ a,
+                                                        const double* __restrict__ b, const double* __restrict__ c,
+                                                        double* __restrict__ out) {
+  const Col ci = colx(d);
+  each(d, ci, [&](int64_t i, int64_t, int) { out[i] = (a[i] - 2.0 * b[i]) + c[i]; });
+}
+
 // x(interior) += alpha d.
 __global__ __launch_bounds__(256) void k_ws_axpy(Dims d, double* __restrict__ x, const double* __restrict__ dv,
                                                  double alpha) {
@@ -448,9 +456,10 @@ struct SbmState {
   // learned warm start: phi_P of the previous step, the network, its input,
   // output and the candidate field
   DevBuf php_prev, phe_prev, nn_x, nn_pred, ws_cand, ws_dir, ws_dir_e, ws_r0, outer_dp, outer_de;
-  // phi_P's change over the step before last (valid when ws_p_hist)
-  DevBuf ws_dir2;
-  bool ws_p_hist = false;
+  // phi_P's changes over the two steps before the last one (ws_p_hist of
+  // them valid)
+  DevBuf ws_dir2, ws_dir3;
+  int ws_p_hist = 0;
   uint16_t* rec_x = nullptr;  // armed recording of the next step's network input
   double* rec_start = nullptr;
   int nprev = 0;
@@ -458,7 +467,7 @@ struct SbmState {
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
                     &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0, &outer_dp,
-                    &outer_de, &ws_dir2};
+                    &outer_de, &ws_dir2, &ws_dir3};
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
     if (hs) (void)hipHostFree(hs);
@@ -655,6 +664,7 @@ int ensure_ws_buffers(SbmState& s) {
   ELSIM_TRY(dev_alloc(s.ws_cand, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir2, G * sizeof(double)));
+  ELSIM_TRY(dev_alloc(s.ws_dir3, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir_e, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_r0, N * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.outer_dp, G * sizeof(double)));
@@ -682,16 +692,17 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   //    transient)
   const bool ws_learned = P.ws_mode == ELSIM_WS_LEARNED && (s.ws_net != nullptr || s.rec_x != nullptr);
   const bool ws_on = (P.ws_mode == ELSIM_WS_EXTRAP || ws_learned) && s.nprev >= 2;
-  bool ws_quad = false;  // the change over the step before last is known too
+  int ws_hist = 0;  // earlier changes of phi_P known besides the last one
   if (ws_on) {
     ELSIM_TRY(ensure_ws_buffers(s));
-    std::swap(s.ws_dir, s.ws_dir2);
+    std::swap(s.ws_dir3, s.ws_dir2);
+    std::swap(s.ws_dir2, s.ws_dir);
     k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.php), s.g(s.php_prev), s.g(s.ws_dir));
     k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.phe), s.g(s.phe_prev), s.g(s.ws_dir_e));
     ELSIM_CHECK_LAUNCH();
-    ws_quad = s.ws_p_hist;
+    ws_hist = s.ws_p_hist;
   }
-  s.ws_p_hist = ws_on;
+  s.ws_p_hist = ws_on ? std::min(ws_hist + 1, 2) : 0;
   ELSIM_HIP(hipMemcpyAsync(s.php_prev.p, s.php.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
   ELSIM_HIP(hipMemcpyAsync(s.phe_prev.p, s.phe.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
   s.nprev++;
@@ -773,7 +784,7 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       // so re-mirroring them changes no value
       ELSIM_TRY(bc(s, s.g(s.php), only_hi));
     }
-    if (nlp == 0 && ws_on && ws_quad) {
+    if (nlp == 0 && ws_on && ws_hist >= 1) {
       // second-order term: a Galerkin step along the second difference
       // d1 - d2 of phi_P's last two changes (step 1 is quadratic
       // extrapolation, 0 keeps the linear one), A-norm optimal, so never
@@ -783,6 +794,13 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       ELSIM_CHECK_LAUNCH();
       double a2 = 0.0;
       ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHP, s.g(s.php), s.g(s.kapp), s.g(s.rhsp), s.g(s.ws_cand), pbc0, &a2));
+      if (ws_hist >= 2) {
+        // third-order term, likewise: the third difference d1 - 2 d2 + d3
+        k_diff3_interior<<<grid, blk, 0, stv>>>(d, s.g(s.ws_dir), s.g(s.ws_dir2), s.g(s.ws_dir3), s.g(s.ws_cand));
+        ELSIM_CHECK_LAUNCH();
+        double a3 = 0.0;
+        ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHP, s.g(s.php), s.g(s.kapp), s.g(s.rhsp), s.g(s.ws_cand), pbc0, &a3));
+      }
     }
     if (nlp == 0 && ws_learned && ws_on) {
       // learned correction of the extrapolated phi_P: the network maps
