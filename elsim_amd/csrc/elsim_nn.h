@@ -25,5 +25,11 @@ int64_t nn_param_count();
 // pred[v] (FP32) for the input features x (bf16 [voxel][16]); grid dims
 // multiples of 8.
 int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred);
+// The network of a handle issued by elsim_nn_create (nullptr if unknown or
+// destroyed).
+NnNet* nn_lookup(int64_t handle);
+// Detach `net` from every SBM state that holds it (called when the network is
+// destroyed).
+void sbm_forget_net(NnNet* net);
 
 }  // namespace elsim

@@ -18,8 +18,11 @@
 // channel roles exchanged (stride 1, zero padding), writing FP32.
 //
 // Channel counts are multiples of 16 (the input features are padded to 16).
-#include <cmath>
 #include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "elsim_common.h"
@@ -816,8 +819,23 @@ int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pre
 using namespace elsim;
 
 namespace {
-NnNet* net_of(const int64_t* h) { return (h && *h) ? (NnNet*)(intptr_t)*h : nullptr; }
+// Networks are held in an id map, like the SBM states: a handle that was
+// never issued, or whose network was destroyed, is rejected instead of
+// dereferenced.
+std::mutex g_nn_mu;
+std::map<int64_t, std::unique_ptr<NnNet>> g_nn;
+int64_t g_nn_next = 1;
+NnNet* net_of(const int64_t* h) {
+  if (!h || !*h) return nullptr;
+  std::lock_guard<std::mutex> lk(g_nn_mu);
+  auto it = g_nn.find(*h);
+  return it == g_nn.end() ? nullptr : it->second.get();
+}
 }  // namespace
+
+namespace elsim {
+NnNet* nn_lookup(int64_t h) { return net_of(&h); }
+}  // namespace elsim
 
 extern "C" {
 
@@ -826,14 +844,16 @@ int elsim_nn_param_count(void) { return (int)nn_param_count(); }
 int elsim_nn_create(int64_t* handle) {
   if (!handle) return ELSIM_ERR_ARG;
   ELSIM_TRY(ensure_init());
-  NnNet* n = new NnNet();
+  auto n = std::make_unique<NnNet>();
   const int rc = n->init();
   if (rc < 0) {
     n->release();
-    delete n;
     return rc;
   }
-  *handle = (int64_t)(intptr_t)n;
+  std::lock_guard<std::mutex> lk(g_nn_mu);
+  const int64_t id = g_nn_next++;
+  g_nn[id] = std::move(n);
+  *handle = id;
   return ELSIM_OK;
 }
 
@@ -841,8 +861,12 @@ int elsim_nn_destroy(const int64_t* handle) {
   NnNet* n = net_of(handle);
   if (!n) return ELSIM_ERR_ARG;
   (void)hipStreamSynchronize(stream());
+  // every SBM state that still holds this network falls back to the
+  // extrapolated start (no state keeps a pointer to freed memory)
+  sbm_forget_net(n);
   n->release();
-  delete n;
+  std::lock_guard<std::mutex> lk(g_nn_mu);
+  g_nn.erase(*handle);
   return ELSIM_OK;
 }
 

@@ -23,6 +23,7 @@ namespace elsim {
 
 struct SbmScal {
   double tpp, tpe, xrf, infx, bve, scrnt, dv, avcne, cvgp, cvgl, red;
+  double shift;  // the current control's uniform shift of phi_E applied at the end of the last step (dv dt)
 };
 
 // Generic reduction ops finished by the last block.
@@ -84,6 +85,7 @@ __device__ __forceinline__ void finish_red(int op, double tot, SbmScal* s, const
       s->scrnt = tot;
       const double dv = (P.trgi - tot) >= 0 ? fabs(P.vsr) : -fabs(P.vsr);
       s->dv = dv;
+      s->shift = dv * P.dt;
       s->bve = s->bve + dv * P.dt;
       break;
     }
@@ -464,6 +466,7 @@ struct SbmState {
   double* rec_start = nullptr;
   int nprev = 0;
   NnNet* ws_net = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // step timer, created once
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
                     &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0, &outer_dp,
@@ -471,6 +474,8 @@ struct SbmState {
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
     if (hs) (void)hipHostFree(hs);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
   }
   double* g(DevBuf& b) { return (double*)b.p; }
   SbmScal* S() { return (SbmScal*)scal.p; }
@@ -672,7 +677,7 @@ int ensure_ws_buffers(SbmState& s) {
   return ELSIM_OK;
 }
 
-int step(SbmState& s, int mode, elsim_step_stats* st) {
+int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
   const Dims& d = s.d;
   const elsim_sbm_params& P = s.P;
   const dim3 grid = col_grid(d), blk(TX, TY, 1);
@@ -681,10 +686,9 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   elsim_step_stats z{};
   int status = 0;
   SolveResult r;
-  hipEvent_t e0, e1;
-  ELSIM_HIP(hipEventCreate(&e0));
-  ELSIM_HIP(hipEventCreate(&e1));
-  ELSIM_HIP(hipEventRecord(e0, stv));
+  if (!s.ev0) ELSIM_HIP(hipEventCreate(&s.ev0));
+  if (!s.ev1) ELSIM_HIP(hipEventCreate(&s.ev1));
+  ELSIM_HIP(hipEventRecord(s.ev0, stv));
 
   // 0. warm start of the coupling loop: keep the potentials' change over
   //    the last step (from the third step on: the first starts from the
@@ -753,9 +757,10 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
     ELSIM_TRY(bc(s, s.g(s.php), pbc0));
     // phi_E: only the solves' part of its last change.  The rest of that
     // change is the current control's uniform shift (SBM:831), which the
-    // previous step applied at its end (dv of that step, still in the
-    // scalars) and which this step applies again with its own sign.
-    k_ws_axpy_shift<<<grid, blk, 0, stv>>>(d, s.g(s.phe), s.g(s.ws_dir_e), s.hs->dv * P.dt);
+    // previous step applied at its end (its dv dt, kept in the scalars with
+    // that step's own dt) and which this step applies again with its own
+    // sign.
+    k_ws_axpy_shift<<<grid, blk, 0, stv>>>(d, s.g(s.phe), s.g(s.ws_dir_e), s.hs->shift);
     ELSIM_CHECK_LAUNCH();
     BcArgs ebc0 = mirror_bc();
     ebc0.lo = 1;
@@ -767,14 +772,19 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   // 5. outer loop (SBM:624-812)
   double cvgp = 1.0, cvgl = 1.0;
   int nlp = 0;
+  auto butler_volmer = [&]() -> int {
+    k_bv<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.kfw), s.g(s.kbw),
+                                s.g(s.avp), s.g(s.rhe), P, s.g(s.rxn), s.g(s.rxc), s.g(s.rhsp), s.g(s.rhse));
+    ELSIM_CHECK_LAUNCH();
+    return ELSIM_OK;
+  };
   while (cvgp > P.outer_tol && cvgl > P.outer_tol) {
     if (nlp >= P.max_outer) {
       if (!status) status = ELSIM_NOT_CONVERGED;
       break;
     }
-    k_bv<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.kfw), s.g(s.kbw),
-                                s.g(s.avp), s.g(s.rhe), P, s.g(s.rxn), s.g(s.rxc), s.g(s.rhsp), s.g(s.rhse));
-    ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(butler_volmer());
+    bool moved = false;  // a start correction below moved phi_P or phi_E
     {
       // SBM:647: phP(:, nx+1, :) = BvP only
       BcArgs only_hi = mirror_bc();
@@ -794,12 +804,14 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       ELSIM_CHECK_LAUNCH();
       double a2 = 0.0;
       ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHP, s.g(s.php), s.g(s.kapp), s.g(s.rhsp), s.g(s.ws_cand), pbc0, &a2));
+      moved = moved || a2 != 0.0;
       if (ws_hist >= 2) {
         // third-order term, likewise: the third difference d1 - 2 d2 + d3
         k_diff3_interior<<<grid, blk, 0, stv>>>(d, s.g(s.ws_dir), s.g(s.ws_dir2), s.g(s.ws_dir3), s.g(s.ws_cand));
         ELSIM_CHECK_LAUNCH();
         double a3 = 0.0;
         ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHP, s.g(s.php), s.g(s.kapp), s.g(s.rhsp), s.g(s.ws_cand), pbc0, &a3));
+        moved = moved || a3 != 0.0;
       }
     }
     if (nlp == 0 && ws_learned && ws_on) {
@@ -845,19 +857,32 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
           z.ws_used = 2;
           z.ws_alpha = alpha;
           z.ws_energy_gain = num * num / -den;  // decrease of the squared A-norm error
+          moved = true;
         }
       }
       s.rec_x = nullptr;
       s.rec_start = nullptr;
     }
+    BcArgs ebc = mirror_bc();
+    ebc.lo = 1;
+    ebc.lo_vp = &s.S()->bve;
     const bool outer_acc = P.ws_mode != ELSIM_WS_PLAIN && nlp >= 1;
     if (outer_acc) {
-      // later coupling iterations: a Galerkin step along the last phP
-      // correction (successive corrections of the linearly converging
+      // later coupling iterations: Galerkin steps along the last phP and
+      // phE corrections (successive corrections of the linearly converging
       // coupling loop are nearly parallel)
-      double a_p = 0.0;
+      double a_p = 0.0, a_e = 0.0;
       ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHP, s.g(s.php), s.g(s.kapp), s.g(s.rhsp), s.g(s.outer_dp), pbc0, &a_p));
+      ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHE, s.g(s.phe), s.g(s.pcl), s.g(s.rhse), s.g(s.outer_de), ebc, &a_e));
+      moved = moved || a_p != 0.0 || a_e != 0.0;
     }
+    // The reference evaluates Butler-Volmer at the fields each coupling
+    // iteration starts from and measures CvgP / CvgL from them (ppO, peO,
+    // SBM:631-644, 726, 721, 802).  When a start correction moved a field
+    // after that evaluation, the reaction is evaluated again at the corrected
+    // start, so that both solves start where the reaction was evaluated and
+    // CvgP / CvgL remain the coupling iteration's fixed-point residual.
+    if (moved) ELSIM_TRY(butler_volmer());
     ELSIM_TRY(solve(s, ELSIM_OP_PHP, mode, &r));
     z.it_php += r.iters;
     z.relres_php = r.relres;
@@ -876,14 +901,6 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
       k_diff_interior<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.outer_dp));
       ELSIM_CHECK_LAUNCH();
     }
-    BcArgs ebc = mirror_bc();
-    ebc.lo = 1;
-    ebc.lo_vp = &s.S()->bve;
-    if (outer_acc) {
-      double a_e = 0.0;
-      ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHE, s.g(s.phe), s.g(s.pcl), s.g(s.rhse), s.g(s.outer_de), ebc, &a_e));
-    }
-
     ELSIM_TRY(solve(s, ELSIM_OP_PHE, mode, &r));
     z.it_phe += r.iters;
     z.relres_phe = r.relres;
@@ -916,12 +933,10 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   ebc.lo = 1;
   ebc.lo_vp = &s.S()->bve;
   ELSIM_TRY(bc(s, s.g(s.phe), ebc));
-  ELSIM_HIP(hipEventRecord(e1, stv));
+  ELSIM_HIP(hipEventRecord(s.ev1, stv));
   ELSIM_TRY(sync_scal(s));
   float ms = 0.f;
-  ELSIM_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  ELSIM_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
   z.xrf = s.hs->xrf;
   z.scrnt = s.hs->scrnt;
   z.status = status;
@@ -931,7 +946,26 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
   return status;
 }
 
+// A step that fails part-way has already recorded its start as the
+// previous fields and counted itself: the warm-start history is then
+// inconsistent, so it is dropped and the next step starts plain.
+int step(SbmState& s, int mode, elsim_step_stats* st) {
+  const int rc = step_impl(s, mode, st);
+  if (rc < 0) {
+    s.nprev = 0;
+    s.ws_p_hist = 0;
+  }
+  return rc;
+}
+
 }  // namespace
+
+void sbm_forget_net(NnNet* net) {
+  std::lock_guard<std::mutex> lk(g_sbm_mu);
+  for (auto& kv : g_sbm)
+    if (kv.second->ws_net == net) kv.second->ws_net = nullptr;
+}
+
 }  // namespace elsim
 
 using namespace elsim;
@@ -1038,6 +1072,12 @@ int elsim_sbm_set(const int64_t* handle, const int* field, const double* in) {
     ELSIM_TRY(red_gw(*s, s->g(s->cnp), s->g(s->psp), R_XRF));
     ELSIM_TRY(sync_scal(*s));
   }
+  // a field set from outside (e.g. a restart) breaks the time history that
+  // the warm start extrapolates: it starts over from the next step
+  if (*field == ELSIM_F_CNP || *field == ELSIM_F_CNE || *field == ELSIM_F_PHP || *field == ELSIM_F_PHE) {
+    s->nprev = 0;
+    s->ws_p_hist = 0;
+  }
   return ELSIM_OK;
 }
 
@@ -1085,7 +1125,14 @@ int elsim_sbm_ch_update(const int64_t* handle) {
 int elsim_sbm_set_warm_start(const int64_t* handle, const int64_t* nn) {
   SbmState* s = lookup(handle);
   if (!s) return ELSIM_ERR_ARG;
-  NnNet* net = (nn && *nn) ? (NnNet*)(intptr_t)*nn : nullptr;
+  NnNet* net = nullptr;
+  if (nn && *nn) {
+    net = nn_lookup(*nn);
+    if (!net) {
+      set_error("unknown network handle %lld", (long long)*nn);
+      return ELSIM_ERR_ARG;
+    }
+  }
   if (net && (s->d.n1 % 8 || s->d.n2 % 8 || s->d.n3 % 8)) {
     set_error("the warm-start network needs grid dimensions that are multiples of 8 (%dx%dx%d)", s->d.n1, s->d.n2,
               s->d.n3);

@@ -371,7 +371,8 @@ int elsim_nn_adam(const int64_t* handle, float lr);
 /* Attach a network to an SBM state (nn NULL or 0: detach) for
  * ELSIM_WS_LEARNED: CG then solves to the same tolerance from a start that is
  * never worse in the A-norm than the extrapolation, so the network can only
- * change the iteration count.  The network must outlive the attachment. */
+ * change the iteration count.  An unknown network handle is ELSIM_ERR_ARG;
+ * destroying an attached network detaches it from every state. */
 int elsim_sbm_set_warm_start(const int64_t* sbm_handle, const int64_t* nn_handle);
 /* Training data: during the next step (ELSIM_WS_LEARNED, from the third
  * step on) write the network input (bf16 [voxel][16]: psi_P, the start's

@@ -830,6 +830,32 @@ static inline double dab_of(const elsim_sbm_params* P, double scale_d0, double c
   return scale_d0 * exp(P->dab_c0 - P->dab_c1 * c + P->dab_c2 * (e * e));
 }
 
+/* Cell-centred coefficients of the step's operators over the ghosted box,
+ * from the electrolyte concentration cne and psi_E (the face arrays of the
+ * reference are their face averages):
+ *   which 0: CnE solve   pce = psE * D0 exp(...)            (SBM:474, 478)
+ *   which 1: RHE         pce = psE * tc1 D0 exp(...)        (SBM:579, 583)
+ *   which 2: phE solve   pcN = psE * cne * tc2 D0 exp(...)  (SBM:601, 604)
+ * dab (optional, ghosted) receives Dab itself. */
+void or_coef(const elsim_sbm_params* P, int n1, int n2, int n3, int which, const double* cne, const double* pse,
+             double* out, double* dab) {
+  size_t G = (size_t)(n1 + 2) * (n2 + 2) * (n3 + 2);
+  double sd0 = which == 0 ? P->d0 : (which == 1 ? P->tc1 * P->d0 : P->tc2 * P->d0);
+  for (size_t q = 0; q < G; q++) {
+    double d = dab_of(P, sd0, cne[q]);
+    if (dab) dab[q] = d;
+    out[q] = which == 2 ? pse[q] * cne[q] * d : pse[q] * d;
+  }
+}
+
+/* RHE = dx^-2 S(pce, CnE) over the interior, with CnE's ghosts as stored
+ * (SBM:591-596); pce is the which-1 coefficient of or_coef. */
+void or_rhe(int n1, int n2, int n3, const double* pce, const double* cne, double dx, double* rhe) {
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) rhe[NI(i, j, k)] = 1.0 / (dx * dx) * stencil_sum(n1, n2, pce, cne, i, j, k);
+}
+
 void* or_sbm_create(int n1, int n2, int n3, const elsim_sbm_params* prm, const float* sdf_s,
                     const float* sdf_l, int sz, const double* tables) {
   OrSbm* s = (OrSbm*)calloc(1, sizeof(OrSbm));
@@ -1032,7 +1058,6 @@ int or_sbm_step(void* h, elsim_step_stats* st) {
   OrSbm* s = (OrSbm*)h;
   int n1 = s->n1, n2 = s->n2, n3 = s->n3, sz = s->sz;
   const elsim_sbm_params* P = &s->p;
-  size_t G = (size_t)(n1 + 2) * (n2 + 2) * (n3 + 2);
   size_t N = (size_t)n1 * n2 * n3;
   double dx = P->dx, dt = P->dt;
   memset(st, 0, sizeof(*st));
@@ -1055,8 +1080,7 @@ int or_sbm_step(void* h, elsim_step_stats* st) {
   for (size_t r = 0; r < N; r++) lsm += s->rxn[r] * (s->avp[r] / dx);
   lsm = lsm * (dx * dx * dx);
   double infx = lsm / ((double)(n1 * n3) * (dx * dx));
-  for (size_t q = 0; q < G; q++) s->dab[q] = dab_of(P, P->d0, s->cne[q]);
-  for (size_t q = 0; q < G; q++) s->pce[q] = s->pse[q] * s->dab[q];
+  or_coef(P, n1, n2, n3, 0, s->cne, s->pse, s->pce, s->dab);
   if (!(P->skip_cne_first && s->iter == 1)) {
     st->it_cne = or_cg(s, ELSIM_OP_CNE, s->cne, s->pce, &stt, &rr);
     st->relres_cne = rr;
@@ -1072,13 +1096,9 @@ int or_sbm_step(void* h, elsim_step_stats* st) {
   }
 
   /* 3. potential coefficients (SBM:579-622) */
-  for (size_t q = 0; q < G; q++) s->dab[q] = dab_of(P, P->tc1 * P->d0, s->cne[q]);
-  for (size_t q = 0; q < G; q++) s->pce[q] = s->pse[q] * s->dab[q];
-  for (int k = 1; k <= n3; k++)
-    for (int j = 1; j <= n2; j++)
-      for (int i = 1; i <= n1; i++) s->rhe[NI(i, j, k)] = 1.0 / (dx * dx) * stencil_sum(n1, n2, s->pce, s->cne, i, j, k);
-  for (size_t q = 0; q < G; q++) s->dab[q] = dab_of(P, P->tc2 * P->d0, s->cne[q]);
-  for (size_t q = 0; q < G; q++) s->pcl[q] = s->pse[q] * s->cne[q] * s->dab[q];
+  or_coef(P, n1, n2, n3, 1, s->cne, s->pse, s->pce, NULL);
+  or_rhe(n1, n2, n3, s->pce, s->cne, dx, s->rhe);
+  or_coef(P, n1, n2, n3, 2, s->cne, s->pse, s->pcl, NULL);
   const double* T = s->tab;
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
