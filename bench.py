@@ -38,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+E_OP_PHP = 1
 
 # HBM bytes per ghosted element moved by the FP32 CG kernels inside the CG
 # loop at 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs over the
@@ -136,25 +137,50 @@ def nn_forward_time(E, net, n, reps=5):
     return ms.value * 1e-3 / reps
 
 
-def cpu_baseline(sdf, prm, tab, iters_per_step, budget_iters=12):
-    """FP64 oracle, one thread: time `budget_iters` CG iterations of the phP
-    system at full size and scale by the FP64 path's iterations per step."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sdf, prm, tab, cnp0, iters_per_step, target_s=12.0):
+    """The FP64 oracle built with OpenMP (every host core) on a bounded sample
+    of the same step: (1) one oracle step from the same state with every CG
+    capped at one iteration and one coupling iteration -- all of the step's
+    work except the CG iterations (CH, coefficients, RHE, Butler-Volmer,
+    residual set-up); (2) FP64 Jacobi-PCG iterations of the phP system at full
+    size (the algorithm of the FP64 GPU path), as many as fill ~target_s.
+    s/step = (1) + (2) per iteration x the FP64 path's CG iterations per
+    step."""
     import oracle as O
-    from elsim_amd._lib import F_KAPP, F_PHP, F_PSP
-    o = O.OracleSbm(sdf, -sdf, prm, tab)
-    kap, php, psp = o.get(F_KAPP), o.get(F_PHP), o.get(F_PSP)
-    n1, n2, n3 = sdf.shape
-    rhs = np.random.default_rng(1).standard_normal(n1 * n2 * n3) * 1e-3
-    it, rr = C.c_int(), C.c_double()
-    xf = O.flat(php)
+    from elsim_amd._lib import F_CNP, F_KAPP
+    L = O.omp_lib()
+    threads = int(L.or_threads())
+    o = O.OracleSbm(sdf, -sdf, prm.copy(max_it_e=1, max_it_p=1, max_it_l=1, max_outer=1, skip_cne_first=0), tab,
+                    L=L)
+    o.set(F_CNP, cnp0)
     t0 = time.perf_counter()
-    O.lib().or_solve_system(1, n1, n2, n3, O.p(O.flat(kap)), O.p(O.flat(psp)), O.p(xf), O.p(rhs),
-                            C.c_double(prm.dt), C.c_double(prm.dx), 1, C.c_double(1e-30), C.c_double(0.0),
-                            budget_iters - 1, O.p(O.flat(psp)), C.c_double(1.0), C.byref(it), C.byref(rr))
-    t = time.perf_counter() - t0
+    o.step()
+    t_rest = time.perf_counter() - t0
+    kap = o.get(F_KAPP)
     o.close()
-    per_it = t / max(it.value, 1)
-    return per_it * iters_per_step, t, it.value
+    n1, n2, n3 = sdf.shape
+    rhs = np.asfortranarray(np.random.default_rng(1).standard_normal((n1, n2, n3)) * 1e-3)
+    t5 = O.pcg_time(E_OP_PHP, kap, kap, rhs, prm.dt, prm.dx, 5)
+    k = int(min(2000, max(10, target_s / max(t5 / 5, 1e-6))))
+    t = O.pcg_time(E_OP_PHP, kap, kap, rhs, prm.dt, prm.dx, k)
+    per_it = t / k
+    return {"value": t_rest + per_it * iters_per_step, "unit": "s/step", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"oracle (C, FP64, OpenMP on {threads} threads): one step with every CG capped at one "
+                      f"iteration ({t_rest:.2f} s: CH, coefficients, RHE, Butler-Volmer, residual set-up) plus "
+                      f"{k} FP64 Jacobi-PCG iterations of the phP system at {n1}^3 ({t:.2f} s, "
+                      f"{per_it * 1e3:.1f} ms each) scaled by the FP64 GPU path's {iters_per_step:.0f} CG "
+                      f"iterations per step"}
 
 
 def reference_cg_cost(E, sdf, prm, tab, cnp0, cap=400):
@@ -178,8 +204,8 @@ def reference_cg_cost(E, sdf, prm, tab, cnp0, cap=400):
             "relres_phe_after_cap": st.relres_phe, "cap_per_solve": cap}
 
 
-def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0, profile=True, net=None):
-    from elsim_amd._lib import F_CNP
+def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0, profile=True, net=None, fields=False):
+    from elsim_amd._lib import F_CNE, F_CNP, F_PHE, F_PHP
     s = E.Sbm(sdf, -sdf, prm, tab)
     s.set(F_CNP, cnp0)
     if net is not None:
@@ -196,12 +222,14 @@ def run_path(E, sdf, prm, tab, mode, warmup, steps, cnp0, profile=True, net=None
         log(f"  step {k}: {st.as_dict()}")
     E.check(E.lib().elsim_synchronize())
     t = time.perf_counter() - t0
+    out_fields = {k: s.get(f)[1:-1, 1:-1, 1:-1] for k, f in (("cnp", F_CNP), ("cne", F_CNE), ("php", F_PHP),
+                                                              ("phe", F_PHE))} if fields else None
     kern = profiled_step(E, s, mode) if profile else {}
     s.set_warm_start(None)
     if kern:
         log("  in-loop kernels: " + ", ".join(f"{k} {v['us']:.1f} us" for k, v in kern.items()))
     s.close()
-    return t / steps, stats, kern
+    return t / steps, stats, kern, out_fields
 
 
 def main():
@@ -243,7 +271,10 @@ def main():
     t0 = time.perf_counter()
     sdf, ns, frac, target = make_case(n, a.seed)
     log(f"case: n={n} spheres={ns} solid={frac:.3f} (target {target:.3f}) gen {time.perf_counter() - t0:.1f}s")
-    prm = E.synthetic_params()
+    # the CnE solve resolves the step's change of the concentration to rtol
+    # (its b_eff is dominated by psi_E c itself, so the b_eff rule would be
+    # met at the start and CnE would do no work)
+    prm = E.synthetic_params(cne_rule=1)
     tab = E.tables()
     cnp0 = E.initial_cnp(n, a.seed)
 
@@ -253,25 +284,25 @@ def main():
         prm = prm.copy(max_it_e=a.cg_cap, max_it_p=a.cg_cap, max_it_l=a.cg_cap, max_outer=1)
     paths = a.paths.split(",")
     nan = float("nan")
-    tfast, stf, t64, st64, ref_it, kf, k64 = nan, [], nan, [], None, {}, {}
+    tfast, stf, t64, st64, ref_it, kf, k64, ffast, f64 = nan, [], nan, [], None, {}, {}, None, None
     from elsim_amd import nn as NN
     net = NN.WarmStartNet.load(a.weights or NN.DEFAULT_WEIGHTS)
     variants = {}
     if "fast" in paths:
         log("fast path (mixed precision + learned warm start)")
-        tfast, stf, kf = run_path(E, sdf, prm.copy(ws_mode=E.WS_LEARNED), tab, E.MODE_MIXED, a.warmup, a.steps, cnp0,
-                                  net=net)
+        tfast, stf, kf, ffast = run_path(E, sdf, prm.copy(ws_mode=E.WS_LEARNED), tab, E.MODE_MIXED, a.warmup,
+                                         a.steps, cnp0, net=net, fields=True)
         variants["learned"] = (tfast, stf)
     for key, wm, label in (("fast_extrap", E.WS_EXTRAP, "extrapolated"), ("fast_plain", E.WS_PLAIN, "plain")):
         if key in paths:
             log(f"mixed precision, {label} warm start")
-            t, st, _ = run_path(E, sdf, prm.copy(ws_mode=wm), tab, E.MODE_MIXED, a.warmup, a.steps, cnp0,
-                                profile=False)
+            t, st, _, _ = run_path(E, sdf, prm.copy(ws_mode=wm), tab, E.MODE_MIXED, a.warmup, a.steps, cnp0,
+                                   profile=False)
             variants[label] = (t, st)
     if "fp64" in paths:
         log("fp64 path (Jacobi-PCG, all FP64, plain warm start)")
-        t64, st64, k64 = run_path(E, sdf, prm.copy(ws_mode=E.WS_PLAIN), tab, E.MODE_FP64_PCG, a.warmup, a.steps,
-                                  cnp0)
+        t64, st64, k64, f64 = run_path(E, sdf, prm.copy(ws_mode=E.WS_PLAIN), tab, E.MODE_FP64_PCG, a.warmup,
+                                       a.steps, cnp0, fields=True)
     t_nn = nn_forward_time(E, net, n) if n % 8 == 0 else nan
     log(f"warm-start network forward at {n}^3: {t_nn * 1e3:.2f} ms")
     if "ref" in paths:
@@ -314,10 +345,15 @@ def main():
     itersf = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in stf])) if stf else nan
     cpu = None
     if not a.no_cpu and st64:
-        v, t, k = cpu_baseline(sdf, prm, tab, iters64)
-        cpu = {"value": v, "unit": "s/step", "cores": 1, "kind": "port",
-               "sample": f"{k} FP64 CG iterations of the phP system at {n}^3 on the oracle ({t:.1f} s, one thread), "
-                         f"scaled by the FP64 path's {iters64:.0f} CG iterations per step"}
+        log("CPU baseline (OpenMP oracle, bounded sample)")
+        cpu = cpu_baseline(sdf, prm, tab, cnp0, iters64)
+        log(f"  {cpu['sample']}")
+    # error of the fast path's fields against the FP64 path's after the same
+    # steps from the same state (both solve every system to relres rtol)
+    field_err = None
+    if ffast is not None and f64 is not None:
+        field_err = {k: float(np.linalg.norm(ffast[k] - f64[k]) / np.linalg.norm(f64[k])) for k in ffast}
+    mean = lambda st, k: float(np.mean([x[k] for x in st])) if st else nan
     out = {
         "metric": "s/electrochem step @256³: FP64 CG vs FP32-refined+NN warm start, rel res 1e-8",
         "value": tfast,
@@ -331,10 +367,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"sbm_electrochem_step_{n}^3", "grid": [n, n, n], "spheres": ns,
-                   "solid_fraction": round(frac, 4), "sphere_radius": [6, 16], "dt": prm.dt, "rtol": prm.rtol,
-                   "fast_path": "fp32 jacobi-pcg + fp64 reliable updates + learned warm start",
+        "config": {"workload": f"sbm_electrochem_step_{n}^3 (nondimensional synthetic parameters)",
+                   "grid": [n, n, n], "spheres": ns, "solid_fraction": round(frac, 4), "sphere_radius": [6, 16],
+                   "hardcore": 0.85, "connect": True, "dt": prm.dt, "dx": prm.dx, "zeta": prm.zeta,
+                   "F_over_RT": prm.cst1, "i0_table": 1.0, "i0_scale": prm.i0_scale, "alpha": 0.5,
+                   "kappa": "psi_P", "D": "exp(-0.2 c)", "rtol": prm.rtol,
+                   "cne_rule": "relative to the step's change (||r0||)", "cne_iters_per_step": mean(stf, "it_cne"),
+                   "coupling_iters_per_step": {"fast": mean(stf, "n_outer"), "fp64": mean(st64, "n_outer")},
+                   "fast_path": "fp32 jacobi-pcg + fp64 reliable updates (restarts) + learned warm start",
                    "learned_warm_start": True},
+        "field_rel_l2_vs_fp64": field_err,
         "fp64_s_per_step": t64,
         "fp64_path": "fp64 jacobi-pcg (same preconditioner, all FP64), plain warm start as the reference",
         "speedup_vs_fp64": t64 / tfast,

@@ -9,13 +9,13 @@ from ._lib import (MODE_FP64, MODE_FP64_PCG, MODE_MIXED, OP_CNE, OP_PHE, OP_PHP,
                    WS_LEARNED, WS_PLAIN, DeviceArray,
                    ElsimError, SbmParams, StepStats, check, exported_symbols, lib)
 from . import fileio
-from .sbm import (Sbm, carve_tunnel, ellipsoid_pack, initial_cnp, reference_params, sphere_pack, synthetic_params,
-                  tables, time_control)
+from .sbm import (Sbm, carve_tunnel, ellipsoid_pack, initial_cnp, potential_problem, reference_params, solve_pot,
+                  sphere_pack, synthetic_params, tables, time_control)
 
 __all__ = [
     "MODE_FP64", "MODE_FP64_PCG", "MODE_MIXED", "OP_CNE", "OP_PHP", "OP_PHE", "STOP_REFERENCE", "STOP_RELRES", "WS_EXTRAP",
     "WS_LEARNED", "WS_PLAIN", "DeviceArray",
     "ElsimError", "SbmParams", "StepStats", "check", "exported_symbols", "lib", "fileio", "Sbm", "carve_tunnel",
     "ellipsoid_pack", "initial_cnp", "reference_params", "sphere_pack", "synthetic_params", "tables",
-    "time_control",
+    "time_control", "potential_problem", "solve_pot",
 ]

@@ -29,7 +29,8 @@ class ElsimError(RuntimeError):
 _PARAM_DOUBLES = ("dx dt zeta psp_off pse_off avp_min eps rho ch_psi_min frd cst1 t_minus d0 tc1 tc2 "
                   "dab_c0 dab_c1 dab_c2 dab_c3 kap_scale i0_scale kfw_cref bvp bve vsr trgi cne_target "
                   "rtol crt_e crt_p crt_l outer_tol inner_rtol").split()
-_PARAM_INTS = "mass_corr_period skip_cne_first stop_rule max_it_e max_it_p max_it_l max_outer ws_mode".split()
+_PARAM_INTS = ("mass_corr_period skip_cne_first stop_rule max_it_e max_it_p max_it_l max_outer ws_mode cne_rule "
+               "reserved_").split()
 
 
 class SbmParams(C.Structure):
@@ -146,6 +147,8 @@ _SIGS = {
     "elsim_gen_ellipsoid_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, D, D, P, IP, DP]),
     "elsim_gen_tunnel": (I, [I, I, I, C.c_uint64, D, P]),
     "elsim_gen_tables": (I, [I, P]),
+    "elsim_solve_pot": (I, [IP, IP, IP, DP, P, P, P, IP, IP, IP, DP, IP, IP, DP]),
+    "elsim_dev_solve_pot": (I, [I, I, I, D, P, P, P, I, I, I, D, I, IP, DP]),
 }
 
 

@@ -37,7 +37,6 @@ HIP_FLAGS = [
     "--offload-arch=" + ARCH,
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics",
 ]
 # the learned warm start is not bound by bit-parity: contraction allowed
 PER_FILE = {"elsim_nn.hip": ["-ffp-contract=fast"]}
@@ -82,11 +81,12 @@ def build_fortran(lib=LIB):
     _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "geom_run"),
           os.path.join(FORTRAN, "geom_run.f90"), "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip",
           "-Wl,-rpath,$ORIGIN"])
-    # the test program of the drop-ins that no driver calls
-    chk = os.path.join(ROOT, "tests", "fortran", "dropin_check.f90")
-    if os.path.exists(chk):
-        _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "dropin_check"), chk,
-              "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip", "-Wl,-rpath,$ORIGIN"])
+    # the test programs of the drop-ins that no driver calls
+    for name in ("dropin_check", "solve_check"):
+        chk = os.path.join(ROOT, "tests", "fortran", name + ".f90")
+        if os.path.exists(chk):
+            _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, name), chk,
+                  "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip", "-Wl,-rpath,$ORIGIN"])
     # the rank stitcher is file I/O only: no GPU library
     _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "sbm_asmb"),
           os.path.join(FORTRAN, "sbm_asmb.f90")])

@@ -30,7 +30,7 @@ struct SbmScal {
 enum RedOp { R_TPP = 0, R_TPE, R_XRF, R_INFX, R_AVCNE, R_CVGP, R_CVGL, R_SCRNT, R_RAW };
 
 struct BcArgs {
-  int lo, hi;          // 0 mirror, 1 value, 2 CnE flux (low face only)
+  int lo, hi;          // 0 mirror, 1 value, 2 CnE flux (low face only), 3 keep the stored value
   double lo_v, hi_v;   // values (kind 1) unless lo_vp / hi_vp given
   const double* lo_vp; // device scalar (e.g. BvE)
   const double* dab01; // CnE flux: Dab on planes j=0 and j=1, (n1+2)(n3+2) each
@@ -184,6 +184,7 @@ __global__ void k_bc(Dims d, double* __restrict__ f, BcArgs b) {
     } else if (j == d.n2 + 1 && b.hi == 1) {
       v = b.hi_v;
     }
+    if ((j == 0 && b.lo == 3) || (j == d.n2 + 1 && b.hi == 3)) v = f[d.gi(i, j, k)];
     f[d.gi(i, j, k)] = v;
   }
 }
@@ -493,13 +494,13 @@ SbmState* lookup(const int64_t* h) {
   return it == g_sbm.end() ? nullptr : it->second.get();
 }
 
-int bc(SbmState& s, double* f, const BcArgs& b) {
-  const Dims& d = s.d;
+int launch_bc(const Dims& d, double* f, const BcArgs& b) {
   const int64_t tot = 2 * d.g12() + 2 * (int64_t)(d.n1 + 2) * d.n3 + 2 * (int64_t)d.n2 * d.n3;
   k_bc<<<flat_blocks(tot), FLAT_T, 0, stream()>>>(d, f, b);
   ELSIM_CHECK_LAUNCH();
   return ELSIM_OK;
 }
+int bc(SbmState& s, double* f, const BcArgs& b) { return launch_bc(s.d, f, b); }
 BcArgs mirror_bc() { return BcArgs{0, 0, 0.0, 0.0, nullptr, nullptr, nullptr, 0.0, 0.0}; }
 
 int sync_scal(SbmState& s) {
@@ -544,6 +545,7 @@ int solve(SbmState& s, int op, int mode, SolveResult* r) {
     a.maxit = P.max_it_e;
     a.w8 = s.g(s.pse);
     a.tw = s.hs->tpe;
+    a.rel_r0 = P.cne_rule == 1;
   } else if (op == ELSIM_OP_PHP) {
     a.x = s.g(s.php);
     a.kap = s.g(s.kapp);
@@ -960,6 +962,74 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
 
 }  // namespace
 
+namespace {
+CgWork g_pot_work;  // work space of the standalone potential solves (grows with the grid)
+
+int pot_args_ok(int n1, int n2, int n3, double dx, const void* kap, const void* rhs, const void* phi, int bc_lo,
+                int bc_hi, int mode, double rtol, int maxit) {
+  if (!dims_ok(n1, n2, n3)) {
+    set_error("bad grid %d x %d x %d", n1, n2, n3);
+    return ELSIM_ERR_ARG;
+  }
+  if (!kap || !rhs || !phi) {
+    set_error("null array argument");
+    return ELSIM_ERR_ARG;
+  }
+  if ((bc_lo != 0 && bc_lo != 1) || (bc_hi != 0 && bc_hi != 1) || (bc_lo == 0 && bc_hi == 0)) {
+    set_error("d2 face conditions (%d, %d): each 0 (Neumann) or 1 (Dirichlet), at least one Dirichlet", bc_lo,
+              bc_hi);
+    return ELSIM_ERR_ARG;
+  }
+  if (mode != ELSIM_MODE_FP64 && mode != ELSIM_MODE_MIXED && mode != ELSIM_MODE_FP64_PCG) {
+    set_error("unknown mode %d", mode);
+    return ELSIM_ERR_ARG;
+  }
+  if (!(rtol > 0.0) || maxit < 1 || !(dx > 0.0)) {
+    set_error("rtol %g, maxit %d, dx %g: need rtol > 0, maxit >= 1, dx > 0", rtol, maxit, dx);
+    return ELSIM_ERR_ARG;
+  }
+  return ELSIM_OK;
+}
+
+// Ghost planes of the unknown: mirrors on the d1 and d3 faces and on a
+// Neumann d2 face; a Dirichlet d2 plane keeps the value the caller stored.
+BcArgs pot_bc(int bc_lo, int bc_hi) {
+  BcArgs b = mirror_bc();
+  b.lo = bc_lo ? 3 : 0;
+  b.hi = bc_hi ? 3 : 0;
+  return b;
+}
+
+int solve_pot_dev(int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi, int bc_lo,
+                  int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres) {
+  const Dims d{n1, n2, n3};
+  const int op = bc_lo && bc_hi ? ELSIM_OP_POT_DD : (bc_hi ? ELSIM_OP_PHP : ELSIM_OP_PHE);
+  ELSIM_TRY(launch_bc(d, phi, pot_bc(bc_lo, bc_hi)));
+  SolveArgs a{};
+  a.d = d;
+  a.op = op;
+  a.x = phi;
+  a.kap = kap;
+  a.pse = nullptr;
+  a.rhs = rhs;
+  a.scale = 1.0 / (dx * dx);
+  a.stop_rule = ELSIM_STOP_RELRES;
+  a.rtol = rtol;
+  a.crt = 0.0;
+  a.maxit = maxit;
+  a.w8 = kap;
+  a.tw = 1.0;
+  a.delta = 0.5;
+  SolveResult r;
+  ELSIM_TRY(cg_solve(g_pot_work, a, mode, &r));
+  ELSIM_TRY(launch_bc(d, phi, pot_bc(bc_lo, bc_hi)));  // SBM:716-719
+  ELSIM_HIP(hipStreamSynchronize(stream()));
+  if (iters) *iters = r.iters;
+  if (relres) *relres = r.relres;
+  return r.status;
+}
+}  // namespace
+
 void sbm_forget_net(NnNet* net) {
   std::lock_guard<std::mutex> lk(g_sbm_mu);
   for (auto& kv : g_sbm)
@@ -1158,6 +1228,39 @@ int elsim_sbm_matvec(const int64_t* handle, const int* op, const double* p, doub
     return launch_matvec_f64(*op, s->d, p, s->g(s->pce), s->g(s->pse), P.dt / (P.dx * P.dx), out, nullptr, true);
   const double* kap = *op == ELSIM_OP_PHP ? s->g(s->kapp) : s->g(s->pcl);
   return launch_matvec_f64(*op, s->d, p, kap, nullptr, 1.0 / (P.dx * P.dx), out, nullptr, true);
+}
+
+int elsim_dev_solve_pot(int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi,
+                        int bc_lo, int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres) {
+  ELSIM_TRY(pot_args_ok(n1, n2, n3, dx, kap, rhs, phi, bc_lo, bc_hi, mode, rtol, maxit));
+  ELSIM_TRY(ensure_init());
+  return solve_pot_dev(n1, n2, n3, dx, kap, rhs, phi, bc_lo, bc_hi, mode, rtol, maxit, iters, relres);
+}
+
+int elsim_solve_pot(const int* n1, const int* n2, const int* n3, const double* dx, const double* kap,
+                    const double* rhs, double* phi, const int* bc_lo, const int* bc_hi, const int* mode,
+                    const double* rtol, const int* maxit, int* iters, double* relres) {
+  if (!n1 || !n2 || !n3 || !dx || !bc_lo || !bc_hi || !mode || !rtol || !maxit) {
+    set_error("null scalar argument");
+    return ELSIM_ERR_ARG;
+  }
+  ELSIM_TRY(pot_args_ok(*n1, *n2, *n3, *dx, kap, rhs, phi, *bc_lo, *bc_hi, *mode, *rtol, *maxit));
+  ELSIM_TRY(ensure_init());
+  const Dims d{*n1, *n2, *n3};
+  const size_t G = (size_t)d.G() * 8, N = (size_t)d.N() * 8;
+  double* dk = (double*)scratch(5, G);
+  double* dr = (double*)scratch(6, N);
+  double* dp = (double*)scratch(7, G);
+  if (!dk || !dr || !dp) return ELSIM_ERR_ALLOC;
+  hipStream_t s = stream();
+  ELSIM_HIP(hipMemcpyAsync(dk, kap, G, hipMemcpyHostToDevice, s));
+  ELSIM_HIP(hipMemcpyAsync(dr, rhs, N, hipMemcpyHostToDevice, s));
+  ELSIM_HIP(hipMemcpyAsync(dp, phi, G, hipMemcpyHostToDevice, s));
+  const int rc = solve_pot_dev(*n1, *n2, *n3, *dx, dk, dr, dp, *bc_lo, *bc_hi, *mode, *rtol, *maxit, iters, relres);
+  if (rc < 0) return rc;
+  ELSIM_HIP(hipMemcpyAsync(phi, dp, G, hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  return rc;
 }
 
 int elsim_tbl_chk_pln_3da(const int* py, const int* px, const int* pz, const int* sz, const double* cord,

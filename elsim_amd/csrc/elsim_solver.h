@@ -27,7 +27,8 @@ struct CgDev {
   int need_reliable;
   int n_reliable;
   int stop_rule;
-  int pending_y;   // mixed: this iteration's y += alpha p is still to be applied (by k_mx_dir)
+  int rel_r0;      // the relative rule's norm is ||r0|| instead of ||b_eff||
+  int pending_y;   // mixed: this iteration's y += alpha p is still to be applied (by k_mx_dir3)
 };
 
 // Device buffers of one grid's solves (allocated once per grid).
@@ -35,7 +36,7 @@ struct CgWork {
   Dims d{0, 0, 0};
   DevBuf res, wmx, prj, x0, r0;      // FP64, G each (x0: the solve's start)
   DevBuf e64;                        // FP64 G (correction, ghosted for the stencil)
-  DevBuf y32, r32, w32, p32;         // FP32: N, N, N, G
+  DevBuf y32, r32, w32, p32;         // FP32, ghosted (even length: pair kernels)
   DevBuf kap32, pse32, dinv32;       // FP32: G, G; BF16 Jacobi inverse: G
   DevBuf dinv64;                     // FP64: N (ELSIM_MODE_FP64_PCG)
   DevBuf cg;                         // CgDev
@@ -59,6 +60,7 @@ struct SolveArgs {
   const double* w8;     // ghosted weight of the reference stop norm (psE or psP)
   double tw;            // its interior total
   double delta;         // mixed: reliable-update trigger
+  int rel_r0 = 0;       // relative rule against ||r0|| (the start's residual) instead of ||b_eff||
 };
 
 struct SolveResult {

@@ -53,6 +53,10 @@ template <>
 struct Rule<ELSIM_OP_PHE> {
   static constexpr bool lo_zero = true, hi_zero = false, diag = false;
 };
+template <>
+struct Rule<ELSIM_OP_POT_DD> {
+  static constexpr bool lo_zero = true, hi_zero = true, diag = false;
+};
 
 // A p over one stencil tile (256 d1 voxels x SJ rows x up to SKC planes);
 // epi(c_ghosted, n_interior, p_centre, value) for every interior voxel of the
@@ -65,9 +69,11 @@ struct Rule<ELSIM_OP_PHE> {
 // L2 or the shared last-level cache), instead of one tile-time apart, which
 // cut the operator's fetched bytes from 1.13 to 1.07 times its algorithmic
 // bytes at 256^3.  Per-voxel arithmetic does not depend on the direction.
-template <typename T, int OP, bool HOM, class Epi>
-__device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __restrict__ p,
-                                      const T* __restrict__ kap, const T* __restrict__ pse, T scale, Epi&& epi) {
+// The values of p come from ld(q) (q a ghosted index), so that a caller can
+// form p on the fly from other arrays (the fused CG direction update).
+template <typename T, int OP, bool HOM, class Ld, class Epi>
+__device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld, const T* __restrict__ kap,
+                                         const T* __restrict__ pse, T scale, Epi&& epi) {
   constexpr int SJ = Rows<T>::SJ;
   const int64_t sz = d.g12(), nz = (int64_t)d.n1 * d.n2;
   const int i = s.i;
@@ -86,10 +92,10 @@ __device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __
   T pc[SJ], kc[SJ], pb[SJ], kb[SJ];  // plane k, the plane behind (k - direction)
 #pragma unroll
   for (int t = 0; t < SJ; t++) {
-    pc[t] = p[rc[t]];
+    pc[t] = ld(rc[t]);
     kc[t] = kap[rc[t]];
     kb[t] = kap[rc[t] - step];
-    pb[t] = p[rc[t] - step];
+    pb[t] = ld(rc[t] - step);
   }
   int64_t off = 0, noff = (int64_t)(i - 1) + (int64_t)d.n1 * ((s.j0 - 1) + (int64_t)d.n2 * (kstart - 1));
   const int nk = s.k1 - s.k0 + 1;
@@ -98,11 +104,11 @@ __device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __
     T pa[SJ], ka[SJ], ps[SJ];  // plane ahead (k + direction)
 #pragma unroll
     for (int t = 0; t < SJ; t++) {
-      pa[t] = p[rc[t] + off + step];
+      pa[t] = ld(rc[t] + off + step);
       ka[t] = kap[rc[t] + off + step];
       if (Rule<OP>::diag) ps[t] = pse[rc[t] + off];
     }
-    const T pjlo = p[clo + off], kjlo = kap[clo + off], pjhi = p[chi + off], kjhi = kap[chi + off];
+    const T pjlo = ld(clo + off), kjlo = kap[clo + off], pjhi = ld(chi + off), kjhi = kap[chi + off];
     T pim[SJ], pip[SJ], kim[SJ], kip[SJ];
 #pragma unroll
     for (int t = 0; t < SJ; t++) {
@@ -114,14 +120,14 @@ __device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __
     if (ld_left) {
 #pragma unroll
       for (int t = 0; t < SJ; t++) {
-        pim[t] = p[rc[t] + off - 1];
+        pim[t] = ld(rc[t] + off - 1);
         kim[t] = kap[rc[t] + off - 1];
       }
     }
     if (ld_right) {
 #pragma unroll
       for (int t = 0; t < SJ; t++) {
-        pip[t] = p[rc[t] + off + 1];
+        pip[t] = ld(rc[t] + off + 1);
         kip[t] = kap[rc[t] + off + 1];
       }
     }
@@ -168,6 +174,13 @@ __device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __
       kc[t] = ka[t];
     }
   }
+}
+
+template <typename T, int OP, bool HOM, class Epi>
+__device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __restrict__ p,
+                                      const T* __restrict__ kap, const T* __restrict__ pse, T scale, Epi&& epi) {
+  sweep_ld<T, OP, HOM>(
+      d, s, [p](int64_t q) { return p[q]; }, kap, pse, scale, epi);
 }
 
 // Pointwise walk over a block's columns (interior only).
@@ -314,7 +327,8 @@ __global__ __launch_bounds__(256) void k_diff(Dims d, const double* __restrict__
 
 // FP64 true residual of the correction system: r = r0 - A_hom e.
 // FINAL: only record rr_true.  Otherwise (mixed reliable update): r32 = r,
-// and the Krylov scalars restart from the true residual.
+// and partials of r.r (FP64) and r32.(dinv r32), from which the FP32 Krylov
+// recurrence restarts.
 template <int OP, bool FINAL>
 __global__ __launch_bounds__(256) void k_true_resid(Dims d, const double* __restrict__ e, const double* __restrict__ kap,
                                                     const double* __restrict__ pse, double scale,
@@ -463,31 +477,10 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
   FLAT_LOOP(n, { v[u] = a[q]; }, { b[q] = (float)v[u]; })
 }
 
-// Mixed start: r32 = r0, p32 = dinv r32, y32 = 0, e64 = 0; rz = r.z.
-__global__ __launch_bounds__(256) void k_mx_start(int64_t G, const double* __restrict__ r0, const uint16_t* __restrict__ dinv,
-                                                  float* __restrict__ r32, float* __restrict__ p32,
-                                                  float* __restrict__ y32, double* __restrict__ e64, CgDev* cg,
-                                                  double* partials) {
-  double acc[1] = {0.0};
-  double rv[FLAT_U];
-  float dv[FLAT_U];
-  FLAT_LOOP(G, { rv[u] = r0[q]; dv[u] = bf16_to_f32(dinv[q]); }, {
-    const float r = (float)rv[u];
-    const float z = dv[u] * r;
-    r32[q] = r;
-    p32[q] = z;
-    y32[q] = 0.0f;
-    e64[q] = 0.0;
-    acc[0] += (double)r * (double)z;
-  })
-  block_partials<1>(acc, partials);
-}
-
-// The two per-iteration vector kernels of the mixed solver work on element
-// pairs: a lane loads 8 B of each FP32 vector and one 4 B word holding two
-// BF16 inverse diagonals (16-bit loads measured slower than the FP32 kernel
-// they replace).  The FP32 work vectors are allocated with an even length;
-// the padding element is zero and stays zero.
+// The vector kernels of the mixed solver work on element pairs: a lane
+// loads 8 B of each FP32 vector and one 4 B word holding two BF16 inverse
+// diagonals (16-bit loads measured slower).  The FP32 work vectors are
+// allocated with an even length; the padding element is zero and stays zero.
 struct BfPair {
   float a, b;
 };
@@ -495,17 +488,46 @@ __device__ __forceinline__ BfPair bf16x2(uint32_t w) {
   return BfPair{bf16_to_f32((uint16_t)(w & 0xffffu)), bf16_to_f32((uint16_t)(w >> 16))};
 }
 
-// r -= alpha w; rz' = r.(dinv r); decide on a reliable update.  The
-// solution update y += alpha p of the same iteration is left to k_mx_dir,
-// which reads p anyway (4 B per element less than doing it here).
-// UPD_P pairs per thread (4096 elements per block): a quarter of the blocks of
-// the other flat kernels, so the finalising kernel that follows every
-// iteration reads a quarter of the partials (2, 4, 8 and 16 pairs measured at
-// 256^3: 8 is the fastest).  G2 = number of element pairs.
+// UPD_P element pairs per thread in the update and start kernels (4096
+// elements per block: a quarter of the blocks of the other flat kernels, so
+// the finalising kernel that follows reads a quarter of the partials; 2, 4,
+// 8 and 16 pairs measured at 256^3: 8 is the fastest).
 constexpr int UPD_P = 8;
-constexpr int UPD_U = 2 * UPD_P;
-__global__ __launch_bounds__(256) void k_mx_update(int64_t G2, float2* __restrict__ r32, const float2* __restrict__ w32,
-                                                   const uint32_t* __restrict__ dinv, CgDev* cg, double* partials) {
+
+// Mixed start: r32 = r0, p32 = dinv r32 (the first direction), y32 = 0,
+// e64 = 0; partials of r.r and r.z.
+__global__ __launch_bounds__(256) void k_mx_start(int64_t G2, const double2* __restrict__ r0,
+                                                  const uint32_t* __restrict__ dinv, float2* __restrict__ r32,
+                                                  float2* __restrict__ p32, float2* __restrict__ y32,
+                                                  double2* __restrict__ e64, double* partials) {
+  double acc[2] = {0.0, 0.0};
+  double2 rv[UPD_P];
+  uint32_t dv[UPD_P];
+  FLAT_LOOP_U(UPD_P, G2, { rv[u] = r0[q]; dv[u] = dinv[q]; }, {
+    const BfPair dd = bf16x2(dv[u]);
+    float2 r;
+    float2 z;
+    r.x = (float)rv[u].x;
+    r.y = (float)rv[u].y;
+    z.x = dd.a * r.x;
+    z.y = dd.b * r.y;
+    r32[q] = r;
+    p32[q] = z;
+    y32[q] = make_float2(0.f, 0.f);
+    e64[q] = make_double2(0.0, 0.0);
+    acc[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+    acc[1] += (double)r.x * (double)z.x + (double)r.y * (double)z.y;
+  })
+  block_partials<2>(acc, partials);
+}
+
+// One iteration of the mixed solver is the FP32 operator (k_matvec, with the
+// p.Ap partials), then the two flat kernels below, each followed by a
+// one-block finalising kernel.
+// r -= alpha w; partials of r.z and r.r.  The solution update of the same
+// iteration is left to k_mx_dir3, which reads p anyway.
+__global__ __launch_bounds__(256) void k_mx_upd3(int64_t G2, float2* __restrict__ r32, const float2* __restrict__ w32,
+                                                 const uint32_t* __restrict__ dinv, CgDev* cg, double* partials) {
   if (cg->done || cg->need_reliable) return;
   const float alp = (float)cg->alpha;
   double acc[2] = {0.0, 0.0};
@@ -517,36 +539,19 @@ __global__ __launch_bounds__(256) void k_mx_update(int64_t G2, float2* __restric
     rn.x = rv[u].x - alp * wv[u].x;
     rn.y = rv[u].y - alp * wv[u].y;
     r32[q] = rn;
-    acc[0] += (double)rn.x * (double)(dd.a * rn.x);
-    acc[0] += (double)rn.y * (double)(dd.b * rn.y);
-    acc[1] += (double)rn.x * (double)rn.x;
-    acc[1] += (double)rn.y * (double)rn.y;
+    acc[0] += (double)rn.x * (double)(dd.a * rn.x) + (double)rn.y * (double)(dd.b * rn.y);
+    acc[1] += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
   })
   block_partials<2>(acc, partials);
 }
 
-// Reliable update, part 1: e64 += y32, y32 = 0.
-__global__ __launch_bounds__(256) void k_mx_fold(int64_t G, double* __restrict__ e64, float* __restrict__ y32,
-                                                 const CgDev* cg) {
-  if (cg->done || !cg->need_reliable) return;
-  double ev[FLAT_U];
-  float yv[FLAT_U];
-  FLAT_LOOP(G, { ev[u] = e64[q]; yv[u] = y32[q]; }, {
-    e64[q] = ev[u] + (double)yv[u];
-    y32[q] = 0.0f;
-  })
-}
-
 // y += alpha p (the solution update of the iteration whose residual update
 // ran, flagged by pending_y), then p = dinv r + beta p unless the iteration
-// ended the batch (converged, or a reliable update is due).  Element pairs,
-// DIR_P per thread (2 measured 5 % faster than 4 or 8 at 512^3 from HBM, and
-// equal at 256^3).  The flags are uniform over the grid, so each launch takes
-// one of the three loops.
+// ended the batch.  DIR_P element pairs per thread.
 constexpr int DIR_P = 2;
-__global__ __launch_bounds__(256) void k_mx_dir(int64_t G2, float2* __restrict__ y32, float2* __restrict__ p32,
-                                                const float2* __restrict__ r32, const uint32_t* __restrict__ dinv,
-                                                const CgDev* cg) {
+__global__ __launch_bounds__(256) void k_mx_dir3(int64_t G2, float2* __restrict__ y32, float2* __restrict__ p32,
+                                                 const float2* __restrict__ r32, const uint32_t* __restrict__ dinv,
+                                                 const CgDev* cg) {
   const bool upd_y = cg->pending_y != 0, upd_p = !(cg->done || cg->need_reliable);
   const float alp = (float)cg->alpha, bet = (float)cg->beta;
   if (upd_y && upd_p) {
@@ -584,6 +589,18 @@ __global__ __launch_bounds__(256) void k_mx_dir(int64_t G2, float2* __restrict__
   }
 }
 
+// Reliable update, part 1: e64 += y32, y32 = 0.
+__global__ __launch_bounds__(256) void k_mx_fold(int64_t G, double* __restrict__ e64, float* __restrict__ y32,
+                                                 const CgDev* cg) {
+  if (cg->done || !cg->need_reliable) return;
+  double ev[FLAT_U];
+  float yv[FLAT_U];
+  FLAT_LOOP(G, { ev[u] = e64[q]; yv[u] = y32[q]; }, {
+    e64[q] = ev[u] + (double)yv[u];
+    y32[q] = 0.0f;
+  })
+}
+
 // x += e (ghosts of e are zero).
 __global__ __launch_bounds__(256) void k_add_e(int64_t G, double* __restrict__ x, const double* __restrict__ e) {
   double xv[FLAT_U], ev[FLAT_U];
@@ -605,10 +622,11 @@ enum FinOp {
   FIN_BEFF,           // bn2, tol2, relative-rule early exit
   FIN_CG64,           // reference CG: beta, stop rules
   FIN_TRUE_FINAL,     // rr_true
-  FIN_TRUE_RELIABLE,  // mixed reliable update: restart from the true residual
+  FIN_TRUE_RELIABLE,  // mixed reliable update: done, cap, or restart from the true residual
   FIN_RZ,             // rz = sum
   FIN_PCG64,          // FP64 Jacobi-PCG: beta, stop
-  FIN_MX_UPDATE       // mixed: beta or request a reliable update
+  FIN_MX_UPDATE,      // three-kernel mixed path: beta or request a reliable update
+  FIN_MX_START        // three-kernel mixed path: rz of the start (second value)
 };
 
 // The skip test repeats the producer's, evaluated on the same (unchanged)
@@ -622,8 +640,8 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
     if (op == FIN_CG64 || op == FIN_PCG64) skip = cg->done;
     if (op == FIN_TRUE_RELIABLE) skip = cg->done || !cg->need_reliable;
   }
-  // mixed path: the iteration whose alpha this is will update r, and its
-  // y += alpha p falls to k_mx_dir; an iteration skipped here updates nothing
+  // three-kernel mixed path: the iteration whose alpha this is will update
+  // r, and its y += alpha p falls to k_mx_dir3
   if (cg && op == FIN_ALPHA_RZ && linear_tid() == 0) cg->pending_y = skip ? 0 : 1;
   if (skip) return;
   // the operations of the two-value reductions read tot[1]; with NV = 1 it
@@ -648,10 +666,13 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
       cg->rmax2 = tot[0];
       break;
     case FIN_BEFF:
-      cg->bn2 = tot[0];
-      cg->tol2 = rtol * rtol * tot[0];
+      cg->bn2 = cg->rel_r0 ? cg->rr : tot[0];
+      cg->tol2 = rtol * rtol * cg->bn2;
       cg->rr_true = cg->rr;
-      if (cg->stop_rule == ELSIM_STOP_RELRES && !(cg->rr > cg->tol2)) cg->done = 1;
+      if (!isfinite(cg->rr) || !isfinite(tot[0]))
+        cg->done = 3;  // a non-finite start or right-hand side is an error, never "converged"
+      else if (cg->stop_rule == ELSIM_STOP_RELRES && !(cg->rr > cg->tol2))
+        cg->done = 1;
       break;
     case FIN_CG64: {
       const double alp = cg->alpha, dmy = cg->rr, rrn = tot[0];
@@ -673,7 +694,6 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
     }
     case FIN_TRUE_FINAL: cg->rr_true = tot[0]; break;
     case FIN_TRUE_RELIABLE:
-      cg->pending_y = 0;  // y was folded into e64 after the last iteration's k_mx_dir
       cg->rr_true = tot[0];
       cg->n_reliable += 1;
       cg->need_reliable = 0;
@@ -684,12 +704,33 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
       } else if (cg->iters >= cg->maxit) {
         cg->done = 2;
       } else {
-        cg->beta = tot[1] / cg->rz;
+        // Restart the FP32 recurrence from the true residual: the next
+        // k_mx_dir3 forms p = dinv r (beta = 0).  With FP32 vectors on these
+        // ill-conditioned systems the recurred residual drifts from the true
+        // one between reliable updates; continuing the old direction with the
+        // replaced residual (beta = r.z ratio) measured 825 phP iterations
+        // per step at 256^3 against 271 with the restart (same start and
+        // tolerance).
+        cg->pending_y = 0;
+        cg->beta = 0.0;
         cg->rz = tot[1];
         cg->rmax2 = tot[0];
       }
       break;
     case FIN_RZ: cg->rz = tot[0]; break;
+    case FIN_MX_START: cg->rz = tot[1]; break;
+    case FIN_MX_UPDATE:
+      cg->iters += 1;
+      cg->rr = tot[1];
+      if (!isfinite(cg->alpha) || !isfinite(tot[0])) {
+        cg->done = 3;
+      } else if (!(tot[1] > cg->delta2 * cg->rmax2) || !(tot[1] > cg->tol2) || cg->iters >= cg->maxit) {
+        cg->need_reliable = 1;
+      } else {
+        cg->beta = tot[0] / cg->rz;
+        cg->rz = tot[0];
+      }
+      break;
     case FIN_PCG64: {
       const double alp = cg->alpha;
       cg->iters += 1;
@@ -705,18 +746,6 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
         cg->done = 2;
       break;
     }
-    case FIN_MX_UPDATE:
-      cg->iters += 1;
-      cg->rr = tot[1];
-      if (!isfinite(cg->alpha) || !isfinite(tot[0])) {
-        cg->done = 3;
-      } else if (!(tot[1] > cg->delta2 * cg->rmax2) || !(tot[1] > cg->tol2) || cg->iters >= cg->maxit) {
-        cg->need_reliable = 1;
-      } else {
-        cg->beta = tot[0] / cg->rz;
-        cg->rz = tot[0];
-      }
-      break;
   }
 }
 
@@ -728,19 +757,20 @@ int CgWork::alloc(const Dims& dd) {
   const size_t G = (size_t)d.G(), Ge = (G + 1) & ~(size_t)1;  // even length: pair kernels
   DevBuf* g64[] = {&res, &wmx, &prj, &x0, &r0, &e64, &dinv64};
   DevBuf* g32[] = {&y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32};
-  for (DevBuf* b : g64) ELSIM_TRY(dev_alloc(*b, G * 8));
+  for (DevBuf* b : g64) ELSIM_TRY(dev_alloc(*b, Ge * 8));
   for (DevBuf* b : g32) ELSIM_TRY(dev_alloc(*b, Ge * 4));
   ELSIM_TRY(dev_alloc(cg, sizeof(CgDev)));
   if (!host) ELSIM_HIP(hipHostMalloc((void**)&host, sizeof(CgDev), hipHostMallocDefault));
   // every work vector keeps zero ghosts (the kernels only write interiors
   // or add zero ghosts to zero ghosts)
-  for (DevBuf* b : g64) ELSIM_HIP(hipMemsetAsync(b->p, 0, G * 8, stream()));
+  for (DevBuf* b : g64) ELSIM_HIP(hipMemsetAsync(b->p, 0, Ge * 8, stream()));
   for (DevBuf* b : g32) ELSIM_HIP(hipMemsetAsync(b->p, 0, Ge * 4, stream()));
   return ELSIM_OK;
 }
 
 void CgWork::release() {
-  DevBuf* bs[] = {&res, &wmx, &prj, &x0, &r0, &e64, &y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32, &dinv64, &cg};
+  DevBuf* bs[] = {&res, &wmx, &prj, &x0, &r0, &e64, &y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32, &dinv64,
+                  &cg};
   for (DevBuf* b : bs) dev_free(*b);
   if (host) (void)hipHostFree(host);
   host = nullptr;
@@ -793,7 +823,6 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   const int fb = flat_blocks(G);
   const int64_t G2 = (G + 1) / 2;  // element pairs of the FP32 pair kernels
   const int fbu = (int)((G2 + (int64_t)FLAT_T * UPD_P - 1) / ((int64_t)FLAT_T * UPD_P));
-  const int fbd = (int)((G2 + (int64_t)FLAT_T * DIR_P - 1) / ((int64_t)FLAT_T * DIR_P));
   if (nb64 > kMaxRedBlocks || nb32 > kMaxRedBlocks || fb > kMaxRedBlocks) {
     set_error("grid too large for the reduction workspace");
     return ELSIM_ERR_ARG;
@@ -805,6 +834,8 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   h.maxit = a.maxit;
   h.stop_rule = mode == ELSIM_MODE_FP64 ? a.stop_rule : ELSIM_STOP_RELRES;
   h.delta2 = a.delta * a.delta;
+  h.rel_r0 = a.rel_r0;
+
   ELSIM_HIP(hipMemcpyAsync(cg, &h, sizeof(CgDev), hipMemcpyHostToDevice, s));
   double* res = (double*)w.res.p;
   double* wmx = (double*)w.wmx.p;
@@ -871,32 +902,33 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     fin2(FIN_TRUE_FINAL, nb64);
   } else {
     uint16_t* dinv = (uint16_t*)w.dinv32.p;  // BF16
+    float *r32 = (float*)w.r32.p, *w32 = (float*)w.w32.p, *y32 = (float*)w.y32.p, *p32 = (float*)w.p32.p;
     k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.kap, (float*)w.kap32.p);
     if (OP == ELSIM_OP_CNE) k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.pse, (float*)w.pse32.p);
     k_dinv<OP, uint16_t><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv);
-    k_mx_start<<<fb, FLAT_T, 0, s>>>(G, r0, dinv, (float*)w.r32.p, (float*)w.p32.p, (float*)w.y32.p, e64, cg, P);
-    fin1(FIN_RZ, fb);
+    k_mx_start<<<fbu, FLAT_T, 0, s>>>(G2, (const double2*)r0, (const uint32_t*)dinv, (float2*)r32, (float2*)p32,
+                                      (float2*)y32, (double2*)e64, P);
+    fin2(FIN_MX_START, fbu);
     ELSIM_CHECK_LAUNCH();
     const float sc32 = (float)a.scale;
+    const int fbd = (int)((G2 + (int64_t)FLAT_T * DIR_P - 1) / ((int64_t)FLAT_T * DIR_P));
     int it_prev = 0;
     for (;;) {
       const bool pr = g_prof.on;
       for (int b = 0; b < batch; b++) {
         const bool t = pr && b == 0;
         ELSIM_TRY(timed(t, ELSIM_PROF_OP32, s, [&] {
-          k_matvec<float, OP, true, true><<<nb32, blk, 0, s>>>(d, (const float*)w.p32.p, (const float*)w.kap32.p,
-                                                               (const float*)w.pse32.p, sc32, (float*)w.w32.p,
-                                                               nullptr, cg, 1, P);
+          k_matvec<float, OP, true, true><<<nb32, blk, 0, s>>>(d, p32, (const float*)w.kap32.p,
+                                                               (const float*)w.pse32.p, sc32, w32, nullptr, cg, 1, P);
         }));
         fin1(FIN_ALPHA_RZ, nb32);
         ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] {
-          k_mx_update<<<fbu, FLAT_T, 0, s>>>(G2, (float2*)w.r32.p, (const float2*)w.w32.p, (const uint32_t*)dinv,
-                                             cg, P);
+          k_mx_upd3<<<fbu, FLAT_T, 0, s>>>(G2, (float2*)r32, (const float2*)w32, (const uint32_t*)dinv, cg, P);
         }));
         fin2(FIN_MX_UPDATE, fbu);
         ELSIM_TRY(timed(t, ELSIM_PROF_DIR32, s, [&] {
-          k_mx_dir<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)w.y32.p, (float2*)w.p32.p, (const float2*)w.r32.p,
-                                          (const uint32_t*)dinv, cg);
+          k_mx_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)r32,
+                                           (const uint32_t*)dinv, cg);
         }));
       }
       ELSIM_CHECK_LAUNCH();
@@ -904,17 +936,17 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
       if (pr && w.host->iters >= it_prev + 2) ELSIM_TRY(prof_collect(ELSIM_PROF_OP32, ELSIM_PROF_DIR32));
       it_prev = w.host->iters;
       if (w.host->done) break;
-      // Once the FP32 residual has dropped by delta (or meets the tolerance)
-      // the device sets need_reliable and the rest of the batch is skipped;
-      // the reliable update runs here, between batches, so the iterations
-      // that do not need it carry no skipped launches.
+      // Once the FP32 residual has dropped by delta (or meets the tolerance,
+      // or the cap is reached) the device sets need_reliable and the rest of
+      // the batch is skipped; the reliable update runs here, between
+      // batches: fold y into the FP64 correction, FP64 true residual, and a
+      // restart of the FP32 recurrence from it.
       if (w.host->need_reliable) {
-        k_mx_fold<<<fb, FLAT_T, 0, s>>>(G, e64, (float*)w.y32.p, cg);
-        k_true_resid<OP, false><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, (float*)w.r32.p, dinv, cg,
-                                                     P);
+        k_mx_fold<<<fb, FLAT_T, 0, s>>>(G, e64, y32, cg);
+        k_true_resid<OP, false><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, r32, dinv, cg, P);
         fin2(FIN_TRUE_RELIABLE, nb64);
-        k_mx_dir<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)w.y32.p, (float2*)w.p32.p, (const float2*)w.r32.p,
-                                        (const uint32_t*)dinv, cg);
+        k_mx_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)r32, (const uint32_t*)dinv,
+                                         cg);
         ELSIM_CHECK_LAUNCH();
         ELSIM_TRY(poll(w, s));
         if (w.host->done) break;
@@ -1020,6 +1052,7 @@ int cg_solve(CgWork& w, const SolveArgs& a, int mode, SolveResult* res) {
     case ELSIM_OP_CNE: return solve_op<ELSIM_OP_CNE>(w, a, mode, res);
     case ELSIM_OP_PHP: return solve_op<ELSIM_OP_PHP>(w, a, mode, res);
     case ELSIM_OP_PHE: return solve_op<ELSIM_OP_PHE>(w, a, mode, res);
+    case ELSIM_OP_POT_DD: return solve_op<ELSIM_OP_POT_DD>(w, a, mode, res);
   }
   set_error("unknown operator %d", a.op);
   return ELSIM_ERR_ARG;
@@ -1045,6 +1078,8 @@ static int launch_mv(int op, const Dims& d, const T* p, const T* kap, const T* p
     if (hom) MV(ELSIM_OP_PHP, true); else MV(ELSIM_OP_PHP, false);
   } else if (op == ELSIM_OP_PHE) {
     if (hom) MV(ELSIM_OP_PHE, true); else MV(ELSIM_OP_PHE, false);
+  } else if (op == ELSIM_OP_POT_DD) {
+    if (hom) MV(ELSIM_OP_POT_DD, true); else MV(ELSIM_OP_POT_DD, false);
   } else {
     set_error("unknown operator %d", op);
     return ELSIM_ERR_ARG;

@@ -8,7 +8,8 @@
 !! and the subroutines carved out of the inline blocks of the SBM and AD
 !! programs (SBM:397-840, AccDiff_3D.f90:103-243, RM_UCNCT_3D.f90:79-82):
 !!   SBM_Init / SBM_Step / SBM_Get / SBM_Set / SBM_Free, AccDiff_Phase,
-!!   Label_Connected, RM_Threshold; LS_FrontEnd, the inline geometry
+!!   Label_Connected, RM_Threshold; SBM_Solve_Pot, one potential solve of the
+!!   SBM program's inline blocks (SBM:643-723, 725-804); LS_FrontEnd, the inline geometry
 !!   front-end of LevelSet_3D's frame loop (LevelSet_3D.f90:84-144); RM_Phase,
 !!   the sign block of one phase of RM_UCNCT_3D (RM_UCNCT_3D.f90:64-96,
 !!   115-136); and Refine2_SDF, the 2x refinement to FP32 that the reference
@@ -170,6 +171,24 @@ SUBROUTINE SBM_Set(handle, field, arr)
    REAL(KIND=8), INTENT(IN) :: arr(*)
    CALL elsim_check(elsim_sbm_set(handle, field, arr), 'SBM_Set')
 END SUBROUTINE SBM_Set
+
+!! One potential solve (SBM:657-714 for phP, 737-795 for phE): kap, phi
+!! ghosted (0:py+1,0:px+1,0:pz+1), rhs interior; bclo / bchi = 0 Neumann, 1
+!! Dirichlet on the low / high d2 ghost plane (the value phi holds there).
+!! Errors and a solve that hits maxit end the program.
+SUBROUTINE SBM_Solve_Pot(py, px, pz, dx, kap, rhs, phi, bclo, bchi, mode, rtol, maxit, iters, relres)
+   USE elsim_gpu
+   IMPLICIT NONE
+   INTEGER, INTENT(IN) :: py, px, pz, bclo, bchi, mode, maxit
+   REAL(KIND=8), INTENT(IN) :: dx, rtol
+   REAL(KIND=8), INTENT(IN), DIMENSION(0:py+1,0:px+1,0:pz+1) :: kap
+   REAL(KIND=8), INTENT(IN), DIMENSION(py,px,pz) :: rhs
+   REAL(KIND=8), INTENT(INOUT), DIMENSION(0:py+1,0:px+1,0:pz+1) :: phi
+   INTEGER, INTENT(OUT) :: iters
+   REAL(KIND=8), INTENT(OUT) :: relres
+   CALL elsim_check_converged(elsim_solve_pot(py, px, pz, dx, kap, rhs, phi, bclo, bchi, mode, rtol, maxit, &
+      iters, relres), 'SBM_Solve_Pot')
+END SUBROUTINE SBM_Solve_Pot
 
 SUBROUTINE SBM_Free(handle)
    USE elsim_gpu

@@ -16,7 +16,7 @@ MODULE elsim_gpu
          d0, tc1, tc2, dab_c0, dab_c1, dab_c2, dab_c3, kap_scale, i0_scale, kfw_cref, bvp, bve, vsr, trgi, &
          cne_target, rtol, crt_e, crt_p, crt_l, outer_tol, inner_rtol
       INTEGER(c_int32_t) :: mass_corr_period, skip_cne_first, stop_rule, max_it_e, max_it_p, max_it_l, &
-         max_outer, ws_mode
+         max_outer, ws_mode, cne_rule, reserved_
    END TYPE elsim_sbm_params
 
    TYPE, BIND(C) :: elsim_step_stats
@@ -132,6 +132,15 @@ MODULE elsim_gpu
          INTEGER(c_int), INTENT(IN) :: mode
          TYPE(elsim_step_stats), INTENT(OUT) :: stats
       END FUNCTION
+      INTEGER(c_int) FUNCTION elsim_solve_pot(n1, n2, n3, dx, kap, rhs, phi, bc_lo, bc_hi, mode, rtol, maxit, &
+         iters, relres) BIND(C, name="elsim_solve_pot")
+         IMPORT :: c_int, c_double
+         INTEGER(c_int), INTENT(IN) :: n1, n2, n3, bc_lo, bc_hi, mode, maxit
+         REAL(c_double), INTENT(IN) :: dx, rtol, kap(*), rhs(*)
+         REAL(c_double), INTENT(INOUT) :: phi(*)
+         INTEGER(c_int), INTENT(OUT) :: iters
+         REAL(c_double), INTENT(OUT) :: relres
+      END FUNCTION
       INTEGER(c_int) FUNCTION elsim_sbm_scalars(handle, tpp, tpe, xrf, bve, iter) BIND(C, name="elsim_sbm_scalars")
          IMPORT :: c_int, c_int64_t, c_double
          INTEGER(c_int64_t), INTENT(IN) :: handle
@@ -170,5 +179,18 @@ CONTAINS
       WRITE (0, '(A,A,I0,A,1024A)') what, ' failed with status ', rc, ': ', msg(1:n)
       ERROR STOP 2
    END SUBROUTINE elsim_check
+
+   !! As elsim_check, and a solve that stopped at its iteration cap also
+   !! stops the program (the reference leaves such a loop silently,
+   !! SBM:551, 710, 791).
+   SUBROUTINE elsim_check_converged(rc, what)
+      INTEGER(c_int), INTENT(IN) :: rc
+      CHARACTER(LEN=*), INTENT(IN) :: what
+      CALL elsim_check(rc, what)
+      IF (rc == ELSIM_NOT_CONVERGED) THEN
+         WRITE (0, '(A,A)') what, ' did not converge within its iteration cap'
+         ERROR STOP 3
+      END IF
+   END SUBROUTINE elsim_check_converged
 
 END MODULE elsim_gpu

@@ -82,6 +82,44 @@ def carve_tunnel(sdf, seed, radius=3.0):
     return flat.reshape((n1, n2, n3), order="F")
 
 
+def potential_problem(n, seed, rmin=4.0, rmax=8.0, target=0.5, xi=2.0, sigma=1.0, i0=1.0, alpha=0.5, eta=0.1,
+                      phase="solid"):
+    """One potential solve on a sphere pack: kappa = psi sigma + 1e-7 (the
+    reference's psi offset, SBM:247) with psi = 0.5(1+tanh(d/xi)) of the
+    chosen phase, rhs = |grad psi| j with the Butler-Volmer current
+    j = i0 (exp(alpha eta) - exp(-(1-alpha) eta)) at the fixed overpotential eta
+    (F/RT = 1), phi = 0 / 1 in the low / high d2 ghost planes (Dirichlet) and 0.5
+    inside.  The spheres do not overlap (hard core 1).  Returns (kappa, rhs,
+    phi0, info): kappa, phi0 ghosted, rhs interior, Fortran order."""
+    sdf, ns, frac = sphere_pack(n, seed, rmin, rmax, target=target, hardcore=1.0, connect=False)
+    d = sdf.astype(np.float64) if phase == "solid" else -sdf.astype(np.float64)
+    psi = np.pad(0.5 * (1.0 + np.tanh(d / xi)), 1, mode="edge")
+    kap = psi * sigma + 1.0e-7
+    g = [(np.roll(psi, -1, a) - np.roll(psi, 1, a))[1:-1, 1:-1, 1:-1] / 2 for a in range(3)]
+    grad = np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2)
+    j = i0 * (np.exp(alpha * eta) - np.exp(-(1.0 - alpha) * eta))
+    rhs = np.asfortranarray(grad * j)
+    phi = np.full(psi.shape, 0.5)
+    phi[:, 0, :] = 0.0
+    phi[:, -1, :] = 1.0
+    return np.asfortranarray(kap), rhs, np.asfortranarray(phi), {"spheres": ns, "solid_fraction": frac}
+
+
+def solve_pot(kap, rhs, phi, bc_lo=1, bc_hi=1, mode=MODE_FP64, dx=1.0, rtol=1e-8, maxit=100000):
+    """elsim_solve_pot on host arrays: returns (phi, status, iterations,
+    relres); phi's Dirichlet ghost planes hold the boundary values."""
+    n1, n2, n3 = rhs.shape
+    k = np.ascontiguousarray(np.asarray(kap, dtype=np.float64).ravel(order="F"))
+    r = np.ascontiguousarray(np.asarray(rhs, dtype=np.float64).ravel(order="F"))
+    x = np.array(np.asarray(phi, dtype=np.float64).ravel(order="F"), copy=True)  # the solve writes into it
+    it, rr = C.c_int(), C.c_double()
+    rc = lib().elsim_solve_pot(iref(n1), iref(n2), iref(n3), C.byref(C.c_double(dx)), ptr(k), ptr(r), ptr(x),
+                               iref(bc_lo), iref(bc_hi), iref(mode), C.byref(C.c_double(rtol)), iref(maxit),
+                               C.byref(it), C.byref(rr))
+    check(rc, "elsim_solve_pot")
+    return x.reshape(phi.shape, order="F"), rc, it.value, rr.value
+
+
 def tables(sz=101):
     t = np.zeros(8 * sz)
     check(lib().elsim_gen_tables(sz, ptr(t)), "gen_tables")

@@ -60,6 +60,8 @@ extern "C" {
 #define ELSIM_OP_CNE 0 /* A_c p = psE p - dt/dx^2 sum_f peN_f (p_nb - p); SBM:515-521; PRJ ghosts SBM:511-513 */
 #define ELSIM_OP_PHP 1 /* L p = dx^-2 sum_f pkN_f (p_nb - p); SBM:670-676; PRJ ghosts SBM:666-668 */
 #define ELSIM_OP_PHE 2 /* L p = dx^-2 sum_f pcN_f (p_nb - p); SBM:751-757; PRJ ghosts SBM:747-749 */
+#define ELSIM_OP_POT_DD 3 /* the potential operator with Dirichlet data on both d2 faces: zero search-direction
+                             ghosts there (elsim_solve_pot with bc_lo = bc_hi = 1) */
 
 /* Parameters of one SBM run.  Defaults (elsim_sbm_default_params) are the
  * reference's literal values with their REAL(4)/REAL(8) kinds resolved to the
@@ -107,6 +109,11 @@ typedef struct elsim_sbm_params {
   int32_t max_it_l;         /* SBM:791 */
   int32_t max_outer;        /* cap on SBM:627 (the reference has none) */
   int32_t ws_mode;          /* warm start of the first phP solve: ELSIM_WS_* */
+  int32_t cne_rule;         /* ELSIM_STOP_RELRES only: 0 = the CnE solve stops at rtol ||b_eff|| like the
+                               potentials; 1 = at rtol ||r0||, r0 = the residual of the last step's CnE,
+                               i.e. the step's change is resolved to rtol (CnE has no warm start, so this
+                               does not depend on a start policy) */
+  int32_t reserved_;        /* zero; keeps the struct free of implicit padding */
 } elsim_sbm_params;
 
 /* Warm start of a step's coupling loop (SBM:624-812), from the third step
@@ -297,6 +304,25 @@ int elsim_sbm_step(const int64_t* handle, const int* mode, elsim_step_stats* sta
  * parameters (elsim_sbm_set_params). */
 int elsim_sbm_time_control(const double* xrf, const double* scrnt, const double* trgi, const double* vsr0,
                            double* dt, double* vsr);
+/* One potential solve on its own: the reference's inline block of the
+ * particle potential (SBM:643-723) and electrolyte potential (SBM:725-804)
+ * solves as a subroutine.  Solves dx^-2 S(kap, phi) = rhs (S the 7-point sum
+ * of SBM:670-676 with face coefficients averaged from the cell-centred kap)
+ * for phi, starting from phi as given, to ||rhs - A phi|| <= rtol ||b_eff||
+ * (b_eff = the residual of a zero interior with phi's boundary data).
+ * Boundary conditions: Neumann (mirror ghosts) on the d1 and d3 faces; on
+ * each d2 face bc = 0 Neumann or 1 Dirichlet at the ghost-cell centre, the
+ * value phi holds in that ghost plane (at least one face Dirichlet; the
+ * reference's phP is (0, 1) with BvP, SBM:647, its phE (1, 0) with BvE,
+ * SBM:798-800).  kap, phi ghosted (0:n1+1, 0:n2+1, 0:n3+1), rhs interior.
+ * mode = ELSIM_MODE_* (FP64: the reference CG, SBM:661-714).  Returns 0,
+ * ELSIM_NOT_CONVERGED after maxit iterations, ELSIM_ERR_NAN on a non-finite
+ * value; argument errors are checked before any device call. */
+int elsim_solve_pot(const int* n1, const int* n2, const int* n3, const double* dx, const double* kap,
+                    const double* rhs, double* phi, const int* bc_lo, const int* bc_hi, const int* mode,
+                    const double* rtol, const int* maxit, int* iters, double* relres);
+int elsim_dev_solve_pot(int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi,
+                        int bc_lo, int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres);
 /* The Cahn-Hilliard stage alone (SBM:397-443). */
 int elsim_sbm_ch_update(const int64_t* handle);
 /* Homogeneous operator of the state's current coefficients applied to a

@@ -12,10 +12,12 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+LIB_OMP = os.path.join(HERE, "liboracle_omp.so")  # bench.py's CPU baseline only
 REF_LS = os.path.join(HERE, "_ref", "libref_ls.so")
 REF_RM = os.path.join(HERE, "_ref", "libref_rm.so")
 
 _o = None
+_omp = None
 
 
 def build():
@@ -27,14 +29,38 @@ def lib():
     if _o is None:
         if not os.path.exists(LIB):
             build()
-        _o = C.CDLL(LIB)
-        _o.or_sbm_create.restype = C.c_void_p
-        _o.or_table.restype = C.c_double
-        _o.or_table.argtypes = [C.c_double, C.c_void_p, C.c_void_p, C.c_int]
-        for f in ("or_sbm_step", "or_sbm_get", "or_sbm_set", "or_ch_update", "or_bv", "or_sbm_destroy",
-                  "or_sbm_scalars", "or_sbm_set_params"):
-            getattr(_o, f).argtypes = None
+        _o = _setup(C.CDLL(LIB))
     return _o
+
+
+def _setup(L):
+    L.or_sbm_create.restype = C.c_void_p
+    L.or_table.restype = C.c_double
+    L.or_table.argtypes = [C.c_double, C.c_void_p, C.c_void_p, C.c_int]
+    for f in ("or_sbm_step", "or_sbm_get", "or_sbm_set", "or_ch_update", "or_bv", "or_sbm_destroy",
+              "or_sbm_scalars", "or_sbm_set_params"):
+        getattr(L, f).argtypes = None
+    return L
+
+
+def omp_lib():
+    """The same oracle built with OpenMP (loops on every host core): used only
+    to time the CPU baseline, never as the checker."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(LIB_OMP):
+            build()
+        _omp = _setup(C.CDLL(LIB_OMP))
+        _omp.or_pcg_time.restype = C.c_double
+        _omp.or_threads.restype = C.c_int
+    return _omp
+
+
+def pcg_time(op, kap, pse, rhs, dt, dx, niter):
+    """Seconds for niter FP64 Jacobi-PCG iterations on the OpenMP build."""
+    n1, n2, n3 = rhs.shape
+    return omp_lib().or_pcg_time(op, n1, n2, n3, p(flat(kap)), p(flat(pse)), p(flat(rhs)), C.c_double(dt),
+                                 C.c_double(dx), niter)
 
 
 def p(a):
@@ -199,6 +225,61 @@ def ghost_mirror(x):
     return unflat(f, x.shape)
 
 
+def coef(params, which, cne, pse):
+    """Cell coefficient of the step's operators from CnE and psi_E (ghosted):
+    which 0 = CnE solve (SBM:474,478), 1 = RHE (SBM:579,583), 2 = phE solve
+    (SBM:601,604).  Returns (coefficient, Dab)."""
+    n1, n2, n3 = (s - 2 for s in cne.shape)
+    out, dab = np.zeros(cne.size), np.zeros(cne.size)
+    lib().or_coef(C.byref(params), n1, n2, n3, which, p(flat(cne)), p(flat(pse)), p(out), p(dab))
+    return unflat(out, cne.shape), unflat(dab, cne.shape)
+
+
+def rhe(pce, cne, dx):
+    """RHE = dx^-2 S(pce, CnE) over the interior (SBM:591-596)."""
+    n1, n2, n3 = (s - 2 for s in cne.shape)
+    out = np.zeros(n1 * n2 * n3)
+    lib().or_rhe(n1, n2, n3, p(flat(pce)), p(flat(cne)), C.c_double(dx), p(out))
+    return unflat(out, (n1, n2, n3))
+
+
+def avp(psp, pse, avp_min):
+    """Interface density AvP (SBM:274-289) of ghosted psi_P, psi_E."""
+    n1, n2, n3 = (s - 2 for s in psp.shape)
+    out = np.zeros(n1 * n2 * n3)
+    lib().or_avp(n1, n2, n3, p(flat(psp)), p(flat(pse)), C.c_double(avp_min), p(out))
+    return unflat(out, (n1, n2, n3))
+
+
+def solve_pot(kap, rhs, phi, bc_lo=1, bc_hi=1, dx=1.0, rtol=1e-8, maxit=100000):
+    """The oracle's potential solve (the step's CG, SBM:657-714) with the
+    ghost rules of elsim_solve_pot: mirror on the d1/d3 faces and a Neumann
+    d2 face, stored values on a Dirichlet d2 plane.  Returns (phi, status,
+    iterations, relres)."""
+    op = 3 if (bc_lo and bc_hi) else (1 if bc_hi else 2)
+    x = np.array(phi, dtype=np.float64)
+    lo, hi = x[:, 0, :].copy(), x[:, -1, :].copy()
+    x = ghost_mirror(x)
+    if bc_lo:
+        x[:, 0, :] = lo
+    if bc_hi:
+        x[:, -1, :] = hi
+    n1, n2, n3 = rhs.shape
+    xf = flat(x)
+    it, rr = C.c_int(), C.c_double()
+    st = lib().or_solve_system(op, n1, n2, n3, p(flat(kap)), p(flat(kap)), p(xf), p(flat(rhs)), C.c_double(1.0),
+                               C.c_double(dx), 1, C.c_double(rtol), C.c_double(0.0), maxit, p(flat(kap)),
+                               C.c_double(1.0), C.byref(it), C.byref(rr))
+    x = unflat(xf, x.shape)
+    lo, hi = x[:, 0, :].copy(), x[:, -1, :].copy()
+    x = ghost_mirror(x)
+    if bc_lo:
+        x[:, 0, :] = lo
+    if bc_hi:
+        x[:, -1, :] = hi
+    return x, st, it.value, rr.value
+
+
 def psi_from_sdf(sdf, zeta):
     n1, n2, n3 = sdf.shape
     psi = np.zeros((n1 + 2) * (n2 + 2) * (n3 + 2))
@@ -211,14 +292,15 @@ class OracleSbm:
 
     GHOSTED = {0, 1, 3, 4, 5, 6, 8}
 
-    def __init__(self, sdf_s, sdf_l, params, tab):
+    def __init__(self, sdf_s, sdf_l, params, tab, L=None):
+        self.L = L if L is not None else lib()
         self.shape = sdf_s.shape
         n1, n2, n3 = self.shape
         self.params = params
         self._s = flat(sdf_s, np.float32)
         self._l = flat(sdf_l, np.float32)
         self._t = np.ascontiguousarray(tab, dtype=np.float64)
-        self.h = C.c_void_p(lib().or_sbm_create(n1, n2, n3, C.byref(params), p(self._s), p(self._l),
+        self.h = C.c_void_p(self.L.or_sbm_create(n1, n2, n3, C.byref(params), p(self._s), p(self._l),
                                                  self._t.size // 8, p(self._t)))
 
     def _shape(self, f):
@@ -228,35 +310,35 @@ class OracleSbm:
     def get(self, f):
         shp = self._shape(f)
         out = np.zeros(int(np.prod(shp)))
-        lib().or_sbm_get(self.h, f, p(out))
+        self.L.or_sbm_get(self.h, f, p(out))
         return unflat(out, shp)
 
     def set(self, f, a):
         x = flat(a)
-        lib().or_sbm_set(self.h, f, p(x))
+        self.L.or_sbm_set(self.h, f, p(x))
 
     def step(self):
         from elsim_amd import StepStats
         st = StepStats()
-        lib().or_sbm_step(self.h, C.byref(st))
+        self.L.or_sbm_step(self.h, C.byref(st))
         return st
 
     def ch_update(self):
-        lib().or_ch_update(self.h)
+        self.L.or_ch_update(self.h)
 
     def set_params(self, params):
         self.params = params
-        lib().or_sbm_set_params(self.h, C.byref(params))
+        self.L.or_sbm_set_params(self.h, C.byref(params))
 
     def scalars(self):
         v = [C.c_double() for _ in range(4)]
         it = C.c_int()
-        lib().or_sbm_scalars(self.h, *[C.byref(x) for x in v], C.byref(it))
+        self.L.or_sbm_scalars(self.h, *[C.byref(x) for x in v], C.byref(it))
         return dict(tpp=v[0].value, tpe=v[1].value, xrf=v[2].value, bve=v[3].value, iter=it.value)
 
     def close(self):
         if self.h:
-            lib().or_sbm_destroy(self.h)
+            self.L.or_sbm_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -650,6 +650,7 @@ static inline double stencil_sum(int n1, int n2, const double* kap, const double
 /* Operator on a ghosted p (ghosts as stored); out interior. */
 void or_apply_op(int op, int n1, int n2, int n3, const double* kap, const double* pse, const double* p,
                  double dt, double dx, double* out) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) {
@@ -680,11 +681,18 @@ void or_prj_bc(int op, int n1, int n2, int n3, double* p) {
         p[GI(i, n2 + 1, k)] = 0.0;
         p[GI(i, 0, k)] = p[GI(i, 1, k)];
       }
-  } else {
+  } else if (op == ELSIM_OP_PHE) {
     /* SBM:747-749 */
     for (int k = 0; k <= n3 + 1; k++)
       for (int i = 0; i <= n1 + 1; i++) {
         p[GI(i, n2 + 1, k)] = p[GI(i, n2, k)];
+        p[GI(i, 0, k)] = 0.0;
+      }
+  } else {
+    /* Dirichlet data on both d2 faces: zero search-direction ghosts there */
+    for (int k = 0; k <= n3 + 1; k++)
+      for (int i = 0; i <= n1 + 1; i++) {
+        p[GI(i, n2 + 1, k)] = 0.0;
         p[GI(i, 0, k)] = 0.0;
       }
   }
@@ -736,6 +744,7 @@ static int or_cg(OrSbm* s, int op, double* x, const double* kap, int* status, do
   bn = sqrt(bn);
   double rr = 0.0;
   for (size_t q = 0; q < N; q++) rr += s->res[q] * s->res[q];
+  if (op == ELSIM_OP_CNE && P->cne_rule == 1) bn = sqrt(rr); /* relative to the step's change */
   int it = 0;
   *status = 0;
   if (P->stop_rule == ELSIM_STOP_RELRES && !(sqrt(rr) > P->rtol * bn)) {
@@ -841,6 +850,7 @@ void or_coef(const elsim_sbm_params* P, int n1, int n2, int n3, int which, const
              double* out, double* dab) {
   size_t G = (size_t)(n1 + 2) * (n2 + 2) * (n3 + 2);
   double sd0 = which == 0 ? P->d0 : (which == 1 ? P->tc1 * P->d0 : P->tc2 * P->d0);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < G; q++) {
     double d = dab_of(P, sd0, cne[q]);
     if (dab) dab[q] = d;
@@ -851,6 +861,7 @@ void or_coef(const elsim_sbm_params* P, int n1, int n2, int n3, int which, const
 /* RHE = dx^-2 S(pce, CnE) over the interior, with CnE's ghosts as stored
  * (SBM:591-596); pce is the which-1 coefficient of or_coef. */
 void or_rhe(int n1, int n2, int n3, const double* pce, const double* cne, double dx, double* rhe) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) rhe[NI(i, j, k)] = 1.0 / (dx * dx) * stencil_sum(n1, n2, pce, cne, i, j, k);
@@ -972,6 +983,7 @@ void or_ch_update(void* h) {
   const elsim_sbm_params* P = &s->p;
   double dx = P->dx;
   double* c = s->cnp;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) {
@@ -982,6 +994,7 @@ void or_ch_update(void* h) {
         s->lap[NI(i, j, k)] = v / (dx * dx);
       }
   const double* T = s->tab;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) {
@@ -999,6 +1012,7 @@ void or_ch_update(void* h) {
   const double* pm = s->pmn;
   const double* mu = s->mu;
   size_t sy = 1, sxx = (size_t)(n1 + 2), szz = (size_t)(n1 + 2) * (n2 + 2);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) {
@@ -1010,6 +1024,7 @@ void or_ch_update(void* h) {
         v = v - 0.5 * (pm[q] + pm[q - szz]) * (mu[q] - mu[q - szz]);
         s->dvc[NI(i, j, k)] = v / (dx * dx);
       }
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) {
@@ -1039,6 +1054,7 @@ void or_bv(void* h) {
   OrSbm* s = (OrSbm*)h;
   int n1 = s->n1, n2 = s->n2, n3 = s->n3;
   const elsim_sbm_params* P = &s->p;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) {
@@ -1238,4 +1254,85 @@ double or_bv_voxel(const elsim_sbm_params* P, const double* tab, int sz, double 
   double kfw = ioc / (P->frd * P->kfw_cref) * exp(0.5 * P->cst1 * ocv);
   double kbw = ioc / (P->frd * cp) * exp(-(0.5 * P->cst1 * ocv));
   return kfw * ce * exp(-(0.5 * P->cst1 * dph)) - kbw * cp * exp(0.5 * P->cst1 * dph);
+}
+
+/* ======================================================================= */
+/* CPU baseline timing (bench.py only)                                     */
+/* ======================================================================= */
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads the build runs its parallel loops on (1 without OpenMP). */
+int or_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* Seconds taken by `niter` iterations of FP64 Jacobi-preconditioned CG on the
+ * system of `op` (the algorithm of the FP64 GPU baseline: p = z + beta p,
+ * w = A_hom p, alpha = r.z / p.w, x += alpha p, r -= alpha w, z = dinv r),
+ * started from r = rhs, x = 0; kap ghosted, rhs interior.  The iterates are
+ * not used: this times the CPU cost per iteration of the same workload. */
+double or_pcg_time(int op, int n1, int n2, int n3, const double* kap, const double* pse, const double* rhs,
+                   double dt, double dx, int niter) {
+  size_t N = (size_t)n1 * n2 * n3, G = (size_t)(n1 + 2) * (n2 + 2) * (n3 + 2);
+  double *x = (double*)calloc(N, sizeof(double)), *r = (double*)malloc(N * sizeof(double));
+  double *z = (double*)malloc(N * sizeof(double)), *w = (double*)malloc(N * sizeof(double));
+  double *dinv = (double*)malloc(N * sizeof(double)), *p = (double*)calloc(G, sizeof(double));
+  const double sc = op == ELSIM_OP_CNE ? dt / (dx * dx) : 1.0 / (dx * dx);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) {
+        size_t c = GI(i, j, k), sx = (size_t)(n1 + 2), sz = sx * (n2 + 2);
+        double kc = kap[c], sum = (kap[c - 1] + kc) * 0.5 + (kap[c + 1] + kc) * 0.5 + (kap[c - sx] + kc) * 0.5 +
+                                  (kap[c + sx] + kc) * 0.5 + (kap[c - sz] + kc) * 0.5 + (kap[c + sz] + kc) * 0.5;
+        double dg = op == ELSIM_OP_CNE ? pse[c] + sc * sum : -sc * sum;
+        dinv[NI(i, j, k)] = 1.0 / dg;
+      }
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  double rz = 0.0;
+#pragma omp parallel for reduction(+ : rz) schedule(static)
+  for (size_t q = 0; q < N; q++) {
+    r[q] = rhs[q];
+    z[q] = dinv[q] * r[q];
+    rz += r[q] * z[q];
+  }
+  double beta = 0.0;
+  for (int it = 0; it < niter; it++) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= n3; k++)
+      for (int j = 1; j <= n2; j++)
+        for (int i = 1; i <= n1; i++) p[GI(i, j, k)] = z[NI(i, j, k)] + beta * p[GI(i, j, k)];
+    or_prj_bc(op, n1, n2, n3, p);
+    or_apply_op(op, n1, n2, n3, kap, pse, p, dt, dx, w);
+    double pw = 0.0;
+#pragma omp parallel for collapse(2) reduction(+ : pw) schedule(static)
+    for (int k = 1; k <= n3; k++)
+      for (int j = 1; j <= n2; j++)
+        for (int i = 1; i <= n1; i++) pw += p[GI(i, j, k)] * w[NI(i, j, k)];
+    const double alpha = rz / pw;
+    double rzn = 0.0;
+#pragma omp parallel for collapse(2) reduction(+ : rzn) schedule(static)
+    for (int k = 1; k <= n3; k++)
+      for (int j = 1; j <= n2; j++)
+        for (int i = 1; i <= n1; i++) {
+          size_t q = NI(i, j, k);
+          x[q] += alpha * p[GI(i, j, k)];
+          r[q] -= alpha * w[q];
+          z[q] = dinv[q] * r[q];
+          rzn += r[q] * z[q];
+        }
+    beta = rzn / rz;
+    rz = rzn;
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  free(x); free(r); free(z); free(w); free(dinv); free(p);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }

@@ -21,8 +21,8 @@ def test_exports_every_declared_symbol():
 
 
 def test_param_layout():
-    # 33 doubles + 8 int32, no padding
-    assert C.sizeof(E.SbmParams) == 33 * 8 + 8 * 4
+    # 33 doubles + 10 int32, no padding
+    assert C.sizeof(E.SbmParams) == 33 * 8 + 10 * 4
     assert C.sizeof(E.StepStats) == 6 * 4 + 8 * 8 + 2 * 4 + 4 * 8
 
 
