@@ -75,6 +75,11 @@ def build_fortran(lib=LIB):
     flib = os.path.join(LIBDIR, "libelsim_fortran.so")
     _run([FLANG, "-shared", "-fPIC", "-o", flib] + objs[:2] + ["-L" + LIBDIR, "-lelsim_hip",
                                                                "-Wl,-rpath,$ORIGIN"])
+    # RM_UCNCT_3D's copies of the level-set subroutines (REAL(4) time step)
+    orm = os.path.join(out, "elsim_dropin_rm.f90.o")
+    _run([FLANG, "-O2", "-fPIC", "-module-dir", out, "-c", os.path.join(FORTRAN, "elsim_dropin_rm.f90"), "-o", orm])
+    _run([FLANG, "-shared", "-fPIC", "-o", os.path.join(LIBDIR, "libelsim_rm.so"), objs[0], orm, "-L" + LIBDIR,
+          "-lelsim_hip", "-Wl,-rpath,$ORIGIN"])
     exe = os.path.join(LIBDIR, "sbm_run")
     _run([FLANG, "-o", exe, objs[2], "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip",
           "-Wl,-rpath,$ORIGIN"])
@@ -87,6 +92,10 @@ def build_fortran(lib=LIB):
         if os.path.exists(chk):
             _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, name), chk,
                   "-L" + LIBDIR, "-lelsim_fortran", "-lelsim_hip", "-Wl,-rpath,$ORIGIN"])
+    chk = os.path.join(ROOT, "tests", "fortran", "rm_check.f90")
+    if os.path.exists(chk):
+        _run([FLANG, "-O2", "-o", os.path.join(LIBDIR, "rm_check"), chk, "-L" + LIBDIR, "-lelsim_rm", "-lelsim_hip",
+              "-Wl,-rpath,$ORIGIN"])
     # the rank stitcher is file I/O only: no GPU library
     _run([FLANG, "-O2", "-module-dir", out, "-o", os.path.join(LIBDIR, "sbm_asmb"),
           os.path.join(FORTRAN, "sbm_asmb.f90")])

@@ -6,7 +6,7 @@
 #include <mutex>
 #include <vector>
 
-#include "elsim_common.h"
+#include "elsim_solver.h"
 
 namespace elsim {
 
@@ -29,6 +29,7 @@ void set_error(const char* fmt, ...) {
 }
 
 hipStream_t stream() { return g_stream; }
+bool initialized() { return g_init; }
 Workspace& workspace() { return g_ws; }
 
 int ensure_init() {

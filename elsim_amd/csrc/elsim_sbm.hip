@@ -964,8 +964,9 @@ int step(SbmState& s, int mode, elsim_step_stats* st) {
 
 namespace {
 CgWork g_pot_work;  // work space of the standalone potential solves (grows with the grid)
+}  // namespace
 
-int pot_args_ok(int n1, int n2, int n3, double dx, const void* kap, const void* rhs, const void* phi, int bc_lo,
+int pot_args_check(int n1, int n2, int n3, double dx, const void* kap, const void* rhs, const void* phi, int bc_lo,
                 int bc_hi, int mode, double rtol, int maxit) {
   if (!dims_ok(n1, n2, n3)) {
     set_error("bad grid %d x %d x %d", n1, n2, n3);
@@ -991,6 +992,7 @@ int pot_args_ok(int n1, int n2, int n3, double dx, const void* kap, const void* 
   return ELSIM_OK;
 }
 
+namespace {
 // Ghost planes of the unknown: mirrors on the d1 and d3 faces and on a
 // Neumann d2 face; a Dirichlet d2 plane keeps the value the caller stored.
 BcArgs pot_bc(int bc_lo, int bc_hi) {
@@ -1000,8 +1002,10 @@ BcArgs pot_bc(int bc_lo, int bc_hi) {
   return b;
 }
 
-int solve_pot_dev(int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi, int bc_lo,
-                  int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres) {
+}  // namespace
+
+int solve_pot_with(CgWork& w, int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi,
+                   int bc_lo, int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres) {
   const Dims d{n1, n2, n3};
   const int op = bc_lo && bc_hi ? ELSIM_OP_POT_DD : (bc_hi ? ELSIM_OP_PHP : ELSIM_OP_PHE);
   ELSIM_TRY(launch_bc(d, phi, pot_bc(bc_lo, bc_hi)));
@@ -1021,12 +1025,18 @@ int solve_pot_dev(int n1, int n2, int n3, double dx, const double* kap, const do
   a.tw = 1.0;
   a.delta = 0.5;
   SolveResult r;
-  ELSIM_TRY(cg_solve(g_pot_work, a, mode, &r));
+  ELSIM_TRY(cg_solve(w, a, mode, &r));
   ELSIM_TRY(launch_bc(d, phi, pot_bc(bc_lo, bc_hi)));  // SBM:716-719
   ELSIM_HIP(hipStreamSynchronize(stream()));
   if (iters) *iters = r.iters;
   if (relres) *relres = r.relres;
   return r.status;
+}
+
+namespace {
+int solve_pot_dev(int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi, int bc_lo,
+                  int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres) {
+  return solve_pot_with(g_pot_work, n1, n2, n3, dx, kap, rhs, phi, bc_lo, bc_hi, mode, rtol, maxit, iters, relres);
 }
 }  // namespace
 
@@ -1232,7 +1242,7 @@ int elsim_sbm_matvec(const int64_t* handle, const int* op, const double* p, doub
 
 int elsim_dev_solve_pot(int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi,
                         int bc_lo, int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres) {
-  ELSIM_TRY(pot_args_ok(n1, n2, n3, dx, kap, rhs, phi, bc_lo, bc_hi, mode, rtol, maxit));
+  ELSIM_TRY(pot_args_check(n1, n2, n3, dx, kap, rhs, phi, bc_lo, bc_hi, mode, rtol, maxit));
   ELSIM_TRY(ensure_init());
   return solve_pot_dev(n1, n2, n3, dx, kap, rhs, phi, bc_lo, bc_hi, mode, rtol, maxit, iters, relres);
 }
@@ -1244,7 +1254,7 @@ int elsim_solve_pot(const int* n1, const int* n2, const int* n3, const double* d
     set_error("null scalar argument");
     return ELSIM_ERR_ARG;
   }
-  ELSIM_TRY(pot_args_ok(*n1, *n2, *n3, *dx, kap, rhs, phi, *bc_lo, *bc_hi, *mode, *rtol, *maxit));
+  ELSIM_TRY(pot_args_check(*n1, *n2, *n3, *dx, kap, rhs, phi, *bc_lo, *bc_hi, *mode, *rtol, *maxit));
   ELSIM_TRY(ensure_init());
   const Dims d{*n1, *n2, *n3};
   const size_t G = (size_t)d.G() * 8, N = (size_t)d.N() * 8;

@@ -41,7 +41,14 @@ struct CgWork {
   DevBuf dinv64;                     // FP64: N (ELSIM_MODE_FP64_PCG)
   DevBuf cg;                         // CgDev
   CgDev* host = nullptr;             // pinned mirror
+  double* partials = nullptr;        // reduction partials (two values); null: the library workspace
+  bool bound = false;                // buffers live in memory the caller provided (never freed here)
   int alloc(const Dims& d);
+  // Lay the buffers of grid d out in [base, base + bytes) (no allocation),
+  // with `host` as the scalars' host mirror, and clear them with a fill
+  // kernel.  ELSIM_ERR_ARG if they do not fit.
+  int bind(const Dims& d, unsigned char* base, size_t bytes, CgDev* host);
+  static size_t bound_bytes(const Dims& d);
   void release();
 };
 
@@ -92,6 +99,17 @@ int launch_resid_store(int op, const Dims& d, const double* x, const double* kap
 // sum over the interior of dv * (rhs - A x), x's ghosts as stored (host result).
 int launch_dot_resid(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
                      const double* rhs, const double* dv, double* out);
+
+static_assert(sizeof(CgDev) <= 256, "CgDev must fit the 256 bytes a bound work space reserves for it");
+
+// The standalone potential solve (elsim_solve_pot): argument checks (no
+// device call) and the solve on work space w.
+int pot_args_check(int n1, int n2, int n3, double dx, const void* kap, const void* rhs, const void* phi, int bc_lo,
+                   int bc_hi, int mode, double rtol, int maxit);
+int solve_pot_with(CgWork& w, int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi,
+                   int bc_lo, int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres);
+// True once elsim_init has run.
+bool initialized();
 
 // Flat kernels shared with the SBM driver.
 int launch_copy_interior_g2n(const Dims& d, const double* g, double* n);  // n = g(interior)

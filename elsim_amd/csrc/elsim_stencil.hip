@@ -31,6 +31,7 @@
 // one-block finalising kernel adds them in index order and updates the CG
 // scalars (alpha, beta, stop tests) on the device, so the host only polls a
 // done flag every few iterations.
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -705,12 +706,14 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
         cg->done = 2;
       } else {
         // Restart the FP32 recurrence from the true residual: the next
-        // k_mx_dir3 forms p = dinv r (beta = 0).  With FP32 vectors on these
-        // ill-conditioned systems the recurred residual drifts from the true
-        // one between reliable updates; continuing the old direction with the
-        // replaced residual (beta = r.z ratio) measured 825 phP iterations
-        // per step at 256^3 against 271 with the restart (same start and
-        // tolerance).
+        // k_mx_dir3 forms p = dinv r (beta = 0).  Continuing the old
+        // direction with the replaced residual (beta = r.z ratio) measured
+        // 823 phP iterations per step at 256^3 with the extrapolated start,
+        // against 212 with restarts (restarting only when the recurred
+        // residual had drifted from the true one by over 1 %: 837): the
+        // restarted solves end further below the tolerance, and the warm
+        // start extrapolates from them.  With the plain start, restarts cost
+        // iterations (3779 against 2369 per step).
         cg->pending_y = 0;
         cg->beta = 0.0;
         cg->rz = tot[1];
@@ -751,7 +754,60 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
 
 // ------------------------------------------------------------------ host --
 
+// Zero fill of n bytes (n a multiple of 8): a kernel rather than a memset,
+// so that the allocation-free path records only kernels.
+__global__ void k_zero8(int64_t n8, uint64_t* __restrict__ p) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n8; q += (int64_t)gridDim.x * blockDim.x) p[q] = 0;
+}
+
+size_t CgWork::bound_bytes(const Dims& dd) {
+  const size_t G = (size_t)dd.G(), Ge = (G + 1) & ~(size_t)1;
+  const size_t v64 = (Ge * 8 + 255) & ~(size_t)255, v32 = (Ge * 4 + 255) & ~(size_t)255;
+  const size_t part = 2 * (size_t)kMaxRedBlocks * 8;  // two-value reductions (slots 0 and 1)
+  return 7 * v64 + 7 * v32 + part + 256;
+}
+
+int CgWork::bind(const Dims& dd, unsigned char* base, size_t bytes, CgDev* hostp) {
+  if (!base || !hostp || bound_bytes(dd) > bytes) {
+    set_error("work space of %zu bytes does not fit %zu bytes", bytes, bound_bytes(dd));
+    return ELSIM_ERR_ARG;
+  }
+  if (!bound) release();
+  bound = true;
+  d = dd;
+  const size_t G = (size_t)d.G(), Ge = (G + 1) & ~(size_t)1;
+  const size_t v64 = (Ge * 8 + 255) & ~(size_t)255, v32 = (Ge * 4 + 255) & ~(size_t)255;
+  unsigned char* q = base;
+  DevBuf* g64[] = {&res, &wmx, &prj, &x0, &r0, &e64, &dinv64};
+  DevBuf* g32[] = {&y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32};
+  for (DevBuf* b : g64) {
+    b->p = q;
+    b->bytes = v64;
+    q += v64;
+  }
+  for (DevBuf* b : g32) {
+    b->p = q;
+    b->bytes = v32;
+    q += v32;
+  }
+  partials = (double*)q;
+  q += 2 * (size_t)kMaxRedBlocks * 8;
+  cg.p = q;
+  cg.bytes = 256;
+  host = hostp;
+  // every work vector starts with zero ghosts (the last call left its own
+  // layout here); the partials and scalars are written before they are read
+  const int64_t n8 = (int64_t)((7 * v64 + 7 * v32) / 8);
+  k_zero8<<<(int)std::min<int64_t>((n8 + 255) / 256, 65536), 256, 0, stream()>>>(n8, (uint64_t*)base);
+  ELSIM_CHECK_LAUNCH();
+  return ELSIM_OK;
+}
+
 int CgWork::alloc(const Dims& dd) {
+  if (bound) {
+    set_error("work space bound to caller memory");
+    return ELSIM_ERR_ARG;
+  }
   if (d.n1 == dd.n1 && d.n2 == dd.n2 && d.n3 == dd.n3 && cg.p) return ELSIM_OK;
   d = dd;
   const size_t G = (size_t)d.G(), Ge = (G + 1) & ~(size_t)1;  // even length: pair kernels
@@ -769,6 +825,10 @@ int CgWork::alloc(const Dims& dd) {
 }
 
 void CgWork::release() {
+  if (bound) {  // caller memory: forget the layout only
+    *this = CgWork{};
+    return;
+  }
   DevBuf* bs[] = {&res, &wmx, &prj, &x0, &r0, &e64, &y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32, &dinv64,
                   &cg};
   for (DevBuf* b : bs) dev_free(*b);
@@ -815,7 +875,7 @@ int poll(CgWork& w, hipStream_t s) {
 template <int OP>
 int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   const Dims& d = a.d;
-  double* P = workspace().partials;
+  double* P = w.partials ? w.partials : workspace().partials;
   hipStream_t s = stream();
   const dim3 gc = col_grid(d), blk(TX, TY, 1);
   const int nb64 = stencil_ntiles<double>(d), nb32 = stencil_ntiles<float>(d);
@@ -1043,7 +1103,11 @@ int launch_dot_resid(int op, const Dims& d, const double* x, const double* kap, 
 }
 
 int cg_solve(CgWork& w, const SolveArgs& a, int mode, SolveResult* res) {
-  ELSIM_TRY(w.alloc(a.d));
+  if (!w.bound) ELSIM_TRY(w.alloc(a.d));
+  if (w.bound && (w.d.n1 != a.d.n1 || w.d.n2 != a.d.n2 || w.d.n3 != a.d.n3)) {
+    set_error("work space laid out for another grid");
+    return ELSIM_ERR_ARG;
+  }
   if (mode != ELSIM_MODE_FP64 && a.stop_rule != ELSIM_STOP_RELRES) {
     set_error("the preconditioned solvers use the relative-residual stop rule only");
     return ELSIM_ERR_ARG;

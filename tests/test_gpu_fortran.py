@@ -234,3 +234,28 @@ def test_dropins_from_fortran(tmp_path, phase):
     lab, con = O.label_connected(rdin, phase)
     assert np.array_equal(rd("lab_out.dat", n, np.int32), lab)
     assert np.array_equal(rd("con_out.dat", n, np.int32), con)
+
+
+def test_rm_dropin_library(tmp_path):
+    """RM_UCNCT_3D's call of ReDist3D_XYZ (RM_UCNCT_3D.f90:100, 148) linked
+    unchanged against libelsim_rm.so (tests/fortran/rm_check.f90): the REAL(4)
+    pseudo-time step of RM_UCNCT_3D.f90:236, bit-exact against the oracle's
+    restatement and, where it was built, the reference's own subroutine."""
+    exe = os.path.join(os.path.dirname(E.__file__), "lib", "rm_check")
+    assert os.path.exists(exe), "RM drop-in check program not built"
+    py, px, pz, lit = 11, 14, 9, 15
+    rng = np.random.default_rng(77)
+    g = (py + 2, px + 2, pz + 2)
+    sg = O.nogrdbc3d(np.where(rng.uniform(size=g) > 0.5, 1.0, -1.0))
+    f = np.zeros(g)
+    for name, a in (("sg", sg), ("f", f)):
+        fileio.write_record(tmp_path / f"{name}.dat", a, np.float64)
+    r = subprocess.run([exe, str(tmp_path), str(lit), str(py), str(px), str(pz)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = fileio.read_record(tmp_path / "f_out.dat", g, np.float64)
+    want = O.redist3d(lit, sg, f, float(np.float32(0.1)))
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got, O.redist3d(lit, sg, f, 0.1))  # the LS step gives other values
+    if O.ref_available():
+        assert np.array_equal(got, O.ref_redist3d(lit, sg, f, "rm"))

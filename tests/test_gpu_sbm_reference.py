@@ -9,9 +9,10 @@ for the reference CG and the mixed-precision solver.  At this dx the
 Dirichlet data dominate ||b_eff|| of the potential solves (kappa/dx^2 BvP is
 ~1e9 against a reaction term of ~1e-3), so relres 1e-8 resolves phi_P's
 interior variation (~3e-4 of its value) only to ~1e-3 of itself (6.5e-6
-relative L2 between the two runs): the comparison solves to relres 1e-10,
-with iteration caps above the reference's (the unpreconditioned reference CG
-needs ~2e4 iterations per solve there).  The mobility table is
+relative L2 between the two runs; the preconditioned mixed solver, on a
+different iterate path, was 9.6e-6 away at relres 1e-10): the comparison
+solves to relres 1e-12, with iteration caps above the reference's (the
+unpreconditioned reference CG needs ~2e4 iterations per solve there).  The mobility table is
 scaled to the magnitude of Li diffusivity in graphite (the reference's table
 files are absent); the explicit CH update is stable with it."""
 import numpy as np
@@ -28,7 +29,7 @@ def setup(oracle_lib):
     sdf, _, _ = E.sphere_pack(N, 2, 4.0, 8.0, target=0.5, hardcore=0.85, connect=True)
     prm = E.reference_params()
     prm.stop_rule = E.STOP_RELRES
-    prm.rtol = 1.0e-10
+    prm.rtol = 1.0e-12
     prm.max_it_e = prm.max_it_p = prm.max_it_l = 400000
     prm.max_outer = 50
     tab = E.tables().copy()
@@ -53,9 +54,11 @@ def test_reference_constants_step_matches_oracle(gpu, oracle_lib, mode):
               so.n_outer)
         assert sg.status == 0 and so.status == 0
         assert abs(sg.n_outer - so.n_outer) <= 1
-        assert sg.relres_php <= 1e-10 and sg.relres_phe <= 1e-10
+        assert sg.relres_php <= 2e-12 and sg.relres_phe <= 2e-12
         if step > 0:
-            assert sg.it_cne > 0 and sg.relres_cne <= 1e-10  # CnE does iterate with these constants
+            # CnE does iterate with these constants; the reference CG stops on its
+            # recurred residual, and at 1e-12 the true one is a rounding factor above
+            assert sg.it_cne > 0 and sg.relres_cne <= 2e-12
         assert sg.xrf == pytest.approx(so.xrf, rel=1e-9)
         assert sg.scrnt == pytest.approx(so.scrnt, rel=1e-6)
     for f in (F_CNP, F_CNE, F_PHP, F_PHE):
