@@ -149,6 +149,7 @@ _SIGS = {
     "elsim_gen_tables": (I, [I, P]),
     "elsim_solve_pot": (I, [IP, IP, IP, DP, P, P, P, IP, IP, IP, DP, IP, IP, DP]),
     "elsim_dev_solve_pot": (I, [I, I, I, D, P, P, P, I, I, I, D, I, IP, DP]),
+    "elsim_dev_solve_pot_noalloc": (I, [I, I, I, D, P, P, P, I, I, I, D, I, IP, DP]),
 }
 
 

@@ -26,6 +26,7 @@ SOURCES = [
     "elsim_levelset.hip",
     "elsim_connect.hip",
     "elsim_nn.hip",
+    "elsim_noalloc.hip",
     "elsim_gen.cpp",
 ]
 

@@ -91,6 +91,7 @@ int elsim_init(int device) {
   ELSIM_HIP(hipMalloc(&g_ws.partials, sizeof(double) * kMaxRedBlocks * kMaxRedVals));
   ELSIM_HIP(hipMalloc(&g_ws.counter, sizeof(unsigned) * 4));
   ELSIM_HIP(hipMemset(g_ws.counter, 0, sizeof(unsigned) * 4));
+  ELSIM_TRY(noalloc_init(device));
   ELSIM_HIP(hipDeviceSynchronize());
   g_init = true;
   return ELSIM_OK;

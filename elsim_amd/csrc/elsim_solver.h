@@ -108,8 +108,12 @@ int pot_args_check(int n1, int n2, int n3, double dx, const void* kap, const voi
                    int bc_hi, int mode, double rtol, int maxit);
 int solve_pot_with(CgWork& w, int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi,
                    int bc_lo, int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres);
-// True once elsim_init has run.
+// True once elsim_init has run (the allocation-free entry points do not
+// initialise the library themselves, since that allocates).
 bool initialized();
+// Set-up of the allocation-free entry points on the current device (called
+// by elsim_init).
+int noalloc_init(int device);
 
 // Flat kernels shared with the SBM driver.
 int launch_copy_interior_g2n(const Dims& d, const double* g, double* n);  // n = g(interior)

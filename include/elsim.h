@@ -323,6 +323,19 @@ int elsim_solve_pot(const int* n1, const int* n2, const int* n3, const double* d
                     const double* rtol, const int* maxit, int* iters, double* relres);
 int elsim_dev_solve_pot(int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi,
                         int bc_lo, int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): elsim_dev_solve_pot with its CG work space in a fixed
+ * device array of the library instead of allocated buffers.  Same arguments
+ * and results (bit-identical).  Grids up to ELSIM_NOALLOC_MAX_G ghosted
+ * elements (n1+2)(n2+2)(n3+2), i.e. 64^3 interior; larger grids are
+ * ELSIM_ERR_ARG.  elsim_init must have run (it sets the array up; this entry
+ * point allocates nothing and does not initialise the library).  The array is
+ * one per GPU and shared by all calls on it: calls that overlap on one GPU
+ * race. */
+#define ELSIM_NOALLOC_MAX_G 287496
+int elsim_dev_solve_pot_noalloc(int n1, int n2, int n3, double dx, const double* kap, const double* rhs, double* phi,
+                                int bc_lo, int bc_hi, int mode, double rtol, int maxit, int* iters, double* relres);
 /* The Cahn-Hilliard stage alone (SBM:397-443). */
 int elsim_sbm_ch_update(const int64_t* handle);
 /* Homogeneous operator of the state's current coefficients applied to a
