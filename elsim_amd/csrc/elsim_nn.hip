@@ -33,6 +33,15 @@ namespace elsim {
 
 typedef short s4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+// v_mfma_f32_16x16x32_bf16 (gfx950): lane l holds A[l % 16][8 (l / 16) .. + 7]
+// and B[8 (l / 16) .. + 7][l % 16]; D[4 (l / 16) + i][l % 16] in acc[i] (the
+// layout of the 16x16x16 form with eight k per lane; checked on the GPU
+// against a CPU product before use).
+__device__ __forceinline__ f4v mfma32(uint4 a, uint4 b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
 
 // ----------------------------------------------------------- convolution --
 
@@ -44,10 +53,20 @@ struct View {
   int cs, co;  // channel stride and offset of a channel slice
 };
 
+// Channels [0, ch) of a convolution's input taken from a tensor at half
+// resolution, nearest-upsampled while they are staged (the decoder's
+// upsampled skip half, read where it lies instead of from a full-resolution
+// copy); ch = 0: every channel from the full-resolution input.
+struct Coarse {
+  const uint16_t* p;
+  int ch;
+  View v;
+};
+
 // MODE 0: out bf16 = relu?(acc + bias); MODE 1: out FP32 = acc.
 template <int COUT, int MT, int MODE>
 __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
-                                               View iv, const uint16_t* __restrict__ wt,
+                                               View iv, Coarse lo, const uint16_t* __restrict__ wt,
                                                const float* __restrict__ bias, int relu, void* __restrict__ out,
                                                View ov) {
   constexpr int NT = COUT / 16;
@@ -76,12 +95,39 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
       const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
       uint4 v = make_uint4(0, 0, 0, 0);
       if (gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3) {
-        const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
-        v = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
+        if (slab < lo.ch) {
+          const int64_t cv = (gx >> 1) + (int64_t)(n1 >> 1) * ((gy >> 1) + (int64_t)(n2 >> 1) * (gz >> 1));
+          v = *(const uint4*)(lo.p + cv * lo.v.cs + lo.v.co + slab + 8 * q);
+        } else {
+          const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
+          v = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
+        }
       }
       *(uint4*)(lds + h * rs + 8 * q) = v;
     }
     __syncthreads();
+    if (cs == 32) {
+      // one 16x16x32 MFMA per output tile and tap: lane group lq holds
+      // channels 8 lq .. 8 lq + 7 of the slab
+      for (int tap = 0; tap < 27; tap++) {
+        const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
+        uint4 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+          const int row = wave * MT + m, y = row % CONV_TY, z = row / CONV_TY;
+          const int h = ((z + dz) * HY + (y + dy)) * HX + (lr + dx);
+          a[m] = *(const uint4*)(lds + h * rs + 8 * lq);
+        }
+        const int k = tap * cin + slab + 8 * lq;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const uint4 bw = *(const uint4*)(wt + (int64_t)(t * 16 + lr) * K + k);
+#pragma unroll
+          for (int m = 0; m < MT; m++) acc[m][t] = mfma32(a[m], bw, acc[m][t]);
+        }
+      }
+      continue;
+    }
     for (int tap = 0; tap < 27; tap++) {
       const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;  // halo offsets 0..2 (= -1..1 + 1)
       for (int cc = 0; cc < cs; cc += 16) {
@@ -129,13 +175,12 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
 
 // The same convolution for 16-channel slabs (input channels not a multiple
 // of 32) and at most 32 output channels: each wave loads the weight
-// fragments of all 27 taps of a slab into registers at once (27 x NT loads in
+// fragments of all 27 taps of a slab into registers at once (14 x NT loads in
 // flight, issued before the slab is staged), instead of one load per tap
-// whose latency the tap's few MFMAs cannot hide.  Same MFMAs in the same order
-// as k_conv3, so the same results.
+// whose latency the tap's few MFMAs cannot hide; two taps per MFMA.
 template <int COUT, int MT, int MODE>
 __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
-                                                   View iv, const uint16_t* __restrict__ wt,
+                                                   View iv, Coarse lo, const uint16_t* __restrict__ wt,
                                                    const float* __restrict__ bias, int relu, void* __restrict__ out,
                                                    View ov) {
   constexpr int NT = COUT / 16, CS = 16;
@@ -155,12 +200,20 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[m][t] = f4v{0.f, 0.f, 0.f, 0.f};
 
+  // taps in pairs, one 16x16x32 MFMA per pair: lane group lq holds tap
+  // 2 pair + (lq >> 1), channels 8 (lq & 1) .. + 7; the 28th tap has zero
+  // weights
+  const int tsub = lq >> 1, csub = 8 * (lq & 1);
   for (int slab = 0; slab < cin; slab += CS) {
-    s4v bw[27][NT];
+    uint4 bw[14][NT];
 #pragma unroll
-    for (int tap = 0; tap < 27; tap++)
+    for (int pr = 0; pr < 14; pr++) {
+      const int tap = 2 * pr + tsub;
 #pragma unroll
-      for (int t = 0; t < NT; t++) bw[tap][t] = *(const s4v*)(wt + (int64_t)(t * 16 + lr) * K + tap * cin + slab + 4 * lq);
+      for (int t = 0; t < NT; t++)
+        bw[pr][t] = tap < 27 ? *(const uint4*)(wt + (int64_t)(t * 16 + lr) * K + tap * cin + slab + csub)
+                             : make_uint4(0, 0, 0, 0);
+    }
     __syncthreads();
     for (int it = tid; it < HN * 2; it += 256) {
       const int h = it >> 1, q = it & 1;
@@ -168,27 +221,32 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
       const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
       uint4 v = make_uint4(0, 0, 0, 0);
       if (gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3) {
-        const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
-        v = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
+        if (slab < lo.ch) {
+          const int64_t cv = (gx >> 1) + (int64_t)(n1 >> 1) * ((gy >> 1) + (int64_t)(n2 >> 1) * (gz >> 1));
+          v = *(const uint4*)(lo.p + cv * lo.v.cs + lo.v.co + slab + 8 * q);
+        } else {
+          const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
+          v = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
+        }
       }
       *(uint4*)(lds + h * RS + 8 * q) = v;
     }
     __syncthreads();
 #pragma unroll
-    for (int tap = 0; tap < 27; tap++) {
+    for (int pr = 0; pr < 14; pr++) {
+      const int tap = min(2 * pr + tsub, 26);  // the 28th tap reads tap 26's values against zero weights
       const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
-      s4v a[MT];
+      uint4 a[MT];
 #pragma unroll
       for (int m = 0; m < MT; m++) {
         const int row = wave * MT + m, y = row % CONV_TY, z = row / CONV_TY;
         const int h = ((z + dz) * HY + (y + dy)) * HX + (lr + dx);
-        a[m] = *(const s4v*)(lds + h * RS + 4 * lq);
+        a[m] = *(const uint4*)(lds + h * RS + csub);
       }
 #pragma unroll
       for (int t = 0; t < NT; t++)
 #pragma unroll
-        for (int m = 0; m < MT; m++)
-          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bw[tap][t], acc[m][t], 0, 0, 0);
+        for (int m = 0; m < MT; m++) acc[m][t] = mfma32(a[m], bw[pr][t], acc[m][t]);
     }
   }
 #pragma unroll
@@ -216,32 +274,37 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
 }
 
 template <int MODE, int COUT, int MT>
-static void conv_launch(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, const uint16_t* w,
+static void conv_launch(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, Coarse lo, const uint16_t* w,
                         const float* bias, int relu, void* out, View ov, hipStream_t s) {
   const int nb = ((n1 + CONV_TX - 1) / CONV_TX) * ((n2 + CONV_TY - 1) / CONV_TY) * ((n3 + MT - 1) / MT);
   if constexpr (COUT <= 32) {
     if (cin % 32 != 0) {
-      k_conv3_w27<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov);
+      k_conv3_w27<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov);
       return;
     }
   }
-  k_conv3<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov);
+  k_conv3<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov);
 }
 
 template <int MODE>
 static int conv(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, const uint16_t* w, const float* bias,
-                int relu, int cout, void* out, View ov, hipStream_t s) {
+                int relu, int cout, void* out, View ov, hipStream_t s, Coarse lo = Coarse{nullptr, 0, View{0, 0}}) {
   if (cin % 16 || cin <= 0 || iv.cs % 8 || iv.co % 8) {
     set_error("convolution: input channels %d (stride %d, offset %d) must be multiples of 16 (8)", cin, iv.cs, iv.co);
     return ELSIM_ERR_ARG;
   }
+  const int cs = cin % 32 == 0 ? 32 : 16;
+  if (lo.ch && (lo.ch % cs || lo.ch > cin || lo.v.cs % 8 || lo.v.co % 8 || n1 % 2 || n2 % 2 || n3 % 2)) {
+    set_error("convolution: upsampled channels %d do not fit slabs of %d", lo.ch, cs);
+    return ELSIM_ERR_ARG;
+  }
   switch (cout) {
-    case 16: conv_launch<MODE, 16, 4>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
-    case 32: conv_launch<MODE, 32, 4>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
-    case 48: conv_launch<MODE, 48, 4>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
-    case 64: conv_launch<MODE, 64, 4>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
-    case 96: conv_launch<MODE, 96, 2>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
-    case 128: conv_launch<MODE, 128, 2>(n1, n2, n3, in, cin, iv, w, bias, relu, out, ov, s); break;
+    case 16: conv_launch<MODE, 16, 4>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
+    case 32: conv_launch<MODE, 32, 4>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
+    case 48: conv_launch<MODE, 48, 4>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
+    case 64: conv_launch<MODE, 64, 4>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
+    case 96: conv_launch<MODE, 96, 2>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
+    case 128: conv_launch<MODE, 128, 2>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
     default: set_error("convolution: %d output channels not supported", cout); return ELSIM_ERR_ARG;
   }
   ELSIM_CHECK_LAUNCH();
@@ -655,7 +718,10 @@ struct NnNet {
   }
 
   // forward into pred (FP32); with a target also dpred and the loss partials
-  int forward(const uint16_t* x0, float* pred, const float* target, double* loss_host) {
+  // infer: the activations are not kept for a backward pass, so the
+  // decoder reads its upsampled halves at half resolution while staging
+  // (no full-resolution copies; the same values, so the same results).
+  int forward(const uint16_t* x0, float* pred, const float* target, double* loss_host, bool infer = false) {
     ELSIM_TRY(pack());
     hipStream_t s = stream();
     const int a1 = n1, a2 = n2, a3 = n3;
@@ -676,12 +742,16 @@ struct NnNet {
     ELSIM_TRY(conv<0>(r1, r2, r3, (uint16_t*)p3.p, 64, View{64, 0}, WT(L_E3), B + b_off[L_E3], 1, 64, e3.p,
                       View{64, 0}, s));
     // decoder
-    k_up<<<nblk_cap(N2 * 8), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
-    ELSIM_TRY(conv<0>(q1, q2, q3, C2, 128, View{128, 0}, WT(L_D2), B + b_off[L_D2], 1, 64, d2.p, View{64, 0}, s));
-    k_up<<<nblk_cap(N1 * 8), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
-    ELSIM_TRY(conv<0>(m1, m2, m3, C1, 96, View{96, 0}, WT(L_D1), B + b_off[L_D1], 1, 32, d1.p, View{32, 0}, s));
-    k_up<<<nblk_cap(N0 * 4), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
-    ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, View{48, 0}, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s));
+    const Coarse none{nullptr, 0, View{0, 0}};
+    if (!infer) k_up<<<nblk_cap(N2 * 8), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
+    ELSIM_TRY(conv<0>(q1, q2, q3, C2, 128, View{128, 0}, WT(L_D2), B + b_off[L_D2], 1, 64, d2.p, View{64, 0}, s,
+                      infer ? Coarse{(uint16_t*)e3.p, 64, View{64, 0}} : none));
+    if (!infer) k_up<<<nblk_cap(N1 * 8), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
+    ELSIM_TRY(conv<0>(m1, m2, m3, C1, 96, View{96, 0}, WT(L_D1), B + b_off[L_D1], 1, 32, d1.p, View{32, 0}, s,
+                      infer ? Coarse{(uint16_t*)d2.p, 64, View{64, 0}} : none));
+    if (!infer) k_up<<<nblk_cap(N0 * 4), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
+    ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, View{48, 0}, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s,
+                      infer ? Coarse{(uint16_t*)d1.p, 32, View{32, 0}} : none));
     const int ob = nblk(N0);
     if (target && ob > kMaxRedBlocks) {
       set_error("grid too large for the loss reduction");
@@ -811,7 +881,7 @@ struct NnNet {
 
 int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred) {
   ELSIM_TRY(net->ensure(n1, n2, n3, false));
-  return net->forward(x, pred, nullptr, nullptr);
+  return net->forward(x, pred, nullptr, nullptr, true);
 }
 
 }  // namespace elsim

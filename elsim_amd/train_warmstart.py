@@ -1,7 +1,7 @@
 """Train the learned CG warm start on solver-generated pairs, on the GPU.
 
-    python -m elsim_amd.train_warmstart --n 64 --train-seeds 40 --test-seeds 10 \\
-        --steps 50 --epochs 3 --out weights.npz
+    python -m elsim_amd.train_warmstart --n 128 --train-seeds 40 --test-seeds 10 \\
+        --steps 50 --epochs 3 --eval-n 256 --out weights.npz
 
 Pairs: for each seeded sphere pack and each step (from the third) of a run
 with the fast solver, the input is the network features of the extrapolated
@@ -36,7 +36,8 @@ def make_state(n, seed, rmin, rmax):
     rng = np.random.default_rng(10_000 + seed)
     target = float(rng.uniform(0.40, 0.60))
     sdf, ns, frac = E.sphere_pack(n, seed, rmin, rmax, target=target, hardcore=0.85, connect=True)
-    s = E.Sbm(sdf, -sdf, E.synthetic_params(), E.tables())
+    # the benchmark's workload: the CnE solve resolves the step's change
+    s = E.Sbm(sdf, -sdf, E.synthetic_params(cne_rule=1), E.tables())
     s.set(F_CNP, E.initial_cnp(n, seed))
     return s, ns, frac
 
@@ -104,6 +105,8 @@ def train(net, samples, n, epochs, lr, loss_kind="energy", seed=0):
             net.adam(max(cur, 1e-6))
             run.append(loss)
             k += 1
+            if k % 400 == 0:
+                log(f"    update {k}/{total}: loss {np.mean(run[-400:]):.4f} ({time.perf_counter() - t0:.0f} s)")
         hist.append(float(np.mean(run)))
         log(f"  epoch {ep}: mean loss {hist[-1]:.4f} ({time.perf_counter() - t0:.0f} s)")
     return hist
@@ -147,15 +150,17 @@ def parse_seeds(spec, default_start):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--eval-n", type=int, default=0, help="also evaluate at this size (held-out seeds)")
+    ap.add_argument("--eval-n-seeds", type=int, default=3)
     ap.add_argument("--train-seeds", default="40")
     ap.add_argument("--test-seeds", default="10")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--eval-steps", type=int, default=12)
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--rmin", type=float, default=4.0)
-    ap.add_argument("--rmax", type=float, default=8.0)
+    ap.add_argument("--rmin", type=float, default=6.0)
+    ap.add_argument("--rmax", type=float, default=16.0)
     ap.add_argument("--init-seed", type=int, default=0)
     ap.add_argument("--loss", choices=["energy", "mse"], default="mse")
     ap.add_argument("--out", default=None)
@@ -179,10 +184,16 @@ def main(argv=None):
         y.free()
     log(f"evaluating on {len(test_seeds)} held-out microstructures x {a.eval_steps} steps")
     ev = evaluate(net, a.n, test_seeds, a.eval_steps, a.rmin, a.rmax, mode)
+    ev_big = None
+    if a.eval_n:
+        log(f"evaluating at {a.eval_n}^3 on {a.eval_n_seeds} held-out microstructures x {a.eval_steps} steps")
+        ev_big = evaluate(net, a.eval_n, test_seeds[:a.eval_n_seeds], a.eval_steps, a.rmin, a.rmax, mode,
+                          warmup=min(6, a.eval_steps - 2))
     report = {"n": a.n, "train_seeds": [train_seeds[0], train_seeds[-1]], "test_seeds": [test_seeds[0], test_seeds[-1]],
               "steps": a.steps, "samples": len(samples), "epochs": a.epochs, "loss": a.loss, "lr": a.lr, "loss_per_epoch": hist,
               "generate_s": t_gen, "train_s": t_train, "updates_per_s": a.epochs * len(samples) / max(t_train, 1e-9),
-              "radius": [a.rmin, a.rmax], "eval": ev}
+              "radius": [a.rmin, a.rmax], "eval": ev,
+              "eval_large": {"n": a.eval_n, "seeds": test_seeds[:a.eval_n_seeds], "eval": ev_big} if ev_big else None}
     log(json.dumps(report))
     if a.out:
         net.meta = {"n": a.n, "features": list(NN.FEATURES),
