@@ -100,7 +100,7 @@ def matvec_roofline(n, reps=20):
 # update into the direction kernel: update r, w, dinv in, r out; direction y,
 # r, p, dinv in, y, p out.  Its dinv is BF16 (2 B), the FP64 solver's FP64.
 KERNEL_BYTES = {"op32": 12, "upd32": 14, "dir32": 22, "op64": 24, "upd64": 56, "dir64": 32}
-KERNEL_NAMES = {"op32": "k_matvec<float> (7-point operator + p.Ap)", "upd32": "k_mx_update (r -= a w, r.z)",
+KERNEL_NAMES = {"op32": "k_matvec4 (7-point operator, 4 voxels a lane, + p.Ap)", "upd32": "k_mx_update (r -= a w, r.z)",
                 "dir32": "k_mx_dir (y += a p, p = z + b p)", "op64": "k_matvec<double>", "upd64": "k_pcg64_update",
                 "dir64": "k_pcg64_dir"}
 

@@ -243,6 +243,230 @@ __global__ __launch_bounds__(256) void k_matvec(Dims d, const T* __restrict__ p,
   if (dot || cg) block_partials<1>(acc, partials);
 }
 
+// ------------------------------------------ FP32 operator, 4 voxels a lane --
+//
+// The FP32 operator of the CG loop with four consecutive d1 voxels per lane:
+// one 16-byte load per row and array (rows start at odd ghosted indices, so
+// these loads are 4-byte aligned, which the hardware serves as one access).
+// A d1 row segment of TW = 64, 128 or 256 voxels takes TW/4 lanes, so a wave
+// covers 256/TW segments, each with its own SJ4 rows; a lane keeps its rows
+// of p and kappa for the planes k and k+1 in registers and marches SKC4
+// planes.  Every face flux f (p_b - p_a) is formed once and used by both
+// voxels it separates: the face coefficient and the difference are the same
+// IEEE operations as in sweep() (the addition commutes) and the six fluxes
+// are combined in the same order, so the results are bit-identical to it.
+// At 256^3 (phP, data from HBM) this takes 44.7 us against 56 us for sweep(),
+// which issues 32 memory instructions per wave and plane (16 of them
+// single-lane d1 edge loads) where this kernel issues 16.
+constexpr int SJ4 = 4, SKC4 = 8;
+__host__ __device__ inline int mv4_tw(const Dims& d) { return d.n1 <= 64 ? 64 : (d.n1 <= 128 ? 128 : 256); }
+// usable when n1 is a multiple of 4 and ghosted indices fit 32 bits
+inline bool mv4_ok(const Dims& d) { return d.n1 % 4 == 0 && d.G() < ((int64_t)1 << 32); }
+__host__ __device__ inline TileGrid mv4_tiles(const Dims& d) {
+  const int tw = mv4_tw(d), rows = 4 * (256 / tw) * SJ4;  // d2 rows per block
+  return TileGrid{(d.n1 + tw - 1) / tw, (d.n2 + rows - 1) / rows, (d.n3 + SKC4 - 1) / SKC4};
+}
+inline int mv4_ntiles(const Dims& d) {
+  const TileGrid t = mv4_tiles(d);
+  return t.gx * t.gy * t.gz;
+}
+__device__ __forceinline__ float4 ld4(const float* __restrict__ a, uint32_t q) {
+  float4 v;
+  __builtin_memcpy(&v, a + q, 16);
+  return v;
+}
+__device__ __forceinline__ void st4(float* __restrict__ a, uint32_t q, float4 v) { __builtin_memcpy(a + q, &v, 16); }
+__device__ __forceinline__ void unpack4(const float4& v, float (&o)[4]) {
+  o[0] = v.x;
+  o[1] = v.y;
+  o[2] = v.z;
+  o[3] = v.w;
+}
+
+// out = A p with the homogeneous ghost rule; out ghosted (OUTG) or interior;
+// partials of p.out.  Launch: mv4_ntiles(d) blocks of 256 threads.  EDGE:
+// the box has more than one d1 tile, so the first and last lane of a row
+// segment may load their d1 neighbour from the next tile (n1 <= 256 leaves
+// that code, and its registers, out).
+template <int OP, bool OUTG, bool EDGE>
+__global__ __launch_bounds__(256) void k_matvec4(Dims d, const float* __restrict__ p, const float* __restrict__ kap,
+                                                 const float* __restrict__ pse, float scale, float* __restrict__ out,
+                                                 CgDev* cg, double* partials) {
+  if (cg && (cg->done || cg->need_reliable)) return;
+  constexpr int SJ = SJ4;
+  double acc = 0.0;
+  const TileGrid tg = mv4_tiles(d);
+  const int nt = tg.gx * tg.gy * tg.gz;
+  const int tw = mv4_tw(d), lpr = tw / 4, rpw = 64 / lpr;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int grp = lane / lpr, x4 = lane % lpr;
+  const float half = 0.5f;
+  const uint32_t sz = (uint32_t)d.g12();
+  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    const int per = nt / 8;
+    int b = tile;
+    if (b < per * 8) b = (b % 8) * per + b / 8;
+    const int tj = b % tg.gy, ti = (b / tg.gy) % tg.gx, tk = b / (tg.gx * tg.gy);
+    const int i0 = ti * tw + 4 * x4 + 1;
+    const bool active = i0 <= d.n1;
+    const int ia = active ? i0 : d.n1 - 3;  // idle lanes read an interior column
+    const bool left_b = i0 == 1, right_b = i0 + 3 == d.n1;
+    const bool ld_l = EDGE && active && x4 == 0 && i0 > 1, ld_r = EDGE && active && x4 == lpr - 1 && i0 + 3 < d.n1;
+    const int j0 = ((tj * 4 + wv) * rpw + grp) * SJ + 1;
+    const int k0 = tk * SKC4 + 1, k1 = min(k0 + SKC4 - 1, d.n3);
+    const bool down = tk & 1;  // odd tiles march downward (see sweep_ld)
+    const int kstart = down ? k1 : k0;
+    const uint32_t step = down ? 0u - sz : sz;
+    uint32_t rc[SJ];
+#pragma unroll
+    for (int t = 0; t < SJ; t++) rc[t] = (uint32_t)d.gi(ia, min(j0 + t, d.n2 + 1), kstart);
+    const uint32_t clo = (uint32_t)d.gi(ia, min(j0 - 1, d.n2 + 1), kstart);
+    const uint32_t chi = (uint32_t)d.gi(ia, min(j0 + SJ, d.n2 + 1), kstart);
+    float4 pc[SJ], kc[SJ];
+    float fb[SJ][4];  // flux through the face between plane k and the plane behind it
+    {
+      const bool bnd = down ? kstart == d.n3 : kstart == 1;
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        pc[t] = ld4(p, rc[t]);
+        kc[t] = ld4(kap, rc[t]);
+        float pcv[4], kcv[4], pbv[4], kbv[4];
+        unpack4(pc[t], pcv);
+        unpack4(kc[t], kcv);
+        unpack4(ld4(p, rc[t] - step), pbv);
+        unpack4(ld4(kap, rc[t] - step), kbv);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          // upward: f5 (x - pm); downward: f6 (pz - x); pm / pz = x on the boundary plane
+          const float f = (kbv[q] + kcv[q]) * half;
+          const float nb = bnd ? pcv[q] : pbv[q];
+          fb[t][q] = down ? f * (nb - pcv[q]) : f * (pcv[q] - nb);
+        }
+      }
+    }
+    uint32_t off = 0;
+    const int nk = k1 - k0 + 1;
+    for (int kk = 0; kk < nk; kk++, off += step) {
+      const int k = down ? k1 - kk : k0 + kk;
+      float4 pa[SJ], ka[SJ];
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        pa[t] = ld4(p, rc[t] + off + step);
+        ka[t] = ld4(kap, rc[t] + off + step);
+      }
+      const float4 hlp = ld4(p, clo + off), hlk = ld4(kap, clo + off);
+      const float4 hhp = ld4(p, chi + off), hhk = ld4(kap, chi + off);
+      float psv[SJ][4];
+      if (Rule<OP>::diag) {
+#pragma unroll
+        for (int t = 0; t < SJ; t++) unpack4(ld4(pse, rc[t] + off), psv[t]);
+      }
+      float elp[SJ], elk[SJ], erp[SJ], erk[SJ];
+#pragma unroll
+      for (int t = 0; t < SJ; t++) elp[t] = elk[t] = erp[t] = erk[t] = 0.0f;
+      if (EDGE && ld_l) {
+#pragma unroll
+        for (int t = 0; t < SJ; t++) {
+          elp[t] = p[rc[t] + off - 1];
+          elk[t] = kap[rc[t] + off - 1];
+        }
+      }
+      if (EDGE && ld_r) {
+#pragma unroll
+        for (int t = 0; t < SJ; t++) {
+          erp[t] = p[rc[t] + off + 4];
+          erk[t] = kap[rc[t] + off + 4];
+        }
+      }
+      const bool kbnd = down ? k == 1 : k == d.n3;  // the face ahead is a boundary face
+      // d2 faces: jf[t] lies between rows j0 + t - 1 and j0 + t
+      float jf[SJ + 1][4];
+#pragma unroll
+      for (int t = 0; t <= SJ; t++) {
+        float plv[4], klv[4], puv[4], kuv[4];
+        unpack4(t == 0 ? hlp : pc[t - 1], plv);
+        unpack4(t == 0 ? hlk : kc[t - 1], klv);
+        unpack4(t == SJ ? hhp : pc[t], puv);
+        unpack4(t == SJ ? hhk : kc[t], kuv);
+        const int jl = j0 + t - 1;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const float f = (klv[q] + kuv[q]) * half;
+          float lo = plv[q], hi = puv[q];
+          if (jl == 0) lo = Rule<OP>::lo_zero ? 0.0f : hi;       // below row 1
+          if (jl == d.n2) hi = Rule<OP>::hi_zero ? 0.0f : lo;    // above row n2
+          jf[t][q] = f * (hi - lo);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        const int j = j0 + t;
+        float pv[4], kv[4], pav[4], kav[4];
+        unpack4(pc[t], pv);
+        unpack4(kc[t], kv);
+        unpack4(pa[t], pav);
+        unpack4(ka[t], kav);
+        float pL = lane_shift<true>(pv[3]), kL = lane_shift<true>(kv[3]);
+        float pR = lane_shift<false>(pv[0]), kR = lane_shift<false>(kv[0]);
+        if (EDGE && ld_l) {
+          pL = elp[t];
+          kL = elk[t];
+        }
+        if (EDGE && ld_r) {
+          pR = erp[t];
+          kR = erk[t];
+        }
+        // d1 faces: xf[q] lies between voxels q - 1 and q
+        float xf[5];
+#pragma unroll
+        for (int q = 0; q <= 4; q++) {
+          const float kl = q == 0 ? kL : kv[q - 1], kr = q == 4 ? kR : kv[q];
+          float pl = q == 0 ? pL : pv[q - 1], pr = q == 4 ? pR : pv[q];
+          if (q == 0 && left_b) pl = pr;
+          if (q == 4 && right_b) pr = pl;
+          xf[q] = ((kl + kr) * half) * (pr - pl);
+        }
+        float o[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const float xv = pv[q];
+          const float f = (kav[q] + kv[q]) * half;
+          const float nb = kbnd ? xv : pav[q];
+          const float fa = down ? f * (xv - nb) : f * (nb - xv);
+          const float fup = down ? fb[t][q] : fa, flo = down ? fa : fb[t][q];
+          float sv = xf[q + 1] - xf[q];
+          sv = sv + jf[t + 1][q];
+          sv = sv - jf[t][q];
+          sv = sv + fup;
+          sv = sv - flo;
+          float v;
+          if (Rule<OP>::diag)
+            v = psv[t][q] * xv - scale * sv;
+          else
+            v = scale * sv;
+          o[q] = v;
+          fb[t][q] = fa;
+        }
+        if (active && j <= d.n2) {
+#pragma unroll
+          for (int q = 0; q < 4; q++) acc += (double)pv[q] * (double)o[q];
+          const uint32_t dst = OUTG ? rc[t] + off : (uint32_t)d.ni(i0, j, k);
+          st4(out, dst, make_float4(o[0], o[1], o[2], o[3]));
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        pc[t] = pa[t];
+        kc[t] = ka[t];
+      }
+    }
+  }
+  if (partials) {
+    double v[1] = {acc};
+    block_partials<1>(v, partials);
+  }
+}
+
 // Initial residual with x's ghosts as stored (SBM:492-499, 649-655, 729-735):
 // res = r0 = rhs - A x; prj(interior) = res; x0 = x(interior); cg->rr = sum res^2.
 template <int OP>
@@ -523,9 +747,13 @@ __global__ __launch_bounds__(256) void k_mx_start(int64_t G2, const double2* __r
   block_partials<2>(acc, partials);
 }
 
-// One iteration of the mixed solver is the FP32 operator (k_matvec, with the
-// p.Ap partials), then the two flat kernels below, each followed by a
-// one-block finalising kernel.
+// One iteration of the mixed solver is the FP32 operator (with the p.Ap
+// partials), then the two flat kernels below, each followed by a one-block
+// finalising kernel.  (Forming alpha inside the update kernel from the
+// operator's partials instead, to save the first finaliser, made the update
+// kernel 7 us slower at 256^3, more than the 4.7 us the finaliser takes;
+// finalising in the last block of the producer needs device-scope fences,
+// which write back the L2 of every XCD: 384 us per iteration.)
 // r -= alpha w; partials of r.z and r.r.  The solution update of the same
 // iteration is left to k_mx_dir3, which reads p anyway.
 __global__ __launch_bounds__(256) void k_mx_upd3(int64_t G2, float2* __restrict__ r32, const float2* __restrict__ w32,
@@ -884,7 +1112,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   const int fb = flat_blocks(G);
   const int64_t G2 = (G + 1) / 2;  // element pairs of the FP32 pair kernels
   const int fbu = (int)((G2 + (int64_t)FLAT_T * UPD_P - 1) / ((int64_t)FLAT_T * UPD_P));
-  if (nb64 > kMaxRedBlocks || nb32 > kMaxRedBlocks || fb > kMaxRedBlocks) {
+  if (nb64 > kMaxRedBlocks || nb32 > kMaxRedBlocks || fb > kMaxRedBlocks || mv4_ntiles(d) > kMaxRedBlocks) {
     set_error("grid too large for the reduction workspace");
     return ELSIM_ERR_ARG;
   }
@@ -978,6 +1206,8 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     fin2(FIN_MX_START, fbu);
     ELSIM_CHECK_LAUNCH();
     const float sc32 = (float)a.scale;
+    const bool mv4 = mv4_ok(d), mv4_edge = mv4_tiles(d).gx > 1;
+    const int nbm = mv4 ? mv4_ntiles(d) : nb32;
     const int fbd = (int)((G2 + (int64_t)FLAT_T * DIR_P - 1) / ((int64_t)FLAT_T * DIR_P));
     int it_prev = 0;
     for (;;) {
@@ -985,10 +1215,17 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
       for (int b = 0; b < batch; b++) {
         const bool t = pr && b == 0;
         ELSIM_TRY(timed(t, ELSIM_PROF_OP32, s, [&] {
-          k_matvec<float, OP, true, true><<<nb32, blk, 0, s>>>(d, p32, (const float*)w.kap32.p,
-                                                               (const float*)w.pse32.p, sc32, w32, nullptr, cg, 1, P);
+          if (mv4 && mv4_edge)
+            k_matvec4<OP, true, true><<<nbm, 256, 0, s>>>(d, p32, (const float*)w.kap32.p, (const float*)w.pse32.p,
+                                                          sc32, w32, cg, P);
+          else if (mv4)
+            k_matvec4<OP, true, false><<<nbm, 256, 0, s>>>(d, p32, (const float*)w.kap32.p, (const float*)w.pse32.p,
+                                                           sc32, w32, cg, P);
+          else
+            k_matvec<float, OP, true, true><<<nb32, blk, 0, s>>>(d, p32, (const float*)w.kap32.p,
+                                                                 (const float*)w.pse32.p, sc32, w32, nullptr, cg, 1, P);
         }));
-        fin1(FIN_ALPHA_RZ, nb32);
+        fin1(FIN_ALPHA_RZ, nbm);
         ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] {
           k_mx_upd3<<<fbu, FLAT_T, 0, s>>>(G2, (float2*)r32, (const float2*)w32, (const uint32_t*)dinv, cg, P);
         }));
@@ -1141,6 +1378,39 @@ static int launch_mv(int op, const Dims& d, const T* p, const T* kap, const T* p
     return ELSIM_ERR_ARG;
   }
   hipStream_t s = stream();
+  if constexpr (std::is_same<T, float>::value) {
+   if (hom && mv4_ok(d)) {
+    // the operator of the FP32 CG loop
+    const int g4 = mv4_ntiles(d);
+    if (g4 > kMaxRedBlocks) {
+      set_error("grid too large for the reduction workspace");
+      return ELSIM_ERR_ARG;
+    }
+    const float *pf = p, *kf = kap, *sf = pse;
+    float* of = out;
+    const float scf = scale;
+    double* Pd = dot ? P : nullptr;
+    const bool edge = mv4_tiles(d).gx > 1;
+#define MV4(OPV)                                                                               \
+  do {                                                                                         \
+    if (edge)                                                                                  \
+      k_matvec4<OPV, false, true><<<g4, 256, 0, s>>>(d, pf, kf, sf, scf, of, nullptr, Pd);     \
+    else                                                                                       \
+      k_matvec4<OPV, false, false><<<g4, 256, 0, s>>>(d, pf, kf, sf, scf, of, nullptr, Pd);    \
+  } while (0)
+    switch (op) {
+      case ELSIM_OP_CNE: MV4(ELSIM_OP_CNE); break;
+      case ELSIM_OP_PHP: MV4(ELSIM_OP_PHP); break;
+      case ELSIM_OP_PHE: MV4(ELSIM_OP_PHE); break;
+      case ELSIM_OP_POT_DD: MV4(ELSIM_OP_POT_DD); break;
+      default: set_error("unknown operator %d", op); return ELSIM_ERR_ARG;
+    }
+#undef MV4
+    if (dot) k_fin<1><<<1, FIN_T, 0, s>>>(FIN_DOT, g4, P, nullptr, dot, 0.0);
+    ELSIM_CHECK_LAUNCH();
+    return ELSIM_OK;
+   }
+  }
 #define MV(OPV, H) \
   k_matvec<T, OPV, H, false><<<grid, blk, 0, s>>>(d, p, kap, pse, scale, out, dot, nullptr, 0, P)
   if (op == ELSIM_OP_CNE) {

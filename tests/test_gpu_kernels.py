@@ -77,6 +77,56 @@ def test_matvec_f32(gpu, oracle_lib, op):
     assert np.abs(got - ref).max() <= 64 * np.finfo(np.float32).eps * np.abs(kap).max() * np.abs(p).max()
 
 
+def apply_op_f32(op, kap, pse, pg, scale):
+    """The 7-point operator in IEEE single precision, term by term in the
+    order of the GPU kernels (SBM:515-521, 670-676, 751-757): face
+    coefficients 0.5 (k_nb + k_c), then +f2 dp - f1 dp + f4 dp - f3 dp + f6 dp
+    - f5 dp.  pg carries the homogeneous ghost rule (oracle prj_bc)."""
+    h = np.float32(0.5)
+    n = tuple(s - 2 for s in pg.shape)
+
+    def sh(a, ax, o):
+        sl = [slice(1, 1 + n[0]), slice(1, 1 + n[1]), slice(1, 1 + n[2])]
+        sl[ax] = slice(1 + o, 1 + o + n[ax])
+        return a[tuple(sl)]
+
+    x, k0 = sh(pg, 0, 0), sh(kap, 0, 0)
+    f = lambda ax, o: (sh(kap, ax, o) + k0) * h  # noqa: E731
+    sv = f(0, 1) * (sh(pg, 0, 1) - x) - f(0, -1) * (x - sh(pg, 0, -1))
+    sv = sv + f(1, 1) * (sh(pg, 1, 1) - x)
+    sv = sv - f(1, -1) * (x - sh(pg, 1, -1))
+    sv = sv + f(2, 1) * (sh(pg, 2, 1) - x)
+    sv = sv - f(2, -1) * (x - sh(pg, 2, -1))
+    return sh(pse, 0, 0) * x - scale * sv if op == E.OP_CNE else scale * sv
+
+
+# d1 lengths: not a multiple of 4 (the one-voxel-a-lane operator), and
+# multiples of 4 for the four-voxels-a-lane one: 64- and 128-wide segments,
+# one full 256 tile, a partial second tile (300: the d1 tile-edge loads) and
+# two full tiles (512)
+SHAPES_F32 = [(7, 5, 6), (70, 33, 20), (64, 9, 17), (100, 7, 5), (128, 6, 18), (256, 20, 19), (300, 7, 5),
+              (512, 5, 9)]
+
+
+@pytest.mark.parametrize("op", [E.OP_CNE, E.OP_PHP, E.OP_PHE, 3])
+@pytest.mark.parametrize("shape", SHAPES_F32)
+def test_matvec_f32_bit_exact(gpu, oracle_lib, op, shape):
+    O = oracle_lib
+    p, kap, pse = rand_fields(shape, 21 + op)
+    scale = np.float32(1e-3 / 0.49 if op == E.OP_CNE else 1.0 / 0.49)
+    pg = O.prj_bc(op, p.astype(np.float32).astype(np.float64)).astype(np.float32)
+    k32, s32 = kap.astype(np.float32), pse.astype(np.float32)
+    ref = apply_op_f32(op, k32, s32, pg, scale)
+    dp, dk, ds = (E.DeviceArray.from_host(O.flat(a, np.float32)) for a in (p, kap, pse))
+    out = E.DeviceArray(int(np.prod(shape)), np.float32)
+    dot = E.DeviceArray(1, np.float64)
+    E.check(gpu.elsim_dev_matvec_f32(op, *shape, dp.ptr, dk.ptr, ds.ptr, C.c_float(scale), out.ptr, dot.ptr))
+    got = O.unflat(out.to_host(), shape)
+    assert np.array_equal(got, ref), (np.abs(got - ref).max(), np.argwhere(got != ref)[:5])
+    want = float(np.sum(pg[1:-1, 1:-1, 1:-1].astype(np.float64) * ref.astype(np.float64)))
+    assert dot.to_host()[0] == pytest.approx(want, rel=1e-12)
+
+
 def sign_field(shape, seed):
     rng = np.random.default_rng(seed)
     g = tuple(s + 2 for s in shape)
