@@ -42,13 +42,14 @@ E_OP_PHP = 1
 
 # HBM bytes per ghosted element moved by the FP32 CG kernels inside the CG
 # loop at 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs over the
-# launches that ran, scaled by the counts of a device copy of known size
-# (FETCH x 2.0, WRITE x 1.0); profiles/cg_kernels_256_pmc.md.  The flat
-# kernels sweep the whole ghosted box, (n+2)^3 elements.
-PMC_BYTES_PER_GHOSTED = {"upd32": (83891.2 * 2 + 67354.2) * 1024 / 258 ** 3,
-                         "dir32": (117425.8 * 2 + 134885.4) * 1024 / 258 ** 3}
+# launches that ran, in KiB per dispatch, scaled FETCH x 2.0 and WRITE x 1.0
+# (the factors under which the pointwise kernels, whose bytes are known
+# exactly, read their algorithmic bytes); profiles/cg_kernels_256_pmc.md.
+# The flat kernels sweep the whole ghosted box, (n+2)^3 elements.
+PMC_BYTES_PER_GHOSTED = {"upd32": (83892.2 * 2 + 67354.1) * 1024 / 258 ** 3,
+                         "dir32": (117411.2 * 2 + 134168.1) * 1024 / 258 ** 3}
 # the phP operator covers the interior: bytes per interior voxel, same runs
-PMC_BYTES_PER_VOXEL_OP32 = (72105.4 * 2 + 67737.6) * 1024 / 256 ** 3
+PMC_BYTES_PER_VOXEL_OP32 = (94073.9 * 2 + 66625.1) * 1024 / 256 ** 3
 
 
 def log(*a):
@@ -344,7 +345,7 @@ def main():
                             "standalone_cold_us": t_mv * 1e6}}
     else:
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "k_matvec<float, phP> standalone, cold cache", "kernel_us": t_mv * 1e6}
+                "traffic": None, "kernel": "k_matvec4<phP> standalone, cold cache", "kernel_us": t_mv * 1e6}
 
     iters64 = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in st64])) if st64 else nan
     itersf = float(np.mean([s["it_cne"] + s["it_php"] + s["it_phe"] for s in stf])) if stf else nan
