@@ -22,13 +22,16 @@ namespace {
 constexpr double kThrUpd = (double)4.0e-1f;   // AD:118
 constexpr double kThrSnap = (double)1.0e-4f;  // AD:133, RM:80
 
-// psi = 0.5(1+tanh(d/0.75)) interior, mirrored ghosts (AD:84-97)
+// psi = 0.5(1+tanh(d/0.75)) (AD:84-97)
+__device__ __forceinline__ double ad_psi(double rd) { return 0.5 * (1.0 + tanh(rd / 0.75)); }
+
+// psi over the ghosted box: interior from d, mirrored ghosts (AD:84-97)
 __global__ void k_ad_psi(Dims d, const double* __restrict__ rdin, double* __restrict__ psi) {
   const int64_t G = d.G();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < G; q += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(q % d.g1()), j = (int)((q / d.g1()) % (d.n2 + 2)), k = (int)(q / d.g12());
     const int ic = min(max(i, 1), d.n1), jc = min(max(j, 1), d.n2), kc = min(max(k, 1), d.n3);
-    psi[q] = 0.5 * (1.0 + tanh(rdin[d.ni(ic, jc, kc)] / 0.75));
+    psi[q] = ad_psi(rdin[d.ni(ic, jc, kc)]);
   }
 }
 
@@ -207,20 +210,50 @@ __global__ __launch_bounds__(256) void k_ad_iter(Dims d, const double* __restric
   if (threadIdx.x == 0) chg_w[my_tile] = (changed ? CHANGED : 0) | (snappable ? SNAPPABLE : 0);
 }
 
-__device__ __forceinline__ int uf_find(const int* __restrict__ L, int x) {
-  int p = __hip_atomic_load(&L[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Connected-component labels (the threshold step of RM_UCNCT_3D, RM:79-82,
+// labels instead of an AD front): every in-phase voxel gets the smallest
+// interior index of its 6-connected component.  Three passes:
+//  1. a one-byte phase mask (psi read once);
+//  2. union-find inside 64 x 8 x 8 tiles in LDS: every component's tile-local
+//     root is its smallest local index, which is also its smallest interior
+//     index (both orders are (k, j, i) lexicographic), and every voxel's
+//     label is set to that root;
+//  3. unions across the three lower faces of every tile on the global
+//     labels (roots are only ever linked to smaller roots, so the smallest
+//     voxel of a component ends as its root, whatever the order of the
+//     atomics), then every label is replaced by its root.
+// At 512^3 the single-pass form (a global union for every in-phase voxel and
+// each of its three lower neighbours, phase read three times in FP64 per
+// voxel) took 27 ms per phase.
+constexpr int LTX = 64, LTY = 8, LTZ = 8, LTV = LTX * LTY * LTZ;
+
+// phase mask over the interior, straight from the distance (the same psi as
+// k_ad_psi's interior, without storing it)
+template <int PHASE>
+__global__ __launch_bounds__(256) void k_lab_mask(int64_t N, const double* __restrict__ rdin, uint8_t* __restrict__ m) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    const double p = ad_psi(rdin[q]);
+    m[q] = (PHASE == 1 ? 1.0 - p : p) >= kThrUpd;
+  }
+}
+
+template <int SCOPE>
+__device__ __forceinline__ int uf_find_s(int* L, int x) {
+  int p = __hip_atomic_load(&L[x], __ATOMIC_RELAXED, SCOPE);
   while (p != x) {
     x = p;
-    p = __hip_atomic_load(&L[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    p = __hip_atomic_load(&L[x], __ATOMIC_RELAXED, SCOPE);
   }
   return x;
 }
 
+// link the roots of a and b, the larger to the smaller
+template <int SCOPE>
 __device__ __forceinline__ void uf_union(int* L, int a, int b) {
   bool done = false;
   while (!done) {
-    a = uf_find(L, a);
-    b = uf_find(L, b);
+    a = uf_find_s<SCOPE>(L, a);
+    b = uf_find_s<SCOPE>(L, b);
     if (a < b) {
       const int old = atomicMin(&L[b], a);
       done = (old == b);
@@ -235,35 +268,97 @@ __device__ __forceinline__ void uf_union(int* L, int a, int b) {
   }
 }
 
-template <int PHASE>
-__device__ __forceinline__ bool in_phase(const double* psi, const Dims& d, int i, int j, int k) {
-  return ps_int<PHASE>(psi, d.gi(i, j, k)) >= kThrUpd;
+struct LabTiles {
+  int gx, gy, gz;
+};
+__host__ __device__ inline LabTiles lab_tiles(const Dims& d) {
+  return LabTiles{(d.n1 + LTX - 1) / LTX, (d.n2 + LTY - 1) / LTY, (d.n3 + LTZ - 1) / LTZ};
 }
 
-template <int PHASE>
-__global__ void k_lab_init(Dims d, const double* __restrict__ psi, int* __restrict__ L) {
-  const int64_t N = d.N();
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(q % d.n1) + 1, j = (int)((q / d.n1) % d.n2) + 1, k = (int)(q / ((int64_t)d.n1 * d.n2)) + 1;
-    L[q] = in_phase<PHASE>(psi, d, i, j, k) ? (int)q : -1;
+// A wave owns whole 64-voxel tile rows: the d1 runs of a row come from one
+// ballot (every voxel starts at the first voxel of its run, so no d1 unions
+// are needed), and a d2 or d3 union is skipped when the voxel's d1
+// predecessor and that predecessor's neighbour are both in phase (the same
+// two runs are joined by the predecessor's union).
+__global__ __launch_bounds__(256) void k_lab_local(Dims d, const uint8_t* __restrict__ m, int* __restrict__ L) {
+  __shared__ int lab[LTV];
+  const LabTiles t = lab_tiles(d);
+  const int tx = blockIdx.x % t.gx, ty = (blockIdx.x / t.gx) % t.gy, tz = blockIdx.x / (t.gx * t.gy);
+  const int i0 = tx * LTX, j0 = ty * LTY, k0 = tz * LTZ;  // 0-based interior coordinates
+  const int64_t nx = d.n1, nxy = (int64_t)d.n1 * d.n2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int i = i0 + lane;
+  const uint64_t below = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
+  for (int r = wv; r < LTY * LTZ; r += nw) {
+    const int j = j0 + (r & (LTY - 1)), k = k0 + r / LTY;
+    const bool in = i < d.n1 && j < d.n2 && k < d.n3 && m[i + nx * j + nxy * k];
+    const uint64_t bits = __ballot(in);
+    const uint64_t starts = bits & ~(bits << 1);
+    const uint64_t sb = starts & below;  // non-zero for an in-phase lane
+    lab[r * LTX + lane] = in && sb ? r * LTX + 63 - __builtin_clzll(sb) : -1;
+  }
+  __syncthreads();
+  for (int r = wv; r < LTY * LTZ; r += nw) {
+    const int v = r * LTX + lane, y = r & (LTY - 1), z = r / LTY;
+    if (lab[v] < 0) continue;
+    const bool prev = lane > 0 && lab[v - 1] >= 0;
+    if (y > 0 && lab[v - LTX] >= 0 && !(prev && lab[v - 1 - LTX] >= 0))
+      uf_union<__HIP_MEMORY_SCOPE_WORKGROUP>(lab, v, v - LTX);
+    if (z > 0 && lab[v - LTX * LTY] >= 0 && !(prev && lab[v - 1 - LTX * LTY] >= 0))
+      uf_union<__HIP_MEMORY_SCOPE_WORKGROUP>(lab, v, v - LTX * LTY);
+  }
+  __syncthreads();
+  for (int r = wv; r < LTY * LTZ; r += nw) {
+    const int v = r * LTX + lane;
+    const int j = j0 + (r & (LTY - 1)), k = k0 + r / LTY;
+    if (i >= d.n1 || j >= d.n2 || k >= d.n3) continue;
+    int out = -1;
+    if (lab[v] >= 0) {
+      const int rt = uf_find_s<__HIP_MEMORY_SCOPE_WORKGROUP>(lab, v);
+      out = (int)((i0 + (rt & (LTX - 1))) + nx * (j0 + ((rt / LTX) & (LTY - 1))) + nxy * (k0 + rt / (LTX * LTY)));
+    }
+    L[i + nx * j + nxy * k] = out;
   }
 }
 
-template <int PHASE>
-__global__ void k_lab_merge(Dims d, const double* __restrict__ psi, int* L) {
-  const int64_t N = d.N(), nx = d.n1, nz = (int64_t)d.n1 * d.n2;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
-    if (L[q] < 0) continue;
-    const int i = (int)(q % d.n1) + 1, j = (int)((q / d.n1) % d.n2) + 1, k = (int)(q / ((int64_t)d.n1 * d.n2)) + 1;
-    if (i > 1 && in_phase<PHASE>(psi, d, i - 1, j, k)) uf_union(L, (int)q, (int)(q - 1));
-    if (j > 1 && in_phase<PHASE>(psi, d, i, j - 1, k)) uf_union(L, (int)q, (int)(q - nx));
-    if (k > 1 && in_phase<PHASE>(psi, d, i, j, k - 1)) uf_union(L, (int)q, (int)(q - nz));
+// unions across the lower d1, d2 and d3 faces of every tile
+__global__ __launch_bounds__(256) void k_lab_link(Dims d, const uint8_t* __restrict__ m, int* L) {
+  const LabTiles t = lab_tiles(d);
+  const int tx = blockIdx.x % t.gx, ty = (blockIdx.x / t.gx) % t.gy, tz = blockIdx.x / (t.gx * t.gy);
+  const int i0 = tx * LTX, j0 = ty * LTY, k0 = tz * LTZ;
+  const int64_t nx = d.n1, nxy = (int64_t)d.n1 * d.n2;
+  constexpr int FX = LTY * LTZ, FY = LTX * LTZ, FZ = LTX * LTY;
+  for (int f = threadIdx.x; f < FX + FY + FZ; f += blockDim.x) {
+    int i, j, k;
+    int64_t step;
+    if (f < FX) {
+      if (i0 == 0) continue;
+      i = i0, j = j0 + f % LTY, k = k0 + f / LTY, step = 1;
+    } else if (f < FX + FY) {
+      if (j0 == 0) continue;
+      const int g = f - FX;
+      i = i0 + g % LTX, j = j0, k = k0 + g / LTX, step = nx;
+    } else {
+      if (k0 == 0) continue;
+      const int g = f - FX - FY;
+      i = i0 + g % LTX, j = j0 + g / LTX, k = k0, step = nxy;
+    }
+    if (i >= d.n1 || j >= d.n2 || k >= d.n3) continue;
+    const int64_t q = i + nx * j + nxy * k;
+    if (!(m[q] && m[q - step])) continue;
+    // the same two tile components are joined by the face predecessor's
+    // union when it and its neighbour are in phase too (d1 predecessor on
+    // the d2 and d3 faces, d2 predecessor on the d1 face)
+    const int64_t pre = step == 1 ? nx : 1;
+    const bool has_pre = step == 1 ? j > j0 : i > i0;
+    if (has_pre && m[q - pre] && m[q - pre - step]) continue;
+    uf_union<__HIP_MEMORY_SCOPE_AGENT>(L, (int)q, (int)(q - step));
   }
 }
 
 __global__ void k_lab_compress(int64_t N, int* L) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x)
-    if (L[q] >= 0) L[q] = uf_find(L, (int)q);
+    if (L[q] >= 0) L[q] = uf_find_s<__HIP_MEMORY_SCOPE_AGENT>(L, (int)q);
 }
 
 // flag roots of components that touch the Dirichlet d2 plane jf
@@ -326,20 +421,21 @@ int accdiff(const Dims& d, const double* rdin, int phase, int term, double* cn) 
 int label(const Dims& d, const double* rdin, int phase, int32_t* L, int32_t* conn) {
   ELSIM_TRY(ensure_init());
   hipStream_t s = stream();
-  double* psi = (double*)scratch(5, (size_t)d.G() * 8);
-  int* flag = (int*)scratch(7, (size_t)d.N() * 4);
-  if (!psi || !flag) return ELSIM_ERR_ALLOC;
   const int64_t N = d.N();
-  k_ad_psi<<<flat_blocks(d.G()), FLAT_T, 0, s>>>(d, rdin, psi);
+  // the component flags (N ints), then the phase mask (N bytes)
+  int* flag = (int*)scratch(7, (size_t)N * 5);
+  if (!flag) return ELSIM_ERR_ALLOC;
+  uint8_t* mask = (uint8_t*)(flag + N);
   ELSIM_HIP(hipMemsetAsync(flag, 0, (size_t)N * 4, s));
   const int nb = flat_blocks(N);
-  if (phase == 0) {
-    k_lab_init<0><<<nb, FLAT_T, 0, s>>>(d, psi, L);
-    k_lab_merge<0><<<nb, FLAT_T, 0, s>>>(d, psi, L);
-  } else {
-    k_lab_init<1><<<nb, FLAT_T, 0, s>>>(d, psi, L);
-    k_lab_merge<1><<<nb, FLAT_T, 0, s>>>(d, psi, L);
-  }
+  if (phase == 0)
+    k_lab_mask<0><<<nb, FLAT_T, 0, s>>>(N, rdin, mask);
+  else
+    k_lab_mask<1><<<nb, FLAT_T, 0, s>>>(N, rdin, mask);
+  const LabTiles lt = lab_tiles(d);
+  const int ntl = lt.gx * lt.gy * lt.gz;
+  k_lab_local<<<ntl, 256, 0, s>>>(d, mask, L);
+  k_lab_link<<<ntl, 256, 0, s>>>(d, mask, L);
   k_lab_compress<<<nb, FLAT_T, 0, s>>>(N, L);
   k_lab_face<<<flat_blocks((int64_t)d.n1 * d.n3), FLAT_T, 0, s>>>(d, L, phase == 0 ? d.n2 : 1, flag);
   k_lab_conn<<<nb, FLAT_T, 0, s>>>(N, L, flag, conn);
