@@ -119,6 +119,47 @@ def build(verbose=False):
     return LIB
 
 
+# Host code of the library under AddressSanitizer and UndefinedBehaviorSanitizer:
+# each sanitizer flag goes to the host compilation only (the device code is
+# built as usual, at -O1).
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+SAN_DIR = os.path.join(BUILD, "sanitize")
+
+
+def build_sanitized():
+    """libelsim_hip_san.so (host code only) and the host driver of
+    tests/sanitize/host_driver.cpp linked against it; returns the driver."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+
+    def comp(src):
+        obj = os.path.join(SAN_DIR, src + ".o")
+        flags = ["-O1", "-g", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include")] + SAN_FLAGS
+        if src.endswith(".hip"):
+            flags += ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off"]
+        _run([HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(comp, SOURCES))
+    lib = os.path.join(SAN_DIR, "libelsim_hip_san.so")
+    _run([HIPCC, "-shared", "-fPIC", "-shared-libsan"] + SAN_FLAGS + ["-o", lib] + objs + ["-lpthread"])
+    drv = os.path.join(SAN_DIR, "host_driver")
+    # the sanitizer runtimes of the compiler (a shared runtime, so that the
+    # library and the driver share one)
+    rt = subprocess.run([os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "lib", "llvm", "bin", "clang"),
+                         "--print-file-name=libclang_rt.asan-x86_64.so"], stdout=subprocess.PIPE, text=True)
+    rtdir = os.path.dirname(rt.stdout.strip()) if rt.returncode == 0 else ""
+    rpath = [SAN_DIR] + ([rtdir] if rtdir and os.path.isdir(rtdir) else [])
+    _run([HIPCC, "-O1", "-g", "-std=c++17", "-shared-libsan"] + SAN_FLAGS +
+         ["-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "sanitize", "host_driver.cpp"),
+          "-o", drv, "-L" + SAN_DIR, "-lelsim_hip_san"] + ["-Wl,-rpath," + r for r in rpath])
+    return drv
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    if "--sanitize" in sys.argv:
+        print("built", build_sanitized())
+    else:
+        build(verbose=True)
     sys.exit(0)

@@ -899,7 +899,11 @@ NnNet* net_of(const int64_t* h) {
   if (!h || !*h) return nullptr;
   std::lock_guard<std::mutex> lk(g_nn_mu);
   auto it = g_nn.find(*h);
-  return it == g_nn.end() ? nullptr : it->second.get();
+  if (it == g_nn.end()) {
+    set_error("unknown network handle %lld", (long long)*h);
+    return nullptr;
+  }
+  return it->second.get();
 }
 }  // namespace
 

@@ -491,7 +491,11 @@ SbmState* lookup(const int64_t* h) {
   if (!h) return nullptr;
   std::lock_guard<std::mutex> lk(g_sbm_mu);
   auto it = g_sbm.find(*h);
-  return it == g_sbm.end() ? nullptr : it->second.get();
+  if (it == g_sbm.end()) {
+    set_error("unknown SBM handle %lld", (long long)*h);
+    return nullptr;
+  }
+  return it->second.get();
 }
 
 int launch_bc(const Dims& d, double* f, const BcArgs& b) {
