@@ -82,7 +82,8 @@ def main(argv=None):
     sdf, cstats = clean(sdf, dcut)
     log(f"{n}^3: {ns} ellipsoids, solid {frac:.3f}, generated in {t_gen:.1f} s; clean-up {cstats}")
     wsm = {"plain": E.WS_PLAIN, "extrap": E.WS_EXTRAP, "learned": E.WS_LEARNED}[a.ws]
-    s = E.Sbm(sdf, -sdf, E.synthetic_params(ws_mode=wsm), E.tables())
+    # as the 256^3 benchmark: the CnE solve resolves the step's change to rtol
+    s = E.Sbm(sdf, -sdf, E.synthetic_params(ws_mode=wsm, cne_rule=1), E.tables())
     s.set(F_CNP, E.initial_cnp(n, a.seed))
     net = None
     if a.ws == "learned":
@@ -115,7 +116,8 @@ def main(argv=None):
             kern[k] = {"us": us, "GBps": KERNEL_BYTES[k] * N / (us * 1e-6) / 1e9, "launches": int(prof.count[i])}
     log(f"  in-loop kernels: {kern}")
     rep = {"config": {"grid": [n, n, n], "shapes": ns, "solid_fraction": frac, "semi_axes": [8, 24], "cone_deg": 15,
-                      "tunnel_radius": 3, "seed": a.seed, "warm_start": a.ws, "mode": "fp32 jacobi-pcg + fp64 reliable"},
+                      "tunnel_radius": 3, "seed": a.seed, "warm_start": a.ws, "mode": "fp32 jacobi-pcg + fp64 reliable",
+                      "cne_rule": "relative to the step's change (||r0||)"},
            "generate_s": t_gen, "cleanup": cstats, "steps": steps,
            "s_per_step_after_first": float(np.mean([x["wall_s"] for x in steps[1:]])) if len(steps) > 1 else None,
            "s_per_step_last_half": float(np.mean([x["wall_s"] for x in steps[len(steps) // 2:]])) if steps else None,
