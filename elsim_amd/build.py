@@ -39,6 +39,9 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# named ranges of a step's phases (rocprofv3 --marker-trace)
+ROCTX = ["-L" + os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib"),
+         "-lrocprofiler-sdk-roctx"]
 # the learned warm start is not bound by bit-parity: contraction allowed
 PER_FILE = {"elsim_nn.hip": ["-ffp-contract=fast"]}
 
@@ -110,7 +113,7 @@ def build(verbose=False):
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ROCTX + ["-lpthread"])
     os.replace(tmp, LIB)
     if all(os.path.exists(os.path.join(FORTRAN, s)) for s in ("elsim_gpu.f90", "elsim_dropin.f90", "sbm_run.f90")):
         build_fortran()
@@ -143,7 +146,7 @@ def build_sanitized():
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(comp, SOURCES))
     lib = os.path.join(SAN_DIR, "libelsim_hip_san.so")
-    _run([HIPCC, "-shared", "-fPIC", "-shared-libsan"] + SAN_FLAGS + ["-o", lib] + objs + ["-lpthread"])
+    _run([HIPCC, "-shared", "-fPIC", "-shared-libsan"] + SAN_FLAGS + ["-o", lib] + objs + ROCTX + ["-lpthread"])
     drv = os.path.join(SAN_DIR, "host_driver")
     # the sanitizer runtimes of the compiler (a shared runtime, so that the
     # library and the driver share one)

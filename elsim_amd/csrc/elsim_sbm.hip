@@ -16,6 +16,8 @@
 #include <memory>
 #include <mutex>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "elsim_nn.h"
 #include "elsim_solver.h"
 
@@ -530,8 +532,18 @@ int field_info(SbmState& s, int f, double** p, size_t* n) {
   return ELSIM_ERR_ARG;
 }
 
+// A named host range over a phase of the step, for rocprofv3 --marker-trace
+// (the solves poll the device, so a range spans its phase's GPU work).
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
+
 // One CG solve of the state's current system.
 int solve(SbmState& s, int op, int mode, SolveResult* r) {
+  const Range range(op == ELSIM_OP_CNE ? "cne_solve" : (op == ELSIM_OP_PHP ? "php_solve" : "phe_solve"));
   const elsim_sbm_params& P = s.P;
   SolveArgs a{};
   a.d = s.d;
@@ -779,6 +791,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
   double cvgp = 1.0, cvgl = 1.0;
   int nlp = 0;
   auto butler_volmer = [&]() -> int {
+    const Range range("butler_volmer");
     k_bv<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.kfw), s.g(s.kbw),
                                 s.g(s.avp), s.g(s.rhe), P, s.g(s.rxn), s.g(s.rxc), s.g(s.rhsp), s.g(s.rhse));
     ELSIM_CHECK_LAUNCH();
@@ -840,7 +853,10 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
         ELSIM_CHECK_LAUNCH();
       }
       if (s.ws_net && rms_r > 0.0) {
-        ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p));
+        {
+          const Range range("ws_network");
+          ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p));
+        }
         double* pd = s.g(s.ws_cand);  // correction direction, zero ghosts
         ELSIM_HIP(hipMemsetAsync(pd, 0, (size_t)d.G() * sizeof(double), stv));
         k_ws_apply<<<grid, blk, 0, stv>>>(d, pd, (const float*)s.nn_pred.p, 1.0, pd);
@@ -956,6 +972,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
 // previous fields and counted itself: the warm-start history is then
 // inconsistent, so it is dropped and the next step starts plain.
 int step(SbmState& s, int mode, elsim_step_stats* st) {
+  const Range range("sbm_step");
   const int rc = step_impl(s, mode, st);
   if (rc < 0) {
     s.nprev = 0;

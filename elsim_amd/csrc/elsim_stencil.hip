@@ -935,14 +935,12 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
         cg->done = 2;
       } else {
         // Restart the FP32 recurrence from the true residual: the next
-        // k_mx_dir3 forms p = dinv r (beta = 0).  Continuing the old
-        // direction with the replaced residual (beta = r.z ratio) measured
-        // 823 phP iterations per step at 256^3 with the extrapolated start,
-        // against 212 with restarts (restarting only when the recurred
-        // residual had drifted from the true one by over 1 %: 837): the
-        // restarted solves end further below the tolerance, and the warm
-        // start extrapolates from them.  With the plain start, restarts cost
-        // iterations (3779 against 2369 per step).
+        // k_mx_dir3 forms p = dinv r (beta = 0).  Against continuing the old
+        // direction with the replaced residual (beta = r.z ratio), and
+        // against restarts with delta = 0.1, the mean over six 256^3 packs x
+        // 48 warm-started steps was 0.159 s/step (restart, delta 0.5), 0.168
+        // and 0.165 s/step: within the spread between packs (0.09-0.24 s
+        // for one setting), which the current control's sign changes set.
         cg->pending_y = 0;
         cg->beta = 0.0;
         cg->rz = tot[1];
