@@ -5,7 +5,7 @@ Python here is host plumbing: the ctypes binding of libelsim_hip.so, synthetic
 inputs and the Fortran record format.  Every numerical operation runs in the
 HIP library; nothing falls back to the CPU.
 """
-from ._lib import (MODE_FP64, MODE_FP64_PCG, MODE_MIXED, OP_CNE, OP_PHE, OP_PHP, STOP_REFERENCE, STOP_RELRES, WS_EXTRAP,
+from ._lib import (MODE_FP64, MODE_FP64_PCG, MODE_MIXED, MODE_MIXED_MG, OP_CNE, OP_PHE, OP_PHP, STOP_REFERENCE, STOP_RELRES, WS_EXTRAP,
                    WS_LEARNED, WS_PLAIN, DeviceArray,
                    ElsimError, SbmParams, StepStats, check, exported_symbols, lib)
 from . import fileio
@@ -13,7 +13,7 @@ from .sbm import (Sbm, carve_tunnel, ellipsoid_pack, initial_cnp, potential_prob
                   sphere_pack, synthetic_params, tables, time_control)
 
 __all__ = [
-    "MODE_FP64", "MODE_FP64_PCG", "MODE_MIXED", "OP_CNE", "OP_PHP", "OP_PHE", "STOP_REFERENCE", "STOP_RELRES", "WS_EXTRAP",
+    "MODE_FP64", "MODE_FP64_PCG", "MODE_MIXED", "MODE_MIXED_MG", "OP_CNE", "OP_PHP", "OP_PHE", "STOP_REFERENCE", "STOP_RELRES", "WS_EXTRAP",
     "WS_LEARNED", "WS_PLAIN", "DeviceArray",
     "ElsimError", "SbmParams", "StepStats", "check", "exported_symbols", "lib", "fileio", "Sbm", "carve_tunnel",
     "ellipsoid_pack", "initial_cnp", "reference_params", "sphere_pack", "synthetic_params", "tables",

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libelsim_hip.so")
 OK, NOT_CONVERGED = 0, 1
 ERR_ARG, ERR_HIP, ERR_NAN, ERR_ALLOC = -1, -2, -3, -4
 STOP_REFERENCE, STOP_RELRES = 0, 1
-MODE_FP64, MODE_MIXED, MODE_FP64_PCG = 0, 1, 2
+MODE_FP64, MODE_MIXED, MODE_FP64_PCG, MODE_MIXED_MG = 0, 1, 2, 3
 OP_CNE, OP_PHP, OP_PHE = 0, 1, 2
 WS_PLAIN, WS_EXTRAP, WS_LEARNED = 0, 1, 2
 F_PSP, F_PSE, F_AVP, F_CNP, F_CNE, F_PHP, F_PHE, F_RXN, F_KAPP = range(9)
@@ -59,12 +59,13 @@ class StepStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
-PROF_OP32, PROF_UPD32, PROF_DIR32, PROF_OP64, PROF_UPD64, PROF_DIR64 = range(6)
+PROF_OP32, PROF_UPD32, PROF_DIR32, PROF_OP64, PROF_UPD64, PROF_DIR64, PROF_MG = range(7)
+PROF_N = 7
 
 
 class KernelProf(C.Structure):
     """Mirror of elsim_kernel_prof."""
-    _fields_ = [("count", C.c_int32 * 6), ("ms_total", C.c_double * 6)]
+    _fields_ = [("count", C.c_int32 * PROF_N), ("ms_total", C.c_double * PROF_N)]
 
     def mean_us(self, slot):
         return self.ms_total[slot] * 1e3 / self.count[slot] if self.count[slot] else float("nan")
@@ -143,6 +144,7 @@ _SIGS = {
     "elsim_nn_conv3_bwd_weight": (I, [I, I, I, P, I, P, I, P]),
     "elsim_prof_read": (I, [C.POINTER(KernelProf)]),
     "elsim_dev_matvec_f32": (I, [I, I, I, I, P, P, P, C.c_float, P, P]),
+    "elsim_dev_mg_apply": (I, [I, I, I, I, P, P, D, P, P]),
     "elsim_gen_sphere_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, I, D, P, IP, DP]),
     "elsim_gen_ellipsoid_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, D, D, P, IP, DP]),
     "elsim_gen_tunnel": (I, [I, I, I, C.c_uint64, D, P]),

@@ -22,6 +22,7 @@ ARCH = os.environ.get("ELSIM_ARCH", "gfx950")
 SOURCES = [
     "elsim_runtime.hip",
     "elsim_stencil.hip",
+    "elsim_mg.hip",
     "elsim_sbm.hip",
     "elsim_levelset.hip",
     "elsim_connect.hip",

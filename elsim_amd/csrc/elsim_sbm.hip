@@ -1002,7 +1002,8 @@ int pot_args_check(int n1, int n2, int n3, double dx, const void* kap, const voi
               bc_hi);
     return ELSIM_ERR_ARG;
   }
-  if (mode != ELSIM_MODE_FP64 && mode != ELSIM_MODE_MIXED && mode != ELSIM_MODE_FP64_PCG) {
+  if (mode != ELSIM_MODE_FP64 && mode != ELSIM_MODE_MIXED && mode != ELSIM_MODE_FP64_PCG &&
+      mode != ELSIM_MODE_MIXED_MG) {
     set_error("unknown mode %d", mode);
     return ELSIM_ERR_ARG;
   }
@@ -1210,7 +1211,9 @@ int elsim_sbm_set_params(const int64_t* handle, const elsim_sbm_params* prm) {
 
 int elsim_sbm_step(const int64_t* handle, const int* mode, elsim_step_stats* stats) {
   SbmState* s = lookup(handle);
-  if (!s || !mode || (*mode != ELSIM_MODE_FP64 && *mode != ELSIM_MODE_MIXED && *mode != ELSIM_MODE_FP64_PCG))
+  if (!s || !mode ||
+      (*mode != ELSIM_MODE_FP64 && *mode != ELSIM_MODE_MIXED && *mode != ELSIM_MODE_FP64_PCG &&
+       *mode != ELSIM_MODE_MIXED_MG))
     return ELSIM_ERR_ARG;
   return step(*s, *mode, stats);
 }

@@ -31,6 +31,8 @@ struct CgDev {
   int pending_y;   // mixed: this iteration's y += alpha p is still to be applied (by k_mx_dir3)
 };
 
+struct MgWork;
+
 // Device buffers of one grid's solves (allocated once per grid).
 struct CgWork {
   Dims d{0, 0, 0};
@@ -43,6 +45,7 @@ struct CgWork {
   CgDev* host = nullptr;             // pinned mirror
   double* partials = nullptr;        // reduction partials (two values); null: the library workspace
   bool bound = false;                // buffers live in memory the caller provided (never freed here)
+  MgWork* mg = nullptr;              // multigrid hierarchy (ELSIM_MODE_MIXED_MG), created on first use
   int alloc(const Dims& d);
   // Lay the buffers of grid d out in [base, base + bytes) (no allocation),
   // with `host` as the scalars' host mirror, and clear them with a fill

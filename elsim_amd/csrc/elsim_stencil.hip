@@ -36,192 +36,11 @@
 #include <type_traits>
 #include <vector>
 
+#include "elsim_mg.h"
 #include "elsim_solver.h"
+#include "elsim_sweep.h"
 
 namespace elsim {
-
-template <int OP>
-struct Rule;
-template <>
-struct Rule<ELSIM_OP_CNE> {
-  static constexpr bool lo_zero = false, hi_zero = false, diag = true;
-};
-template <>
-struct Rule<ELSIM_OP_PHP> {
-  static constexpr bool lo_zero = false, hi_zero = true, diag = false;
-};
-template <>
-struct Rule<ELSIM_OP_PHE> {
-  static constexpr bool lo_zero = true, hi_zero = false, diag = false;
-};
-template <>
-struct Rule<ELSIM_OP_POT_DD> {
-  static constexpr bool lo_zero = true, hi_zero = true, diag = false;
-};
-
-// A p over one stencil tile (256 d1 voxels x SJ rows x up to SKC planes);
-// epi(c_ghosted, n_interior, p_centre, value) for every interior voxel of the
-// tile.  Every lane of the block must call it (the d1 neighbours are exchanged
-// between lanes).  Rows past n2 + 1 and columns past n1 + 1 read the last
-// ghost row/column, so the voxel next to an edge finds the ghost value in the
-// neighbouring register or lane, as the non-homogeneous operator needs.
-// Even d3 tiles march upward, odd ones downward: a tile then reads its halo
-// plane at the same time as the neighbouring tile that owns it (in the same
-// L2 or the shared last-level cache), instead of one tile-time apart, which
-// cut the operator's fetched bytes from 1.13 to 1.07 times its algorithmic
-// bytes at 256^3.  Per-voxel arithmetic does not depend on the direction.
-// The values of p come from ld(q) (q a ghosted index), so that a caller can
-// form p on the fly from other arrays (the fused CG direction update).
-template <typename T, int OP, bool HOM, class Ld, class Epi>
-__device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld, const T* __restrict__ kap,
-                                         const T* __restrict__ pse, T scale, Epi&& epi) {
-  constexpr int SJ = Rows<T>::SJ;
-  const int64_t sz = d.g12(), nz = (int64_t)d.n1 * d.n2;
-  const int i = s.i;
-  const bool col_ok = i <= d.n1;
-  const bool down = ((s.k0 - 1) / SKC) & 1;
-  const int kstart = down ? s.k1 : s.k0;
-  const int64_t step = down ? -sz : sz, nstep = down ? -nz : nz;
-  int64_t rc[SJ];
-#pragma unroll
-  for (int t = 0; t < SJ; t++) rc[t] = d.gi(s.ic, min(s.j0 + t, d.n2 + 1), kstart);
-  const int64_t clo = d.gi(s.ic, min(s.j0 - 1, d.n2 + 1), kstart),
-                chi = d.gi(s.ic, min(s.j0 + SJ, d.n2 + 1), kstart);
-  const bool ld_left = s.lane == 0 && (i > 1 || !HOM);
-  const bool ld_right = s.lane == 63 && (i < d.n1 || (!HOM && i == d.n1));
-  const T half = T(0.5);
-  T pc[SJ], kc[SJ], pb[SJ], kb[SJ];  // plane k, the plane behind (k - direction)
-#pragma unroll
-  for (int t = 0; t < SJ; t++) {
-    pc[t] = ld(rc[t]);
-    kc[t] = kap[rc[t]];
-    kb[t] = kap[rc[t] - step];
-    pb[t] = ld(rc[t] - step);
-  }
-  int64_t off = 0, noff = (int64_t)(i - 1) + (int64_t)d.n1 * ((s.j0 - 1) + (int64_t)d.n2 * (kstart - 1));
-  const int nk = s.k1 - s.k0 + 1;
-  for (int kk = 0; kk < nk; kk++, off += step, noff += nstep) {
-    const int k = down ? s.k1 - kk : s.k0 + kk;
-    T pa[SJ], ka[SJ], ps[SJ];  // plane ahead (k + direction)
-#pragma unroll
-    for (int t = 0; t < SJ; t++) {
-      pa[t] = ld(rc[t] + off + step);
-      ka[t] = kap[rc[t] + off + step];
-      if (Rule<OP>::diag) ps[t] = pse[rc[t] + off];
-    }
-    const T pjlo = ld(clo + off), kjlo = kap[clo + off], pjhi = ld(chi + off), kjhi = kap[chi + off];
-    T pim[SJ], pip[SJ], kim[SJ], kip[SJ];
-#pragma unroll
-    for (int t = 0; t < SJ; t++) {
-      pim[t] = lane_shift<true>(pc[t]);
-      pip[t] = lane_shift<false>(pc[t]);
-      kim[t] = lane_shift<true>(kc[t]);
-      kip[t] = lane_shift<false>(kc[t]);
-    }
-    if (ld_left) {
-#pragma unroll
-      for (int t = 0; t < SJ; t++) {
-        pim[t] = ld(rc[t] + off - 1);
-        kim[t] = kap[rc[t] + off - 1];
-      }
-    }
-    if (ld_right) {
-#pragma unroll
-      for (int t = 0; t < SJ; t++) {
-        pip[t] = ld(rc[t] + off + 1);
-        kip[t] = kap[rc[t] + off + 1];
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < SJ; t++) {
-      const int j = s.j0 + t;
-      const T x = pc[t];
-      T pz = down ? pb[t] : pa[t], pm = down ? pa[t] : pb[t];
-      const T kz = down ? kb[t] : ka[t], km = down ? ka[t] : kb[t];
-      T pl = pim[t], pr = pip[t];
-      T pjm = t == 0 ? pjlo : pc[t - 1];
-      T pjp = t == SJ - 1 ? pjhi : pc[t + 1];
-      const T kjm = t == 0 ? kjlo : kc[t - 1];
-      const T kjp = t == SJ - 1 ? kjhi : kc[t + 1];
-      if (HOM) {
-        if (k == 1) pm = x;
-        if (k == d.n3) pz = x;
-        if (i == 1) pl = x;
-        if (i == d.n1) pr = x;
-        if (j == 1) pjm = Rule<OP>::lo_zero ? T(0) : x;
-        if (j == d.n2) pjp = Rule<OP>::hi_zero ? T(0) : x;
-      }
-      const T k0v = kc[t];
-      const T f1 = (kim[t] + k0v) * half, f2 = (kip[t] + k0v) * half;
-      const T f3 = (kjm + k0v) * half, f4 = (kjp + k0v) * half;
-      const T f5 = (km + k0v) * half, f6 = (kz + k0v) * half;
-      T sv = f2 * (pr - x) - f1 * (x - pl);
-      sv = sv + f4 * (pjp - x);
-      sv = sv - f3 * (x - pjm);
-      sv = sv + f6 * (pz - x);
-      sv = sv - f5 * (x - pm);
-      T v;
-      if (Rule<OP>::diag)
-        v = ps[t] * x - scale * sv;
-      else
-        v = scale * sv;
-      if (col_ok && j <= d.n2) epi(rc[t] + off, noff + (int64_t)t * d.n1, x, v);
-    }
-#pragma unroll
-    for (int t = 0; t < SJ; t++) {
-      pb[t] = pc[t];
-      pc[t] = pa[t];
-      kb[t] = kc[t];
-      kc[t] = ka[t];
-    }
-  }
-}
-
-template <typename T, int OP, bool HOM, class Epi>
-__device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __restrict__ p,
-                                      const T* __restrict__ kap, const T* __restrict__ pse, T scale, Epi&& epi) {
-  sweep_ld<T, OP, HOM>(
-      d, s, [p](int64_t q) { return p[q]; }, kap, pse, scale, epi);
-}
-
-// Pointwise walk over a block's columns (interior only).
-template <class F>
-__device__ __forceinline__ void for_col(const Dims& d, const ColIdx& ci, F&& f) {
-  if (!ci.active) return;
-  const int64_t sz = d.g12(), nz = (int64_t)d.n1 * d.n2;
-  int64_t c = d.gi(ci.i, ci.j, ci.k0), n = d.ni(ci.i, ci.j, ci.k0);
-  for (int k = ci.k0; k <= ci.k1; k++, c += sz, n += nz) f(c, n, k);
-}
-
-// Same with U items per thread (fewer blocks, so fewer reduction partials).
-#define FLAT_LOOP_U(U, n, LOADS, BODY)                                                  \
-  {                                                                                      \
-    const int64_t base_ = (int64_t)blockIdx.x * (FLAT_T * (U)) + threadIdx.x;           \
-    _Pragma("unroll") for (int u = 0; u < (U); u++) {                                    \
-      const int64_t q = base_ + u * FLAT_T;                                              \
-      if (q < (n)) { LOADS }                                                             \
-    }                                                                                    \
-    _Pragma("unroll") for (int u = 0; u < (U); u++) {                                    \
-      const int64_t q = base_ + u * FLAT_T;                                              \
-      if (q < (n)) { BODY }                                                              \
-    }                                                                                    \
-  }
-
-// Flat loop over [0, n): block b owns the contiguous tile
-// [b*FLAT_T*FLAT_U, (b+1)*FLAT_T*FLAT_U); a load phase then a compute/store
-// phase, so the FLAT_U loads of a thread are in flight together.
-#define FLAT_LOOP(n, LOADS, BODY)                                                        \
-  {                                                                                      \
-    const int64_t base_ = (int64_t)blockIdx.x * (FLAT_T * FLAT_U) + threadIdx.x;        \
-    _Pragma("unroll") for (int u = 0; u < FLAT_U; u++) {                                 \
-      const int64_t q = base_ + u * FLAT_T;                                              \
-      if (q < (n)) { LOADS }                                                             \
-    }                                                                                    \
-    _Pragma("unroll") for (int u = 0; u < FLAT_U; u++) {                                 \
-      const int64_t q = base_ + u * FLAT_T;                                              \
-      if (q < (n)) { BODY }                                                              \
-    }                                                                                    \
-  }
 
 // ------------------------------------------------------------- kernels --
 
@@ -819,6 +638,65 @@ __global__ __launch_bounds__(256) void k_mx_dir3(int64_t G2, float2* __restrict_
   }
 }
 
+// ------------------------------------- multigrid-preconditioned mixed CG --
+// The same FP32 recurrence with z = M^-1 r from one V-cycle (elsim_mg.hip)
+// written into w32 once the update kernel has consumed A p there.
+
+// Start: r32 = r0, y32 = p32 = 0 (the first direction is z itself), e64 = 0.
+__global__ __launch_bounds__(256) void k_mg_start(int64_t G2, const double2* __restrict__ r0, float2* __restrict__ r32,
+                                                  float2* __restrict__ p32, float2* __restrict__ y32,
+                                                  double2* __restrict__ e64) {
+  double2 rv[UPD_P];
+  FLAT_LOOP_U(UPD_P, G2, { rv[u] = r0[q]; }, {
+    r32[q] = make_float2((float)rv[u].x, (float)rv[u].y);
+    p32[q] = make_float2(0.f, 0.f);
+    y32[q] = make_float2(0.f, 0.f);
+    e64[q] = make_double2(0.0, 0.0);
+  })
+}
+
+// r -= alpha w (the V-cycle that follows reduces r.r and r.z).
+__global__ __launch_bounds__(256) void k_mg_upd(int64_t G2, float2* __restrict__ r32, const float2* __restrict__ w32,
+                                                const CgDev* cg) {
+  if (cg->done || cg->need_reliable) return;
+  const float alp = (float)cg->alpha;
+  float2 rv[UPD_P], wv[UPD_P];
+  FLAT_LOOP_U(UPD_P, G2, { rv[u] = r32[q]; wv[u] = w32[q]; }, {
+    float2 rn;
+    rn.x = rv[u].x - alp * wv[u].x;
+    rn.y = rv[u].y - alp * wv[u].y;
+    r32[q] = rn;
+  })
+}
+
+// y += alpha p (pending_y), then p = z + beta p unless the iteration ended
+// the batch.
+__global__ __launch_bounds__(256) void k_mg_dir3(int64_t G2, float2* __restrict__ y32, float2* __restrict__ p32,
+                                                 const float2* __restrict__ z32, const CgDev* cg) {
+  const bool upd_y = cg->pending_y != 0, upd_p = !(cg->done || cg->need_reliable);
+  const float alp = (float)cg->alpha, bet = (float)cg->beta;
+  if (!upd_y && !upd_p) return;
+  float2 yv[DIR_P], zv[DIR_P], pv[DIR_P];
+  FLAT_LOOP_U(DIR_P, G2, {
+    pv[u] = p32[q];
+    if (upd_y) yv[u] = y32[q];
+    if (upd_p) zv[u] = z32[q];
+  }, {
+    if (upd_y) {
+      float2 yn;
+      yn.x = yv[u].x + alp * pv[u].x;
+      yn.y = yv[u].y + alp * pv[u].y;
+      y32[q] = yn;
+    }
+    if (upd_p) {
+      float2 pn;
+      pn.x = zv[u].x + bet * pv[u].x;
+      pn.y = zv[u].y + bet * pv[u].y;
+      p32[q] = pn;
+    }
+  })
+}
+
 // Reliable update, part 1: e64 += y32, y32 = 0.
 __global__ __launch_bounds__(256) void k_mx_fold(int64_t G, double* __restrict__ e64, float* __restrict__ y32,
                                                  const CgDev* cg) {
@@ -856,7 +734,9 @@ enum FinOp {
   FIN_RZ,             // rz = sum
   FIN_PCG64,          // FP64 Jacobi-PCG: beta, stop
   FIN_MX_UPDATE,      // three-kernel mixed path: beta or request a reliable update
-  FIN_MX_START        // three-kernel mixed path: rz of the start (second value)
+  FIN_MX_START,       // three-kernel mixed path: rz of the start (second value)
+  FIN_MG_UPDATE,      // multigrid path: r.r (first value) and r.z (second) after the V-cycle
+  FIN_MG_START        // multigrid path: rz of the start or restart, beta = 0
 };
 
 // The skip test repeats the producer's, evaluated on the same (unchanged)
@@ -866,8 +746,9 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
                                                double* dot, double rtol) {
   bool skip = false;
   if (cg) {
-    if (op == FIN_ALPHA_RR || op == FIN_ALPHA_RZ || op == FIN_MX_UPDATE) skip = cg->done || cg->need_reliable;
-    if (op == FIN_CG64 || op == FIN_PCG64) skip = cg->done;
+    if (op == FIN_ALPHA_RR || op == FIN_ALPHA_RZ || op == FIN_MX_UPDATE || op == FIN_MG_UPDATE)
+      skip = cg->done || cg->need_reliable;
+    if (op == FIN_CG64 || op == FIN_PCG64 || op == FIN_MG_START) skip = cg->done;
     if (op == FIN_TRUE_RELIABLE) skip = cg->done || !cg->need_reliable;
   }
   // three-kernel mixed path: the iteration whose alpha this is will update
@@ -959,6 +840,26 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
       } else {
         cg->beta = tot[0] / cg->rz;
         cg->rz = tot[0];
+      }
+      break;
+    case FIN_MG_UPDATE:
+      cg->iters += 1;
+      cg->rr = tot[0];
+      if (!isfinite(cg->alpha) || !isfinite(tot[0]) || !isfinite(tot[1])) {
+        cg->done = 3;
+      } else if (!(tot[0] > cg->delta2 * cg->rmax2) || !(tot[0] > cg->tol2) || cg->iters >= cg->maxit) {
+        cg->need_reliable = 1;
+      } else {
+        cg->beta = tot[1] / cg->rz;
+        cg->rz = tot[1];
+      }
+      break;
+    case FIN_MG_START:
+      if (!isfinite(tot[1])) {
+        cg->done = 3;
+      } else {
+        cg->rz = tot[1];
+        cg->beta = 0.0;
       }
       break;
     case FIN_PCG64: {
@@ -1059,6 +960,11 @@ void CgWork::release() {
   DevBuf* bs[] = {&res, &wmx, &prj, &x0, &r0, &e64, &y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32, &dinv64,
                   &cg};
   for (DevBuf* b : bs) dev_free(*b);
+  if (mg) {
+    mg->release();
+    delete mg;
+    mg = nullptr;
+  }
   if (host) (void)hipHostFree(host);
   host = nullptr;
   d = Dims{0, 0, 0};
@@ -1120,7 +1026,8 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   h.tw = a.tw;
   h.maxit = a.maxit;
   h.stop_rule = mode == ELSIM_MODE_FP64 ? a.stop_rule : ELSIM_STOP_RELRES;
-  h.delta2 = a.delta * a.delta;
+  const double delta = mode == ELSIM_MODE_MIXED_MG ? mg_cfg().delta : a.delta;
+  h.delta2 = delta * delta;
   h.rel_r0 = a.rel_r0;
 
   ELSIM_HIP(hipMemcpyAsync(cg, &h, sizeof(CgDev), hipMemcpyHostToDevice, s));
@@ -1193,6 +1100,81 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     k_diff<<<gc, blk, 0, s>>>(d, a.x, x0, e64);
     k_true_resid<OP, true><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, nullptr, nullptr, cg, P);
     fin2(FIN_TRUE_FINAL, nb64);
+  } else if (mode == ELSIM_MODE_MIXED_MG) {
+    // FP32 CG preconditioned by one multigrid V-cycle, FP64 reliable updates
+    // as in the Jacobi-preconditioned mixed path below.  Per iteration: the
+    // operator (p.Ap) -> alpha; r -= alpha w; the V-cycle z = M^-1 r (r.r
+    // and r.z partials) -> beta or a reliable-update request; y += alpha p,
+    // p = z + beta p.
+    if (!w.mg) w.mg = new MgWork();
+    uint16_t* dinv = (uint16_t*)w.dinv32.p;  // BF16
+    float *r32 = (float*)w.r32.p, *w32 = (float*)w.w32.p, *y32 = (float*)w.y32.p, *p32 = (float*)w.p32.p;
+    const float* k32 = (const float*)w.kap32.p;
+    const float* s32 = (const float*)w.pse32.p;
+    k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.kap, (float*)w.kap32.p);
+    if (OP == ELSIM_OP_CNE) k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.pse, (float*)w.pse32.p);
+    k_dinv<OP, uint16_t><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv);
+    ELSIM_TRY(mg_setup(*w.mg, OP, d, a.kap, a.pse, a.scale));
+    const float sc32 = (float)a.scale;
+    const int nbv = mg_partial_blocks(d);
+    const bool mv4 = mv4_ok(d), mv4_edge = mv4_tiles(d).gx > 1;
+    const int nbm = mv4 ? mv4_ntiles(d) : nb32;
+    const int fbd = (int)((G2 + (int64_t)FLAT_T * DIR_P - 1) / ((int64_t)FLAT_T * DIR_P));
+    auto precondition = [&]() { return mg_vcycle(*w.mg, OP, d, k32, s32, dinv, sc32, r32, w32, cg, P); };
+    k_mg_start<<<fbu, FLAT_T, 0, s>>>(G2, (const double2*)r0, (float2*)r32, (float2*)p32, (float2*)y32,
+                                      (double2*)e64);
+    ELSIM_TRY(precondition());
+    fin2(FIN_MG_START, nbv);
+    k_mg_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)w32, cg);
+    ELSIM_CHECK_LAUNCH();
+    int it_prev = 0;
+    for (;;) {
+      const bool pr = g_prof.on;
+      for (int b = 0; b < batch; b++) {
+        const bool t = pr && b == 0;
+        ELSIM_TRY(timed(t, ELSIM_PROF_OP32, s, [&] {
+          if (mv4 && mv4_edge)
+            k_matvec4<OP, true, true><<<nbm, 256, 0, s>>>(d, p32, k32, s32, sc32, w32, cg, P);
+          else if (mv4)
+            k_matvec4<OP, true, false><<<nbm, 256, 0, s>>>(d, p32, k32, s32, sc32, w32, cg, P);
+          else
+            k_matvec<float, OP, true, true><<<nb32, blk, 0, s>>>(d, p32, k32, s32, sc32, w32, nullptr, cg, 1, P);
+        }));
+        fin1(FIN_ALPHA_RZ, nbm);
+        ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] {
+          k_mg_upd<<<fbu, FLAT_T, 0, s>>>(G2, (float2*)r32, (const float2*)w32, cg);
+        }));
+        int prc = ELSIM_OK;
+        ELSIM_TRY(timed(t, ELSIM_PROF_MG, s, [&] { prc = precondition(); }));
+        ELSIM_TRY(prc);
+        fin2(FIN_MG_UPDATE, nbv);
+        ELSIM_TRY(timed(t, ELSIM_PROF_DIR32, s, [&] {
+          k_mg_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)w32, cg);
+        }));
+      }
+      ELSIM_CHECK_LAUNCH();
+      ELSIM_TRY(poll(w, s));
+      if (pr && w.host->iters >= it_prev + 2) {
+        ELSIM_TRY(prof_collect(ELSIM_PROF_OP32, ELSIM_PROF_DIR32));
+        ELSIM_TRY(prof_collect(ELSIM_PROF_MG, ELSIM_PROF_MG));
+      }
+      it_prev = w.host->iters;
+      if (w.host->done) break;
+      if (w.host->need_reliable) {
+        // fold y into the FP64 correction, FP64 true residual, restart from
+        // it with z = M^-1 r
+        k_mx_fold<<<fb, FLAT_T, 0, s>>>(G, e64, y32, cg);
+        k_true_resid<OP, false><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, r32, dinv, cg, P);
+        fin2(FIN_TRUE_RELIABLE, nb64);
+        ELSIM_TRY(precondition());
+        fin2(FIN_MG_START, nbv);
+        k_mg_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)w32, cg);
+        ELSIM_CHECK_LAUNCH();
+        ELSIM_TRY(poll(w, s));
+        if (w.host->done) break;
+      }
+    }
+    k_add_e<<<fb, FLAT_T, 0, s>>>(G, a.x, e64);
   } else {
     uint16_t* dinv = (uint16_t*)w.dinv32.p;  // BF16
     float *r32 = (float*)w.r32.p, *w32 = (float*)w.w32.p, *y32 = (float*)w.y32.p, *p32 = (float*)w.p32.p;
@@ -1354,6 +1336,13 @@ int cg_solve(CgWork& w, const SolveArgs& a, int mode, SolveResult* res) {
     set_error("the preconditioned solvers use the relative-residual stop rule only");
     return ELSIM_ERR_ARG;
   }
+  // CnE's reaction term keeps it well conditioned: Jacobi-PCG needs a few
+  // iterations there, fewer than a V-cycle costs
+  if (mode == ELSIM_MODE_MIXED_MG && a.op == ELSIM_OP_CNE) mode = ELSIM_MODE_MIXED;
+  if (mode == ELSIM_MODE_MIXED_MG && w.bound) {
+    set_error("the multigrid solver needs library-allocated work space");
+    return ELSIM_ERR_ARG;
+  }
   switch (a.op) {
     case ELSIM_OP_CNE: return solve_op<ELSIM_OP_CNE>(w, a, mode, res);
     case ELSIM_OP_PHP: return solve_op<ELSIM_OP_PHP>(w, a, mode, res);
@@ -1460,6 +1449,46 @@ int elsim_dev_matvec_f32(int op, int n1, int n2, int n3, const float* p, const f
                          float scale, float* out, double* dot) {
   if (!dims_ok(n1, n2, n3) || !p || !kap || !out || (op == ELSIM_OP_CNE && !pse)) return ELSIM_ERR_ARG;
   return launch_matvec_f32(op, Dims{n1, n2, n3}, p, kap, pse, scale, out, dot, true);
+}
+
+int elsim_dev_mg_apply(int op, int n1, int n2, int n3, const double* kap, const double* pse, double scale,
+                       const float* r, float* z) {
+  if (!dims_ok(n1, n2, n3) || !kap || !r || !z || (op == ELSIM_OP_CNE && !pse) || op < ELSIM_OP_CNE ||
+      op > ELSIM_OP_POT_DD) {
+    set_error("elsim_dev_mg_apply: bad arguments");
+    return ELSIM_ERR_ARG;
+  }
+  ELSIM_TRY(ensure_init());
+  static MgWork mg;
+  static DevBuf k32, s32, dv;
+  const Dims d{n1, n2, n3};
+  if (stencil_ntiles<float>(d) > kMaxRedBlocks) {
+    set_error("grid too large for the reduction workspace");
+    return ELSIM_ERR_ARG;
+  }
+  const size_t Ge = ((size_t)d.G() + 1) & ~(size_t)1;
+  ELSIM_TRY(dev_alloc(k32, Ge * 4));
+  ELSIM_TRY(dev_alloc(s32, Ge * 4));
+  ELSIM_TRY(dev_alloc(dv, Ge * 2));
+  hipStream_t s = stream();
+  ELSIM_HIP(hipMemsetAsync(dv.p, 0, Ge * 2, s));
+  ELSIM_HIP(hipMemsetAsync(z, 0, (size_t)d.G() * 4, s));
+  const int fb = flat_blocks(d.G());
+  k_to_f32<<<fb, FLAT_T, 0, s>>>(d.G(), kap, (float*)k32.p);
+  if (op == ELSIM_OP_CNE) k_to_f32<<<fb, FLAT_T, 0, s>>>(d.G(), pse, (float*)s32.p);
+  const dim3 gc = col_grid(d), blk(TX, TY, 1);
+  switch (op) {
+    case ELSIM_OP_CNE: k_dinv<ELSIM_OP_CNE, uint16_t><<<gc, blk, 0, s>>>(d, kap, pse, scale, (uint16_t*)dv.p); break;
+    case ELSIM_OP_PHP: k_dinv<ELSIM_OP_PHP, uint16_t><<<gc, blk, 0, s>>>(d, kap, pse, scale, (uint16_t*)dv.p); break;
+    case ELSIM_OP_PHE: k_dinv<ELSIM_OP_PHE, uint16_t><<<gc, blk, 0, s>>>(d, kap, pse, scale, (uint16_t*)dv.p); break;
+    default: k_dinv<ELSIM_OP_POT_DD, uint16_t><<<gc, blk, 0, s>>>(d, kap, pse, scale, (uint16_t*)dv.p); break;
+  }
+  ELSIM_CHECK_LAUNCH();
+  ELSIM_TRY(mg_setup(mg, op, d, kap, pse, scale));
+  ELSIM_TRY(mg_vcycle(mg, op, d, (const float*)k32.p, (const float*)s32.p, (const uint16_t*)dv.p, (float)scale, r, z,
+                      nullptr, workspace().partials));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  return ELSIM_OK;
 }
 
 int elsim_prof_enable(int on) {

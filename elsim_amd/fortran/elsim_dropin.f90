@@ -141,7 +141,7 @@ SUBROUTINE SBM_Init(ny, nx, nz, prm, DmyS, DmyL, sz, Tbls, handle)
    CALL elsim_check(elsim_sbm_create(ny, nx, nz, prm, DmyS, DmyL, sz, Tbls, handle), 'SBM_Init')
 END SUBROUTINE SBM_Init
 
-!! One electrochemical step SBM:397-840; mode = ELSIM_MODE_FP64 / _MIXED.
+!! One electrochemical step SBM:397-840; mode = ELSIM_MODE_FP64 / _MIXED / _FP64_PCG / _MIXED_MG.
 SUBROUTINE SBM_Step(handle, mode, stats)
    USE elsim_gpu
    IMPLICIT NONE

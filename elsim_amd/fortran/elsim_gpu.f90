@@ -7,7 +7,8 @@ MODULE elsim_gpu
 
    INTEGER(c_int), PARAMETER :: ELSIM_OK = 0, ELSIM_NOT_CONVERGED = 1
    INTEGER(c_int), PARAMETER :: ELSIM_STOP_REFERENCE = 0, ELSIM_STOP_RELRES = 1
-   INTEGER(c_int), PARAMETER :: ELSIM_MODE_FP64 = 0, ELSIM_MODE_MIXED = 1
+   INTEGER(c_int), PARAMETER :: ELSIM_MODE_FP64 = 0, ELSIM_MODE_MIXED = 1, ELSIM_MODE_FP64_PCG = 2, &
+      ELSIM_MODE_MIXED_MG = 3
    INTEGER(c_int), PARAMETER :: ELSIM_F_PSP = 0, ELSIM_F_PSE = 1, ELSIM_F_AVP = 2, ELSIM_F_CNP = 3, &
       ELSIM_F_CNE = 4, ELSIM_F_PHP = 5, ELSIM_F_PHE = 6, ELSIM_F_RXN = 7, ELSIM_F_KAPP = 8
 

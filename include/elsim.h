@@ -54,6 +54,8 @@ extern "C" {
 #define ELSIM_MODE_FP64      0 /* reference CG (unpreconditioned) in FP64 */
 #define ELSIM_MODE_MIXED     1 /* FP32 Jacobi-PCG inner iterations, FP64 reliable updates */
 #define ELSIM_MODE_FP64_PCG  2 /* Jacobi-PCG in FP64 (same preconditioner as MIXED, all FP64) */
+#define ELSIM_MODE_MIXED_MG  3 /* FP32 CG preconditioned by a geometric-multigrid V-cycle, FP64 reliable updates
+                                  (the potential solves; CnE runs as ELSIM_MODE_MIXED) */
 
 /* Which linear system (operator + homogeneous ghost rule of the search
  * direction) a CG solve works on. */
@@ -353,13 +355,23 @@ int elsim_dev_matvec_f64(int op, int n1, int n2, int n3, const double* p, const 
                          const double* pse, double scale, double* out, double* dot);
 int elsim_dev_matvec_f32(int op, int n1, int n2, int n3, const float* p, const float* kap,
                          const float* pse, float scale, float* out, double* dot);
+/* One V-cycle of the ELSIM_MODE_MIXED_MG preconditioner, z = M^-1 r, on
+ * device arrays: kap (and pse for ELSIM_OP_CNE) FP64 ghosted, scale the
+ * operator's dt/dx^2 or 1/dx^2; r and z FP32 ghosted (r's ghosts zero, z's are
+ * written zero).  The hierarchy is built on every call.  No counterpart in
+ * the reference (its CG is unpreconditioned, SBM:657-714, 737-795): it
+ * exposes the preconditioner to tests. */
+int elsim_dev_mg_apply(int op, int n1, int n2, int n3, const double* kap, const double* pse, double scale,
+                       const float* r, float* z);
 /* Kernel timing inside the CG loops.  When enabled, the first iteration of
  * every batch of CG iterations is bracketed by HIP events on the library
- * stream (operator, update, direction kernels); read-out averages them over
+ * stream (operator, update, direction kernels, V-cycle); read-out averages them over
  * the batches whose iteration ran, then resets.  Slots: ELSIM_PROF_*. */
 enum {
   ELSIM_PROF_OP32 = 0, ELSIM_PROF_UPD32 = 1, ELSIM_PROF_DIR32 = 2,
-  ELSIM_PROF_OP64 = 3, ELSIM_PROF_UPD64 = 4, ELSIM_PROF_DIR64 = 5, ELSIM_PROF_N = 6
+  ELSIM_PROF_OP64 = 3, ELSIM_PROF_UPD64 = 4, ELSIM_PROF_DIR64 = 5,
+  ELSIM_PROF_MG = 6, /* the V-cycle of ELSIM_MODE_MIXED_MG, all its kernels */
+  ELSIM_PROF_N = 7
 };
 typedef struct elsim_kernel_prof {
   int32_t count[ELSIM_PROF_N];

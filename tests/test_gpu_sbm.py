@@ -53,7 +53,7 @@ def test_ch_update_bit_exact(gpu, oracle_lib):
     assert np.array_equal(g.get(F_CNP), o.get(F_CNP))
 
 
-@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_MIXED, E.MODE_FP64_PCG])
+@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_MIXED, E.MODE_FP64_PCG, E.MODE_MIXED_MG])
 def test_step_matches_oracle(gpu, oracle_lib, mode):
     n = 24
     sdf, sdl, prm = case(n, seed=2)
@@ -81,7 +81,7 @@ def test_step_matches_oracle(gpu, oracle_lib, mode):
     assert g.scalars()["bve"] == pytest.approx(o.scalars()["bve"], rel=1e-12)
 
 
-@pytest.mark.parametrize("mode", [E.MODE_MIXED, E.MODE_FP64_PCG])
+@pytest.mark.parametrize("mode", [E.MODE_MIXED, E.MODE_FP64_PCG, E.MODE_MIXED_MG])
 def test_step_matches_oracle_odd_box(gpu, oracle_lib, mode):
     """A non-cubic box whose ghosted size (23 * 19 * 15) is odd, with partial
     stencil tiles in every direction: the FP32 vector kernels work on element

@@ -41,7 +41,7 @@ def setup(oracle_lib):
     return sdf, prm, tab, o
 
 
-@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_MIXED])
+@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_MIXED, E.MODE_MIXED_MG])
 def test_reference_constants_step_matches_oracle(gpu, oracle_lib, mode):
     sdf, prm, tab, o = setup(oracle_lib)
     g = E.Sbm(sdf, -sdf, prm, tab)

@@ -41,7 +41,7 @@ def c1_oracle(c1, oracle_lib):
     return x, it
 
 
-@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_FP64_PCG, E.MODE_MIXED])
+@pytest.mark.parametrize("mode", [E.MODE_FP64, E.MODE_FP64_PCG, E.MODE_MIXED, E.MODE_MIXED_MG])
 def test_c1_64_matches_oracle(gpu, c1, c1_oracle, mode):
     kap, rhs, phi, info = c1
     ref, it_ref = c1_oracle
@@ -89,7 +89,7 @@ def test_single_dirichlet_face_matches_oracle(gpu, oracle_lib, bc):
     kap, rhs, phi, _ = E.potential_problem(24, 3, rmin=3.0, rmax=6.0)
     ref, st, _, _ = oracle_lib.solve_pot(kap, rhs, phi, *bc)
     assert st == 0
-    for mode in (E.MODE_FP64, E.MODE_MIXED):
+    for mode in (E.MODE_FP64, E.MODE_MIXED, E.MODE_MIXED_MG):
         x, rc, _, rr = E.solve_pot(kap, rhs, phi, *bc, mode=mode)
         assert rc == 0 and rr <= 1e-8
         assert rel(x[1:-1, 1:-1, 1:-1], ref[1:-1, 1:-1, 1:-1]) <= 1e-6
@@ -97,7 +97,7 @@ def test_single_dirichlet_face_matches_oracle(gpu, oracle_lib, bc):
 
 def test_iteration_cap_reports_not_converged(gpu, c1, tmp_path):
     kap, rhs, phi, _ = c1
-    for mode in (E.MODE_FP64, E.MODE_FP64_PCG, E.MODE_MIXED):
+    for mode in (E.MODE_FP64, E.MODE_FP64_PCG, E.MODE_MIXED, E.MODE_MIXED_MG):
         _, rc, it, rr = E.solve_pot(kap, rhs, phi, mode=mode, maxit=1)
         assert rc == E._lib.NOT_CONVERGED, (mode, rc)
         assert it <= 2 and rr > 1e-8
@@ -116,7 +116,7 @@ def test_nonfinite_rhs_is_an_error(gpu):
     kap, rhs, phi, _ = E.potential_problem(24, 3, rmin=3.0, rmax=6.0)
     rhs = rhs.copy()
     rhs[3, 4, 5] = np.nan
-    for mode in (E.MODE_FP64, E.MODE_FP64_PCG, E.MODE_MIXED):
+    for mode in (E.MODE_FP64, E.MODE_FP64_PCG, E.MODE_MIXED, E.MODE_MIXED_MG):
         with pytest.raises(E.ElsimError, match=r"\(-3\)"):
             E.solve_pot(kap, rhs, phi, mode=mode)
     # the library stays usable after the error
