@@ -52,6 +52,12 @@ PMC_BYTES_PER_GHOSTED = {"upd32": (83892.2 * 2 + 67354.1) * 1024 / 258 ** 3,
                          "dir32": (117411.2 * 2 + 134168.1) * 1024 / 258 ** 3}
 # the phP operator covers the interior: bytes per interior voxel, same runs
 PMC_BYTES_PER_VOXEL_OP32 = (94073.9 * 2 + 66625.1) * 1024 / 256 ** 3
+# the multigrid path's kernels, same method (profiles/mg_kernels_256_pmc.md):
+# flat kernels over the ghosted box, the fine-level sweeps over the interior
+PMC_BYTES_PER_GHOSTED.update({"upd_mg": (100642.7 * 2 + 134168.1) * 1024 / 258 ** 3,
+                              "dir_mg": (100640.7 * 2 + 134168.1) * 1024 / 258 ** 3})
+PMC_BYTES_PER_VOXEL_MG = {"mg_up": (182570.4 * 2 + 67035.2) * 1024 / 256 ** 3,
+                          "mg_down": (126173.2 * 2 + 8208.0) * 1024 / 256 ** 3}
 
 
 def log(*a):
@@ -102,10 +108,22 @@ def matvec_roofline(n, reps=20):
 # direction: r, p, dinv in; p out.  The mixed solver moves the solution
 # update into the direction kernel: update r, w, dinv in, r out; direction y,
 # r, p, dinv in, y, p out.  Its dinv is BF16 (2 B), the FP64 solver's FP64.
-KERNEL_BYTES = {"op32": 12, "upd32": 14, "dir32": 22, "op64": 24, "upd64": 56, "dir64": 32}
+# The multigrid-preconditioned solver's update reads r, w, dinv and writes r
+# and the pre-smoothed iterate x1 (20 B); its direction kernel reads y, p, z
+# and writes y, p (20 B).  The V-cycle's fine-level down-sweep reads x1, kappa,
+# r and writes the level-1 right-hand side (12 + 0.5 B per voxel); the
+# up-sweep reads x1, kappa, r, dinv and the level-1 iterate and writes z
+# (20 + 0.5 B); the coarse levels move about 1/7 of the fine bytes again
+# (not counted in "vcycle").
+KERNEL_BYTES = {"op32": 12, "upd32": 14, "dir32": 22, "op64": 24, "upd64": 56, "dir64": 32, "upd_mg": 20,
+                "dir_mg": 20, "vcycle": 33, "mg_down": 12.5, "mg_up": 20.5}
 KERNEL_NAMES = {"op32": "k_matvec4 (7-point operator, 4 voxels a lane, + p.Ap)", "upd32": "k_mx_update (r -= a w, r.z)",
                 "dir32": "k_mx_dir (y += a p, p = z + b p)", "op64": "k_matvec<double>", "upd64": "k_pcg64_update",
-                "dir64": "k_pcg64_dir"}
+                "dir64": "k_pcg64_dir", "upd_mg": "k_mg_update (r -= a w, x1 = omega dinv r)",
+                "dir_mg": "k_mg_dir3 (y += a p, p = z + b p)",
+                "vcycle": "V-cycle (fine-level bytes; all levels' kernels)",
+                "mg_down": "k_mg_down4 (x1 -> sigma (r - A x1), restricted in registers)",
+                "mg_up": "k_mg_up4 (x2 = x1 + oc P x_c, z = x2 + omega dinv (r - A x2))"}
 
 
 def profiled_step(E, s, mode):
@@ -120,8 +138,16 @@ def profiled_step(E, s, mode):
     s.step(mode)
     E.check(L.elsim_prof_read(C.byref(prof)))
     E.check(L.elsim_prof_enable(0))
-    keys = ["op32", "upd32", "dir32", "op64", "upd64", "dir64"]
+    keys = ["op32", "upd32", "dir32", "op64", "upd64", "dir64", "vcycle", "mg_down", "mg_up"]
+    if mode == E.MODE_MIXED_MG:
+        keys[1], keys[2] = "upd_mg", "dir_mg"
     return {k: {"us": prof.mean_us(i), "launches": int(prof.count[i])} for i, k in enumerate(keys) if prof.count[i]}
+
+
+def mg_settings(E):
+    v = [C.c_double() for _ in range(3)]
+    E.check(E.lib().elsim_mg_config_get(*[C.byref(x) for x in v]))
+    return [x.value for x in v]
 
 
 def nn_forward_time(E, net, n, reps=5):
@@ -243,10 +269,13 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fast,fast_extrap,fast_plain,fp64,ref",
-                    help="comma list of fast, fast_extrap, fast_plain, fp64, fp64_same_start, ref")
+    ap.add_argument("--paths", default="fast,fast_extrap,fast_plain,jacobi,jacobi_extrap,fp64,ref",
+                    help="comma list of fast, fast_extrap, fast_plain (multigrid-preconditioned mixed solver), "
+                         "jacobi, jacobi_extrap, jacobi_plain (Jacobi-preconditioned mixed solver), fp64, "
+                         "fp64_same_start, ref")
     ap.add_argument("--weights", default=None, help="warm-start network (default: the packaged one)")
     ap.add_argument("--inner-rtol", type=float, default=None)
+    ap.add_argument("--mg", default=None, help="omega,oc,delta of the multigrid path (default: the library's)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the operator timing and a 512 MiB device copy (for counter runs)")
     ap.add_argument("--cg-cap", type=int, default=0,
@@ -260,6 +289,8 @@ def main():
     import elsim_amd as E
     L = E.lib()
     E.check(L.elsim_init(0), "elsim_init")
+    if a.mg:
+        E.check(L.elsim_mg_config(*[float(v) for v in a.mg.split(",")]), "elsim_mg_config")
     n = a.n
     if a.roofline_only:
         t_mv, gbs = matvec_roofline(n, reps=10)
@@ -291,17 +322,32 @@ def main():
     from elsim_amd import nn as NN
     net = NN.WarmStartNet.load(a.weights or NN.DEFAULT_WEIGHTS)
     variants = {}
+    # The multigrid paths extrapolate phi_P with the second-difference
+    # Galerkin step but not the third: over three 256^3 packs x 48 steps the
+    # third saved no iterations and cost ~0.45 ms a step (12.6 against 13.0
+    # ms/step; linear extrapolation alone 15.4 ms/step).
+    prm_mg = prm.copy(ws_order=2)
     if "fast" in paths:
-        log("fast path (mixed precision + learned warm start)")
-        tfast, stf, kf, ffast = run_path(E, sdf, prm.copy(ws_mode=E.WS_LEARNED), tab, E.MODE_MIXED, a.warmup,
+        log("fast path (mixed precision, multigrid-preconditioned CG, learned warm start)")
+        tfast, stf, kf, ffast = run_path(E, sdf, prm_mg.copy(ws_mode=E.WS_LEARNED), tab, E.MODE_MIXED_MG, a.warmup,
                                          a.steps, cnp0, net=net, fields=True)
         variants["learned"] = (tfast, stf)
     for key, wm, label in (("fast_extrap", E.WS_EXTRAP, "extrapolated"), ("fast_plain", E.WS_PLAIN, "plain")):
         if key in paths:
-            log(f"mixed precision, {label} warm start")
-            t, st, _, _ = run_path(E, sdf, prm.copy(ws_mode=wm), tab, E.MODE_MIXED, a.warmup, a.steps, cnp0,
+            log(f"mixed precision, multigrid, {label} warm start")
+            t, st, _, _ = run_path(E, sdf, prm_mg.copy(ws_mode=wm), tab, E.MODE_MIXED_MG, a.warmup, a.steps, cnp0,
                                    profile=False)
             variants[label] = (t, st)
+    jacobi = {}
+    kj = {}
+    for key, wm, label in (("jacobi", E.WS_LEARNED, "learned"), ("jacobi_extrap", E.WS_EXTRAP, "extrapolated"),
+                           ("jacobi_plain", E.WS_PLAIN, "plain")):
+        if key in paths:
+            log(f"mixed precision, Jacobi-preconditioned CG, {label} warm start")
+            t, st, k, _ = run_path(E, sdf, prm.copy(ws_mode=wm), tab, E.MODE_MIXED, a.warmup, a.steps, cnp0,
+                                   profile=key == "jacobi", net=net if wm == E.WS_LEARNED else None)
+            jacobi[label] = (t, st)
+            kj.update(k)
     if "fp64" in paths:
         log("fp64 path (Jacobi-PCG, all FP64, plain warm start)")
         t64, st64, k64, f64 = run_path(E, sdf, prm.copy(ws_mode=E.WS_PLAIN), tab, E.MODE_FP64_PCG, a.warmup,
@@ -309,8 +355,8 @@ def main():
     f64s = None
     if "fp64_same_start" in paths:
         log("fp64 path with the fast path's warm start (for the precision-only field error)")
-        _, _, _, f64s = run_path(E, sdf, prm.copy(ws_mode=E.WS_LEARNED), tab, E.MODE_FP64_PCG, a.warmup, a.steps,
-                                 cnp0, profile=False, net=net, fields=True)
+        _, _, _, f64s = run_path(E, sdf, prm_mg.copy(ws_mode=E.WS_LEARNED), tab, E.MODE_FP64_PCG, a.warmup,
+                                 a.steps, cnp0, profile=False, net=net, fields=True)
     t_nn = nn_forward_time(E, net, n) if n % 8 == 0 else nan
     log(f"warm-start network forward at {n}^3: {t_nn * 1e3:.2f} ms")
     if "ref" in paths:
@@ -324,21 +370,28 @@ def main():
     for k, v in list(kf.items()) + list(k64.items()):
         kernels[k] = {"kernel": KERNEL_NAMES[k], "us": v["us"], "bytes_per_voxel": KERNEL_BYTES[k],
                       "GBps": KERNEL_BYTES[k] * N / (v["us"] * 1e-6) / 1e9, "launches_timed": v["launches"]}
+    kernels_jacobi = {k: {"kernel": KERNEL_NAMES[k], "us": v["us"], "bytes_per_voxel": KERNEL_BYTES[k],
+                          "GBps": KERNEL_BYTES[k] * N / (v["us"] * 1e-6) / 1e9, "launches_timed": v["launches"]}
+                      for k, v in kj.items()}
     # dominant kernel of the fast path: largest share of the CG iteration.
     # The operator and the direction kernel take the same time to within a
     # few tenths of a microsecond, so a tie within 2 % goes to the operator,
     # the one further from its roofline (the conservative figure).
-    dom = max(kf, key=lambda k: kf[k]["us"]) if kf else None
+    single = {k: v for k, v in kf.items() if k != "vcycle"}
+    dom = max(single, key=lambda k: single[k]["us"]) if single else None
     if dom and "op32" in kf and kf["op32"]["us"] >= 0.98 * kf[dom]["us"]:
         dom = "op32"
     if dom:
         if dom == "op32":
             traffic = PMC_BYTES_PER_VOXEL_OP32 * N
+        elif dom in PMC_BYTES_PER_VOXEL_MG:
+            traffic = PMC_BYTES_PER_VOXEL_MG[dom] * N
         else:
             traffic = PMC_BYTES_PER_GHOSTED[dom] * (n + 2) ** 3 if dom in PMC_BYTES_PER_GHOSTED else None
         roof = {"bound": "hbm", "achieved": kernels[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": kernels[dom]["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
-                "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per launch, profiles/cg_kernels_256_pmc.md",
+                "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per launch, profiles/mg_kernels_256_pmc.md "
+                                  "(multigrid kernels) and profiles/cg_kernels_256_pmc.md",
                 "kernel": kernels[dom]["kernel"],
                 "kernel_us": kernels[dom]["us"], "bytes_per_launch": KERNEL_BYTES[dom] * N,
                 "stencil": {"kernel": kernels["op32"]["kernel"], "us": kernels["op32"]["us"],
@@ -382,7 +435,10 @@ def main():
                    "kappa": "psi_P", "D": "exp(-0.2 c)", "rtol": prm.rtol,
                    "cne_rule": "relative to the step's change (||r0||)", "cne_iters_per_step": mean(stf, "it_cne"),
                    "coupling_iters_per_step": {"fast": mean(stf, "n_outer"), "fp64": mean(st64, "n_outer")},
-                   "fast_path": "fp32 jacobi-pcg + fp64 reliable updates (restarts) + learned warm start",
+                   "fast_path": "fp32 cg preconditioned by a multigrid v-cycle + fp64 reliable updates + learned "
+                                "warm start (CnE: fp32 jacobi-pcg)",
+                   "multigrid": dict(zip(("omega", "oc", "delta"), mg_settings(E))),
+                   "ws_order_fast": 2,
                    "learned_warm_start": True},
         "field_rel_l2_vs_fp64": field_err,
         "field_rel_l2_vs_fp64_same_start": field_err_same,
@@ -399,12 +455,19 @@ def main():
                 "cg_iters_per_step": float(np.mean([x["it_cne"] + x["it_php"] + x["it_phe"] for x in v[1]]))
                 if v[1] else nan}
             for k, v in variants.items()},
+        "jacobi_preconditioned": {
+            k: {"s_per_step": v[0],
+                "php_iters_per_step": float(np.mean([x["it_php"] for x in v[1]])) if v[1] else nan,
+                "cg_iters_per_step": float(np.mean([x["it_cne"] + x["it_php"] + x["it_phe"] for x in v[1]]))
+                if v[1] else nan}
+            for k, v in jacobi.items()},
         "nn_forward_ms": t_nn * 1e3,
         "nn_weights": os.path.relpath(a.weights or NN.DEFAULT_WEIGHTS, ROOT),
         "cg_iters_per_step": {"fp64": iters64, "fast": itersf},
         "voxel_updates_per_s": {"fp64": N * iters64 / t64, "fast": N * itersf / tfast},
         "roofline": roof,
         "kernels_in_loop": kernels,
+        "kernels_in_loop_jacobi": kernels_jacobi,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

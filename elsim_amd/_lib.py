@@ -30,7 +30,7 @@ _PARAM_DOUBLES = ("dx dt zeta psp_off pse_off avp_min eps rho ch_psi_min frd cst
                   "dab_c0 dab_c1 dab_c2 dab_c3 kap_scale i0_scale kfw_cref bvp bve vsr trgi cne_target "
                   "rtol crt_e crt_p crt_l outer_tol inner_rtol").split()
 _PARAM_INTS = ("mass_corr_period skip_cne_first stop_rule max_it_e max_it_p max_it_l max_outer ws_mode cne_rule "
-               "reserved_").split()
+               "ws_order").split()
 
 
 class SbmParams(C.Structure):
@@ -59,8 +59,8 @@ class StepStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
-PROF_OP32, PROF_UPD32, PROF_DIR32, PROF_OP64, PROF_UPD64, PROF_DIR64, PROF_MG = range(7)
-PROF_N = 7
+PROF_OP32, PROF_UPD32, PROF_DIR32, PROF_OP64, PROF_UPD64, PROF_DIR64, PROF_MG, PROF_MG_DOWN, PROF_MG_UP = range(9)
+PROF_N = 9
 
 
 class KernelProf(C.Structure):
@@ -145,6 +145,8 @@ _SIGS = {
     "elsim_prof_read": (I, [C.POINTER(KernelProf)]),
     "elsim_dev_matvec_f32": (I, [I, I, I, I, P, P, P, C.c_float, P, P]),
     "elsim_dev_mg_apply": (I, [I, I, I, I, P, P, D, P, P]),
+    "elsim_mg_config": (I, [D, D, D]),
+    "elsim_mg_config_get": (I, [DP, DP, DP]),
     "elsim_gen_sphere_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, I, D, P, IP, DP]),
     "elsim_gen_ellipsoid_pack": (I, [I, I, I, C.c_uint64, D, D, D, D, D, D, P, IP, DP]),
     "elsim_gen_tunnel": (I, [I, I, I, C.c_uint64, D, P]),

@@ -63,6 +63,45 @@ struct Coarse {
   View v;
 };
 
+// Stage one slab of the input tile plus its one-voxel halo (HN voxels, nq
+// 16-byte chunks each, nq <= NQ) into LDS rows of rs elements, zero outside
+// the box.  Each thread issues all of its global loads before its first LDS
+// store, so they are in flight together (one load latency per slab instead
+// of one per chunk).
+template <int HN, int NQ>
+__device__ __forceinline__ void stage_slab(int nq, int rs, int x0, int y0, int z0, int n1, int n2, int n3,
+                                           const uint16_t* __restrict__ in, View iv, Coarse lo, int slab,
+                                           uint16_t* __restrict__ lds) {
+  constexpr int HX = CONV_TX + 2, HY = CONV_TY + 2;
+  constexpr int NU = (HN * NQ + 255) / 256;
+  const int tid = threadIdx.x, tot = HN * nq;
+  uint4 v[NU];
+#pragma unroll
+  for (int u = 0; u < NU; u++) {
+    const int it = tid + 256 * u;
+    v[u] = make_uint4(0, 0, 0, 0);
+    if (it < tot) {
+      const int h = it / nq, q = it % nq;
+      const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+      if (gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3) {
+        if (slab < lo.ch) {
+          const int64_t cv = (gx >> 1) + (int64_t)(n1 >> 1) * ((gy >> 1) + (int64_t)(n2 >> 1) * (gz >> 1));
+          v[u] = *(const uint4*)(lo.p + cv * lo.v.cs + lo.v.co + slab + 8 * q);
+        } else {
+          const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
+          v[u] = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NU; u++) {
+    const int it = tid + 256 * u;
+    if (it < tot) *(uint4*)(lds + (it / nq) * rs + 8 * (it % nq)) = v[u];
+  }
+}
+
 // MODE 0: out bf16 = relu?(acc + bias); MODE 1: out FP32 = acc.
 template <int COUT, int MT, int MODE>
 __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
@@ -88,23 +127,7 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
 
   for (int slab = 0; slab < cin; slab += cs) {
     __syncthreads();
-    const int nq = cs / 8;  // 16-byte chunks per staged voxel
-    for (int it = tid; it < HN * nq; it += 256) {
-      const int h = it / nq, q = it % nq;
-      const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
-      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3) {
-        if (slab < lo.ch) {
-          const int64_t cv = (gx >> 1) + (int64_t)(n1 >> 1) * ((gy >> 1) + (int64_t)(n2 >> 1) * (gz >> 1));
-          v = *(const uint4*)(lo.p + cv * lo.v.cs + lo.v.co + slab + 8 * q);
-        } else {
-          const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
-          v = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
-        }
-      }
-      *(uint4*)(lds + h * rs + 8 * q) = v;
-    }
+    stage_slab<HN, CS_MAX / 8>(cs / 8, rs, x0, y0, z0, n1, n2, n3, in, iv, lo, slab, lds);
     __syncthreads();
     if (cs == 32) {
       // one 16x16x32 MFMA per output tile and tap: lane group lq holds
@@ -215,22 +238,7 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
                              : make_uint4(0, 0, 0, 0);
     }
     __syncthreads();
-    for (int it = tid; it < HN * 2; it += 256) {
-      const int h = it >> 1, q = it & 1;
-      const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
-      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3) {
-        if (slab < lo.ch) {
-          const int64_t cv = (gx >> 1) + (int64_t)(n1 >> 1) * ((gy >> 1) + (int64_t)(n2 >> 1) * (gz >> 1));
-          v = *(const uint4*)(lo.p + cv * lo.v.cs + lo.v.co + slab + 8 * q);
-        } else {
-          const int64_t vox = gx + (int64_t)n1 * (gy + (int64_t)n2 * gz);
-          v = *(const uint4*)(in + vox * iv.cs + iv.co + slab + 8 * q);
-        }
-      }
-      *(uint4*)(lds + h * RS + 8 * q) = v;
-    }
+    stage_slab<HN, 2>(2, RS, x0, y0, z0, n1, n2, n3, in, iv, lo, slab, lds);
     __syncthreads();
 #pragma unroll
     for (int pr = 0; pr < 14; pr++) {

@@ -468,6 +468,9 @@ struct SbmState {
   uint16_t* rec_x = nullptr;  // armed recording of the next step's network input
   double* rec_start = nullptr;
   int nprev = 0;
+  // names the current contents of kapp for the solver's cached multigrid
+  // hierarchy: bumped whenever a field may have been changed from outside
+  uint64_t kap_version = 1;
   NnNet* ws_net = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // step timer, created once
   ~SbmState() {
@@ -568,6 +571,7 @@ int solve(SbmState& s, int op, int mode, SolveResult* r) {
     a.pse = s.g(s.pse);
     a.rhs = s.g(s.rhsp);
     a.scale = 1.0 / (P.dx * P.dx);
+    a.coef_id = s.kap_version;  // kapp = kap_scale psi_P, fixed over the run
     a.crt = P.crt_p;
     a.maxit = P.max_it_p;
     a.w8 = s.g(s.psp);
@@ -813,7 +817,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
       // so re-mirroring them changes no value
       ELSIM_TRY(bc(s, s.g(s.php), only_hi));
     }
-    if (nlp == 0 && ws_on && ws_hist >= 1) {
+    if (nlp == 0 && ws_on && ws_hist >= 1 && P.ws_order != 1) {
       // second-order term: a Galerkin step along the second difference
       // d1 - d2 of phi_P's last two changes (step 1 is quadratic
       // extrapolation, 0 keeps the linear one), A-norm optimal, so never
@@ -824,7 +828,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
       double a2 = 0.0;
       ELSIM_TRY(galerkin_step(s, ELSIM_OP_PHP, s.g(s.php), s.g(s.kapp), s.g(s.rhsp), s.g(s.ws_cand), pbc0, &a2));
       moved = moved || a2 != 0.0;
-      if (ws_hist >= 2) {
+      if (ws_hist >= 2 && P.ws_order != 2) {
         // third-order term, likewise: the third difference d1 - 2 d2 + d3
         k_diff3_interior<<<grid, blk, 0, stv>>>(d, s.g(s.ws_dir), s.g(s.ws_dir2), s.g(s.ws_dir3), s.g(s.ws_cand));
         ELSIM_CHECK_LAUNCH();
@@ -1170,6 +1174,7 @@ int elsim_sbm_set(const int64_t* handle, const int* field, const double* in) {
   ELSIM_TRY(field_info(*s, *field, &p, &n));
   ELSIM_HIP(hipMemcpyAsync(p, in, n * 8, hipMemcpyHostToDevice, stream()));
   ELSIM_HIP(hipStreamSynchronize(stream()));
+  s->kap_version++;
   if (*field == ELSIM_F_CNP) {
     ELSIM_TRY(red_gw(*s, s->g(s->cnp), s->g(s->psp), R_XRF));
     ELSIM_TRY(sync_scal(*s));
@@ -1187,6 +1192,7 @@ int elsim_sbm_field_ptr(const int64_t* handle, const int* field, double** ptr) {
   SbmState* s = lookup(handle);
   if (!s || !field || !ptr) return ELSIM_ERR_ARG;
   size_t n;
+  s->kap_version++;  // the caller may write through the pointer
   return field_info(*s, *field, ptr, &n);
 }
 
@@ -1206,6 +1212,7 @@ int elsim_sbm_set_params(const int64_t* handle, const elsim_sbm_params* prm) {
   SbmState* s = lookup(handle);
   if (!s || !prm) return ELSIM_ERR_ARG;
   s->P = *prm;
+  s->kap_version++;
   return ELSIM_OK;
 }
 

@@ -45,7 +45,7 @@ struct CgWork {
   CgDev* host = nullptr;             // pinned mirror
   double* partials = nullptr;        // reduction partials (two values); null: the library workspace
   bool bound = false;                // buffers live in memory the caller provided (never freed here)
-  MgWork* mg = nullptr;              // multigrid hierarchy (ELSIM_MODE_MIXED_MG), created on first use
+  MgWork* mg[4] = {};                // multigrid work space per operator (ELSIM_MODE_MIXED_MG), made on first use
   int alloc(const Dims& d);
   // Lay the buffers of grid d out in [base, base + bytes) (no allocation),
   // with `host` as the scalars' host mirror, and clear them with a fill
@@ -71,6 +71,7 @@ struct SolveArgs {
   double tw;            // its interior total
   double delta;         // mixed: reliable-update trigger
   int rel_r0 = 0;       // relative rule against ||r0|| (the start's residual) instead of ||b_eff||
+  uint64_t coef_id = 0;  // nonzero: names kap (and pse); the multigrid set-up is reused while it is unchanged
 };
 
 struct SolveResult {
@@ -117,6 +118,11 @@ bool initialized();
 // Set-up of the allocation-free entry points on the current device (called
 // by elsim_init).
 int noalloc_init(int device);
+
+// In-loop kernel timing (elsim_prof_enable): when on, records the event at
+// the start (end = 0) or end (end = 1) of slot's bracket on the library
+// stream.
+int prof_event(int slot, int end);
 
 // Flat kernels shared with the SBM driver.
 int launch_copy_interior_g2n(const Dims& d, const double* g, double* n);  // n = g(interior)

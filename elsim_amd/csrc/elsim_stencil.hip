@@ -62,46 +62,6 @@ __global__ __launch_bounds__(256) void k_matvec(Dims d, const T* __restrict__ p,
   if (dot || cg) block_partials<1>(acc, partials);
 }
 
-// ------------------------------------------ FP32 operator, 4 voxels a lane --
-//
-// The FP32 operator of the CG loop with four consecutive d1 voxels per lane:
-// one 16-byte load per row and array (rows start at odd ghosted indices, so
-// these loads are 4-byte aligned, which the hardware serves as one access).
-// A d1 row segment of TW = 64, 128 or 256 voxels takes TW/4 lanes, so a wave
-// covers 256/TW segments, each with its own SJ4 rows; a lane keeps its rows
-// of p and kappa for the planes k and k+1 in registers and marches SKC4
-// planes.  Every face flux f (p_b - p_a) is formed once and used by both
-// voxels it separates: the face coefficient and the difference are the same
-// IEEE operations as in sweep() (the addition commutes) and the six fluxes
-// are combined in the same order, so the results are bit-identical to it.
-// At 256^3 (phP, data from HBM) this takes 44.7 us against 56 us for sweep(),
-// which issues 32 memory instructions per wave and plane (16 of them
-// single-lane d1 edge loads) where this kernel issues 16.
-constexpr int SJ4 = 4, SKC4 = 8;
-__host__ __device__ inline int mv4_tw(const Dims& d) { return d.n1 <= 64 ? 64 : (d.n1 <= 128 ? 128 : 256); }
-// usable when n1 is a multiple of 4 and ghosted indices fit 32 bits
-inline bool mv4_ok(const Dims& d) { return d.n1 % 4 == 0 && d.G() < ((int64_t)1 << 32); }
-__host__ __device__ inline TileGrid mv4_tiles(const Dims& d) {
-  const int tw = mv4_tw(d), rows = 4 * (256 / tw) * SJ4;  // d2 rows per block
-  return TileGrid{(d.n1 + tw - 1) / tw, (d.n2 + rows - 1) / rows, (d.n3 + SKC4 - 1) / SKC4};
-}
-inline int mv4_ntiles(const Dims& d) {
-  const TileGrid t = mv4_tiles(d);
-  return t.gx * t.gy * t.gz;
-}
-__device__ __forceinline__ float4 ld4(const float* __restrict__ a, uint32_t q) {
-  float4 v;
-  __builtin_memcpy(&v, a + q, 16);
-  return v;
-}
-__device__ __forceinline__ void st4(float* __restrict__ a, uint32_t q, float4 v) { __builtin_memcpy(a + q, &v, 16); }
-__device__ __forceinline__ void unpack4(const float4& v, float (&o)[4]) {
-  o[0] = v.x;
-  o[1] = v.y;
-  o[2] = v.z;
-  o[3] = v.w;
-}
-
 // out = A p with the homogeneous ghost rule; out ghosted (OUTG) or interior;
 // partials of p.out.  Launch: mv4_ntiles(d) blocks of 256 threads.  EDGE:
 // the box has more than one d1 tile, so the first and last lane of a row
@@ -112,174 +72,17 @@ __global__ __launch_bounds__(256) void k_matvec4(Dims d, const float* __restrict
                                                  const float* __restrict__ pse, float scale, float* __restrict__ out,
                                                  CgDev* cg, double* partials) {
   if (cg && (cg->done || cg->need_reliable)) return;
-  constexpr int SJ = SJ4;
   double acc = 0.0;
-  const TileGrid tg = mv4_tiles(d);
-  const int nt = tg.gx * tg.gy * tg.gz;
-  const int tw = mv4_tw(d), lpr = tw / 4, rpw = 64 / lpr;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int grp = lane / lpr, x4 = lane % lpr;
-  const float half = 0.5f;
-  const uint32_t sz = (uint32_t)d.g12();
-  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    const int per = nt / 8;
-    int b = tile;
-    if (b < per * 8) b = (b % 8) * per + b / 8;
-    const int tj = b % tg.gy, ti = (b / tg.gy) % tg.gx, tk = b / (tg.gx * tg.gy);
-    const int i0 = ti * tw + 4 * x4 + 1;
-    const bool active = i0 <= d.n1;
-    const int ia = active ? i0 : d.n1 - 3;  // idle lanes read an interior column
-    const bool left_b = i0 == 1, right_b = i0 + 3 == d.n1;
-    const bool ld_l = EDGE && active && x4 == 0 && i0 > 1, ld_r = EDGE && active && x4 == lpr - 1 && i0 + 3 < d.n1;
-    const int j0 = ((tj * 4 + wv) * rpw + grp) * SJ + 1;
-    const int k0 = tk * SKC4 + 1, k1 = min(k0 + SKC4 - 1, d.n3);
-    const bool down = tk & 1;  // odd tiles march downward (see sweep_ld)
-    const int kstart = down ? k1 : k0;
-    const uint32_t step = down ? 0u - sz : sz;
-    uint32_t rc[SJ];
+  const int nt = mv4_ntiles(d);
+  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x)
+    mv4_tile<OP, EDGE>(
+        d, tile, [&](uint32_t q, int, int, int) { return ld4(p, q); }, [&](uint32_t q, int, int, int) { return p[q]; },
+        kap, pse, scale, [&](int, int j, int k, uint32_t q, const float (&pv)[4], const float (&o)[4]) {
 #pragma unroll
-    for (int t = 0; t < SJ; t++) rc[t] = (uint32_t)d.gi(ia, min(j0 + t, d.n2 + 1), kstart);
-    const uint32_t clo = (uint32_t)d.gi(ia, min(j0 - 1, d.n2 + 1), kstart);
-    const uint32_t chi = (uint32_t)d.gi(ia, min(j0 + SJ, d.n2 + 1), kstart);
-    float4 pc[SJ], kc[SJ];
-    float fb[SJ][4];  // flux through the face between plane k and the plane behind it
-    {
-      const bool bnd = down ? kstart == d.n3 : kstart == 1;
-#pragma unroll
-      for (int t = 0; t < SJ; t++) {
-        pc[t] = ld4(p, rc[t]);
-        kc[t] = ld4(kap, rc[t]);
-        float pcv[4], kcv[4], pbv[4], kbv[4];
-        unpack4(pc[t], pcv);
-        unpack4(kc[t], kcv);
-        unpack4(ld4(p, rc[t] - step), pbv);
-        unpack4(ld4(kap, rc[t] - step), kbv);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          // upward: f5 (x - pm); downward: f6 (pz - x); pm / pz = x on the boundary plane
-          const float f = (kbv[q] + kcv[q]) * half;
-          const float nb = bnd ? pcv[q] : pbv[q];
-          fb[t][q] = down ? f * (nb - pcv[q]) : f * (pcv[q] - nb);
-        }
-      }
-    }
-    uint32_t off = 0;
-    const int nk = k1 - k0 + 1;
-    for (int kk = 0; kk < nk; kk++, off += step) {
-      const int k = down ? k1 - kk : k0 + kk;
-      float4 pa[SJ], ka[SJ];
-#pragma unroll
-      for (int t = 0; t < SJ; t++) {
-        pa[t] = ld4(p, rc[t] + off + step);
-        ka[t] = ld4(kap, rc[t] + off + step);
-      }
-      const float4 hlp = ld4(p, clo + off), hlk = ld4(kap, clo + off);
-      const float4 hhp = ld4(p, chi + off), hhk = ld4(kap, chi + off);
-      float psv[SJ][4];
-      if (Rule<OP>::diag) {
-#pragma unroll
-        for (int t = 0; t < SJ; t++) unpack4(ld4(pse, rc[t] + off), psv[t]);
-      }
-      float elp[SJ], elk[SJ], erp[SJ], erk[SJ];
-#pragma unroll
-      for (int t = 0; t < SJ; t++) elp[t] = elk[t] = erp[t] = erk[t] = 0.0f;
-      if (EDGE && ld_l) {
-#pragma unroll
-        for (int t = 0; t < SJ; t++) {
-          elp[t] = p[rc[t] + off - 1];
-          elk[t] = kap[rc[t] + off - 1];
-        }
-      }
-      if (EDGE && ld_r) {
-#pragma unroll
-        for (int t = 0; t < SJ; t++) {
-          erp[t] = p[rc[t] + off + 4];
-          erk[t] = kap[rc[t] + off + 4];
-        }
-      }
-      const bool kbnd = down ? k == 1 : k == d.n3;  // the face ahead is a boundary face
-      // d2 faces: jf[t] lies between rows j0 + t - 1 and j0 + t
-      float jf[SJ + 1][4];
-#pragma unroll
-      for (int t = 0; t <= SJ; t++) {
-        float plv[4], klv[4], puv[4], kuv[4];
-        unpack4(t == 0 ? hlp : pc[t - 1], plv);
-        unpack4(t == 0 ? hlk : kc[t - 1], klv);
-        unpack4(t == SJ ? hhp : pc[t], puv);
-        unpack4(t == SJ ? hhk : kc[t], kuv);
-        const int jl = j0 + t - 1;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const float f = (klv[q] + kuv[q]) * half;
-          float lo = plv[q], hi = puv[q];
-          if (jl == 0) lo = Rule<OP>::lo_zero ? 0.0f : hi;       // below row 1
-          if (jl == d.n2) hi = Rule<OP>::hi_zero ? 0.0f : lo;    // above row n2
-          jf[t][q] = f * (hi - lo);
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < SJ; t++) {
-        const int j = j0 + t;
-        float pv[4], kv[4], pav[4], kav[4];
-        unpack4(pc[t], pv);
-        unpack4(kc[t], kv);
-        unpack4(pa[t], pav);
-        unpack4(ka[t], kav);
-        float pL = lane_shift<true>(pv[3]), kL = lane_shift<true>(kv[3]);
-        float pR = lane_shift<false>(pv[0]), kR = lane_shift<false>(kv[0]);
-        if (EDGE && ld_l) {
-          pL = elp[t];
-          kL = elk[t];
-        }
-        if (EDGE && ld_r) {
-          pR = erp[t];
-          kR = erk[t];
-        }
-        // d1 faces: xf[q] lies between voxels q - 1 and q
-        float xf[5];
-#pragma unroll
-        for (int q = 0; q <= 4; q++) {
-          const float kl = q == 0 ? kL : kv[q - 1], kr = q == 4 ? kR : kv[q];
-          float pl = q == 0 ? pL : pv[q - 1], pr = q == 4 ? pR : pv[q];
-          if (q == 0 && left_b) pl = pr;
-          if (q == 4 && right_b) pr = pl;
-          xf[q] = ((kl + kr) * half) * (pr - pl);
-        }
-        float o[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const float xv = pv[q];
-          const float f = (kav[q] + kv[q]) * half;
-          const float nb = kbnd ? xv : pav[q];
-          const float fa = down ? f * (xv - nb) : f * (nb - xv);
-          const float fup = down ? fb[t][q] : fa, flo = down ? fa : fb[t][q];
-          float sv = xf[q + 1] - xf[q];
-          sv = sv + jf[t + 1][q];
-          sv = sv - jf[t][q];
-          sv = sv + fup;
-          sv = sv - flo;
-          float v;
-          if (Rule<OP>::diag)
-            v = psv[t][q] * xv - scale * sv;
-          else
-            v = scale * sv;
-          o[q] = v;
-          fb[t][q] = fa;
-        }
-        if (active && j <= d.n2) {
-#pragma unroll
-          for (int q = 0; q < 4; q++) acc += (double)pv[q] * (double)o[q];
-          const uint32_t dst = OUTG ? rc[t] + off : (uint32_t)d.ni(i0, j, k);
+          for (int c = 0; c < 4; c++) acc += (double)pv[c] * (double)o[c];
+          const uint32_t dst = OUTG ? q : (uint32_t)d.ni((int)(q % (uint32_t)d.g1()), j, k);
           st4(out, dst, make_float4(o[0], o[1], o[2], o[3]));
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < SJ; t++) {
-        pc[t] = pa[t];
-        kc[t] = ka[t];
-      }
-    }
-  }
+        });
   if (partials) {
     double v[1] = {acc};
     block_partials<1>(v, partials);
@@ -288,6 +91,7 @@ __global__ __launch_bounds__(256) void k_matvec4(Dims d, const float* __restrict
 
 // Initial residual with x's ghosts as stored (SBM:492-499, 649-655, 729-735):
 // res = r0 = rhs - A x; prj(interior) = res; x0 = x(interior); cg->rr = sum res^2.
+// res and prj may be null (the mixed solvers start from r0 alone).
 template <int OP>
 __global__ __launch_bounds__(256) void k_resid64(Dims d, const double* __restrict__ x, const double* __restrict__ kap,
                                                  const double* __restrict__ pse, double scale,
@@ -300,7 +104,7 @@ __global__ __launch_bounds__(256) void k_resid64(Dims d, const double* __restric
     sweep<double, OP, false>(d, stencil_idx<Rows<double>::SJ>(d, tile), x, kap, pse, scale,
                              [&](int64_t c, int64_t n, double xc, double v) {
       const double r = rhs[n] - v;
-      res[c] = r;
+      if (res) res[c] = r;
       r0[c] = r;
       if (prj) prj[c] = r;
       x0[c] = xc;
@@ -389,7 +193,7 @@ __global__ __launch_bounds__(256) void k_true_resid(Dims d, const double* __rest
       if (!FINAL) {
         const float rf = (float)r;
         r32[c] = rf;
-        acc[1] += (double)rf * (double)(bf16_to_f32(dinv[c]) * rf);
+        if (dinv) acc[1] += (double)rf * (double)(bf16_to_f32(dinv[c]) * rf);
       }
     });
   block_partials<2>(acc, partials);
@@ -445,30 +249,7 @@ __global__ __launch_bounds__(256) void k_dot_resid(Dims d, const double* __restr
 
 // ------------------------------------------------------------- Jacobi --
 
-// dinv = 1 / diag(A_hom) (FP64), stored as T (uint16_t: BF16).  Mirror
-// faces do not contribute; zero-Dirichlet faces do.  Interior of a ghosted
-// array.
-template <int OP, typename T>
-__global__ __launch_bounds__(256) void k_dinv(Dims d, const double* __restrict__ kap, const double* __restrict__ pse,
-                                              double scale, T* __restrict__ dinv) {
-  const ColIdx ci = col_idx(d);
-  const int64_t sx = d.g1(), sz = d.g12();
-  for_col(d, ci, [&](int64_t c, int64_t, int k) {
-    const double kc = kap[c];
-    double s = 0.0;
-    if (ci.i > 1) s += (kap[c - 1] + kc) * 0.5;
-    if (ci.i < d.n1) s += (kap[c + 1] + kc) * 0.5;
-    if (ci.j > 1 || Rule<OP>::lo_zero) s += (kap[c - sx] + kc) * 0.5;
-    if (ci.j < d.n2 || Rule<OP>::hi_zero) s += (kap[c + sx] + kc) * 0.5;
-    if (k > 1) s += (kap[c - sz] + kc) * 0.5;
-    if (k < d.n3) s += (kap[c + sz] + kc) * 0.5;
-    const double dg = Rule<OP>::diag ? pse[c] + scale * s : -scale * s;
-    if constexpr (std::is_same<T, uint16_t>::value)
-      dinv[c] = f32_to_bf16((float)(1.0 / dg));
-    else
-      dinv[c] = (T)(1.0 / dg);
-  });
-}
+
 
 // ------------------------------------------------------ FP64 Jacobi-PCG --
 
@@ -652,20 +433,6 @@ __global__ __launch_bounds__(256) void k_mg_start(int64_t G2, const double2* __r
     p32[q] = make_float2(0.f, 0.f);
     y32[q] = make_float2(0.f, 0.f);
     e64[q] = make_double2(0.0, 0.0);
-  })
-}
-
-// r -= alpha w (the V-cycle that follows reduces r.r and r.z).
-__global__ __launch_bounds__(256) void k_mg_upd(int64_t G2, float2* __restrict__ r32, const float2* __restrict__ w32,
-                                                const CgDev* cg) {
-  if (cg->done || cg->need_reliable) return;
-  const float alp = (float)cg->alpha;
-  float2 rv[UPD_P], wv[UPD_P];
-  FLAT_LOOP_U(UPD_P, G2, { rv[u] = r32[q]; wv[u] = w32[q]; }, {
-    float2 rn;
-    rn.x = rv[u].x - alp * wv[u].x;
-    rn.y = rv[u].y - alp * wv[u].y;
-    r32[q] = rn;
   })
 }
 
@@ -960,11 +727,12 @@ void CgWork::release() {
   DevBuf* bs[] = {&res, &wmx, &prj, &x0, &r0, &e64, &y32, &r32, &w32, &p32, &kap32, &pse32, &dinv32, &dinv64,
                   &cg};
   for (DevBuf* b : bs) dev_free(*b);
-  if (mg) {
-    mg->release();
-    delete mg;
-    mg = nullptr;
-  }
+  for (MgWork*& m : mg)
+    if (m) {
+      m->release();
+      delete m;
+      m = nullptr;
+    }
   if (host) (void)hipHostFree(host);
   host = nullptr;
   d = Dims{0, 0, 0};
@@ -1005,6 +773,35 @@ int poll(CgWork& w, hipStream_t s) {
   return ELSIM_OK;
 }
 
+// Iterations to launch before the next poll of a mixed solver: as many as
+// the FP32 residual needs to reach its next threshold (the tolerance or the
+// reliable-update trigger) at the per-iteration reduction rate seen so far,
+// 1 .. kMaxBatch.  Fewer cost extra host round trips, more cost launches
+// that return at once.
+constexpr int kMaxBatch = 8;
+struct BatchPlan {
+  double rate = -1.0;  // per-iteration factor of r.r (unknown: < 0)
+  double rr_base = -1.0;
+  int it_base = 0;
+  // after a poll: learn the rate from the iterations since the last one
+  void observe(const CgDev& h) {
+    if (rr_base > 0.0 && h.iters > it_base && h.rr > 0.0 && !h.need_reliable)
+      rate = std::pow(h.rr / rr_base, 1.0 / (h.iters - it_base));
+  }
+  // the next batch from the state after the poll (and a reliable update,
+  // which leaves the true residual in rmax2)
+  int next(const CgDev& h, bool restarted, int dflt) {
+    const double rr = restarted ? h.rmax2 : h.rr;
+    rr_base = rr;
+    it_base = h.iters;
+    if (!(rate > 0.0) || !(rate < 0.999) || !(rr > 0.0)) return dflt;
+    const double thr = std::max(h.tol2, h.delta2 * h.rmax2);
+    if (!(rr > thr)) return 1;
+    const double need = std::ceil(std::log(thr / rr) / std::log(rate));
+    return (int)std::min<double>(kMaxBatch, std::max(1.0, need));
+  }
+};
+
 template <int OP>
 int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   const Dims& d = a.d;
@@ -1039,13 +836,16 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   double* e64 = (double*)w.e64.p;
   auto fin1 = [&](int op, int nb) { k_fin<1><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
   auto fin2 = [&](int op, int nb) { k_fin<2><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
-  k_resid64<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, res, r0,
-                                     mode == ELSIM_MODE_MIXED ? nullptr : prj, x0, cg, P);
+  const bool fp32_inner = mode == ELSIM_MODE_MIXED || mode == ELSIM_MODE_MIXED_MG;
+  k_resid64<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, fp32_inner ? nullptr : res, r0,
+                                     fp32_inner ? nullptr : prj, x0, cg, P);
   fin1(FIN_RESID, nb64);
-  k_beff<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, r0, a.rtol, cg, P);
-  fin1(FIN_BEFF, nb64);
+  // b_eff's norm is not used when the rule is relative to ||r0|| (CnE's
+  // increment rule): the finaliser then only sets the tolerance
+  if (!a.rel_r0) k_beff<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, r0, a.rtol, cg, P);
+  fin1(FIN_BEFF, a.rel_r0 ? 0 : nb64);
   ELSIM_CHECK_LAUNCH();
-  const int batch = 8;
+  const int batch = kMaxBatch;
   if (mode == ELSIM_MODE_FP64_PCG) {
     double* dinv64 = (double*)w.dinv64.p;
     k_dinv<OP, double><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv64);
@@ -1103,34 +903,36 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   } else if (mode == ELSIM_MODE_MIXED_MG) {
     // FP32 CG preconditioned by one multigrid V-cycle, FP64 reliable updates
     // as in the Jacobi-preconditioned mixed path below.  Per iteration: the
-    // operator (p.Ap) -> alpha; r -= alpha w; the V-cycle z = M^-1 r (r.r
-    // and r.z partials) -> beta or a reliable-update request; y += alpha p,
-    // p = z + beta p.
-    if (!w.mg) w.mg = new MgWork();
-    uint16_t* dinv = (uint16_t*)w.dinv32.p;  // BF16
+    // operator (p.Ap) -> alpha; r -= alpha w with the pre-smoothed iterate
+    // omega dinv r; the V-cycle z = M^-1 r (r.r and r.z partials) -> beta or a
+    // reliable-update request; y += alpha p, p = z + beta p.  The operator's
+    // FP32 coefficient and the hierarchy live in the operator's multigrid
+    // work space and are reused while coef_id names the same coefficient.
+    if (!w.mg[OP]) w.mg[OP] = new MgWork();
+    MgWork& m = *w.mg[OP];
     float *r32 = (float*)w.r32.p, *w32 = (float*)w.w32.p, *y32 = (float*)w.y32.p, *p32 = (float*)w.p32.p;
-    const float* k32 = (const float*)w.kap32.p;
-    const float* s32 = (const float*)w.pse32.p;
-    k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.kap, (float*)w.kap32.p);
-    if (OP == ELSIM_OP_CNE) k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.pse, (float*)w.pse32.p);
-    k_dinv<OP, uint16_t><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv);
-    ELSIM_TRY(mg_setup(*w.mg, OP, d, a.kap, a.pse, a.scale));
+    ELSIM_TRY(mg_setup(m, OP, d, a.kap, a.pse, a.scale, a.coef_id));
+    const float* k32 = m.kap32;
+    const float* s32 = m.pse32;
     const float sc32 = (float)a.scale;
     const int nbv = mg_partial_blocks(d);
     const bool mv4 = mv4_ok(d), mv4_edge = mv4_tiles(d).gx > 1;
     const int nbm = mv4 ? mv4_ntiles(d) : nb32;
     const int fbd = (int)((G2 + (int64_t)FLAT_T * DIR_P - 1) / ((int64_t)FLAT_T * DIR_P));
-    auto precondition = [&]() { return mg_vcycle(*w.mg, OP, d, k32, s32, dinv, sc32, r32, w32, cg, P); };
+    auto precondition = [&]() { return mg_vcycle(m, OP, d, r32, w32, cg, P, false); };
     k_mg_start<<<fbu, FLAT_T, 0, s>>>(G2, (const double2*)r0, (float2*)r32, (float2*)p32, (float2*)y32,
                                       (double2*)e64);
+    ELSIM_TRY(mg_presmooth(m, d, r32, cg));
     ELSIM_TRY(precondition());
     fin2(FIN_MG_START, nbv);
     k_mg_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)w32, cg);
     ELSIM_CHECK_LAUNCH();
     int it_prev = 0;
+    BatchPlan plan;
+    int nrun = 2;  // a few iterations before the first poll: warm-started solves often need only these
     for (;;) {
       const bool pr = g_prof.on;
-      for (int b = 0; b < batch; b++) {
+      for (int b = 0; b < nrun; b++) {
         const bool t = pr && b == 0;
         ELSIM_TRY(timed(t, ELSIM_PROF_OP32, s, [&] {
           if (mv4 && mv4_edge)
@@ -1141,12 +943,11 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
             k_matvec<float, OP, true, true><<<nb32, blk, 0, s>>>(d, p32, k32, s32, sc32, w32, nullptr, cg, 1, P);
         }));
         fin1(FIN_ALPHA_RZ, nbm);
-        ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] {
-          k_mg_upd<<<fbu, FLAT_T, 0, s>>>(G2, (float2*)r32, (const float2*)w32, cg);
-        }));
-        int prc = ELSIM_OK;
-        ELSIM_TRY(timed(t, ELSIM_PROF_MG, s, [&] { prc = precondition(); }));
-        ELSIM_TRY(prc);
+        int rc = ELSIM_OK;
+        ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] { rc = mg_update(m, d, r32, w32, cg); }));
+        ELSIM_TRY(rc);
+        ELSIM_TRY(timed(t, ELSIM_PROF_MG, s, [&] { rc = mg_vcycle(m, OP, d, r32, w32, cg, P, t); }));
+        ELSIM_TRY(rc);
         fin2(FIN_MG_UPDATE, nbv);
         ELSIM_TRY(timed(t, ELSIM_PROF_DIR32, s, [&] {
           k_mg_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)w32, cg);
@@ -1156,16 +957,19 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
       ELSIM_TRY(poll(w, s));
       if (pr && w.host->iters >= it_prev + 2) {
         ELSIM_TRY(prof_collect(ELSIM_PROF_OP32, ELSIM_PROF_DIR32));
-        ELSIM_TRY(prof_collect(ELSIM_PROF_MG, ELSIM_PROF_MG));
+        ELSIM_TRY(prof_collect(ELSIM_PROF_MG, ELSIM_PROF_MG_UP));
       }
       it_prev = w.host->iters;
       if (w.host->done) break;
-      if (w.host->need_reliable) {
+      plan.observe(*w.host);
+      const bool restart = w.host->need_reliable != 0;
+      if (restart) {
         // fold y into the FP64 correction, FP64 true residual, restart from
         // it with z = M^-1 r
         k_mx_fold<<<fb, FLAT_T, 0, s>>>(G, e64, y32, cg);
-        k_true_resid<OP, false><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, r32, dinv, cg, P);
+        k_true_resid<OP, false><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, r32, nullptr, cg, P);
         fin2(FIN_TRUE_RELIABLE, nb64);
+        ELSIM_TRY(mg_presmooth(m, d, r32, cg));
         ELSIM_TRY(precondition());
         fin2(FIN_MG_START, nbv);
         k_mg_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)w32, cg);
@@ -1173,6 +977,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
         ELSIM_TRY(poll(w, s));
         if (w.host->done) break;
       }
+      nrun = plan.next(*w.host, restart, 2);
     }
     k_add_e<<<fb, FLAT_T, 0, s>>>(G, a.x, e64);
   } else {
@@ -1190,9 +995,11 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     const int nbm = mv4 ? mv4_ntiles(d) : nb32;
     const int fbd = (int)((G2 + (int64_t)FLAT_T * DIR_P - 1) / ((int64_t)FLAT_T * DIR_P));
     int it_prev = 0;
+    BatchPlan plan;
+    int nrun = batch;
     for (;;) {
       const bool pr = g_prof.on;
-      for (int b = 0; b < batch; b++) {
+      for (int b = 0; b < nrun; b++) {
         const bool t = pr && b == 0;
         ELSIM_TRY(timed(t, ELSIM_PROF_OP32, s, [&] {
           if (mv4 && mv4_edge)
@@ -1220,12 +1027,14 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
       if (pr && w.host->iters >= it_prev + 2) ELSIM_TRY(prof_collect(ELSIM_PROF_OP32, ELSIM_PROF_DIR32));
       it_prev = w.host->iters;
       if (w.host->done) break;
+      plan.observe(*w.host);
+      const bool restart = w.host->need_reliable != 0;
       // Once the FP32 residual has dropped by delta (or meets the tolerance,
       // or the cap is reached) the device sets need_reliable and the rest of
       // the batch is skipped; the reliable update runs here, between
       // batches: fold y into the FP64 correction, FP64 true residual, and a
       // restart of the FP32 recurrence from it.
-      if (w.host->need_reliable) {
+      if (restart) {
         k_mx_fold<<<fb, FLAT_T, 0, s>>>(G, e64, y32, cg);
         k_true_resid<OP, false><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, r32, dinv, cg, P);
         fin2(FIN_TRUE_RELIABLE, nb64);
@@ -1235,6 +1044,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
         ELSIM_TRY(poll(w, s));
         if (w.host->done) break;
       }
+      nrun = plan.next(*w.host, restart, batch);
     }
     k_add_e<<<fb, FLAT_T, 0, s>>>(G, a.x, e64);
   }
@@ -1250,6 +1060,11 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
 }
 
 }  // namespace
+
+int prof_event(int slot, int end) {
+  if (g_prof.on) ELSIM_HIP(hipEventRecord(g_prof.ev[2 * slot + (end ? 1 : 0)], stream()));
+  return ELSIM_OK;
+}
 
 int launch_resid_norm2(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
                        const double* rhs, double* out) {
@@ -1338,7 +1153,13 @@ int cg_solve(CgWork& w, const SolveArgs& a, int mode, SolveResult* res) {
   }
   // CnE's reaction term keeps it well conditioned: Jacobi-PCG needs a few
   // iterations there, fewer than a V-cycle costs
-  if (mode == ELSIM_MODE_MIXED_MG && a.op == ELSIM_OP_CNE) mode = ELSIM_MODE_MIXED;
+  // (with the multigrid path's reliable-update threshold: CnE converges in a
+  // few iterations, each reducing its residual by more than half)
+  if (mode == ELSIM_MODE_MIXED_MG && a.op == ELSIM_OP_CNE) {
+    SolveArgs b = a;
+    b.delta = mg_cfg().delta;
+    return solve_op<ELSIM_OP_CNE>(w, b, ELSIM_MODE_MIXED, res);
+  }
   if (mode == ELSIM_MODE_MIXED_MG && w.bound) {
     set_error("the multigrid solver needs library-allocated work space");
     return ELSIM_ERR_ARG;
@@ -1460,33 +1281,12 @@ int elsim_dev_mg_apply(int op, int n1, int n2, int n3, const double* kap, const 
   }
   ELSIM_TRY(ensure_init());
   static MgWork mg;
-  static DevBuf k32, s32, dv;
   const Dims d{n1, n2, n3};
-  if (stencil_ntiles<float>(d) > kMaxRedBlocks) {
-    set_error("grid too large for the reduction workspace");
-    return ELSIM_ERR_ARG;
-  }
-  const size_t Ge = ((size_t)d.G() + 1) & ~(size_t)1;
-  ELSIM_TRY(dev_alloc(k32, Ge * 4));
-  ELSIM_TRY(dev_alloc(s32, Ge * 4));
-  ELSIM_TRY(dev_alloc(dv, Ge * 2));
   hipStream_t s = stream();
-  ELSIM_HIP(hipMemsetAsync(dv.p, 0, Ge * 2, s));
   ELSIM_HIP(hipMemsetAsync(z, 0, (size_t)d.G() * 4, s));
-  const int fb = flat_blocks(d.G());
-  k_to_f32<<<fb, FLAT_T, 0, s>>>(d.G(), kap, (float*)k32.p);
-  if (op == ELSIM_OP_CNE) k_to_f32<<<fb, FLAT_T, 0, s>>>(d.G(), pse, (float*)s32.p);
-  const dim3 gc = col_grid(d), blk(TX, TY, 1);
-  switch (op) {
-    case ELSIM_OP_CNE: k_dinv<ELSIM_OP_CNE, uint16_t><<<gc, blk, 0, s>>>(d, kap, pse, scale, (uint16_t*)dv.p); break;
-    case ELSIM_OP_PHP: k_dinv<ELSIM_OP_PHP, uint16_t><<<gc, blk, 0, s>>>(d, kap, pse, scale, (uint16_t*)dv.p); break;
-    case ELSIM_OP_PHE: k_dinv<ELSIM_OP_PHE, uint16_t><<<gc, blk, 0, s>>>(d, kap, pse, scale, (uint16_t*)dv.p); break;
-    default: k_dinv<ELSIM_OP_POT_DD, uint16_t><<<gc, blk, 0, s>>>(d, kap, pse, scale, (uint16_t*)dv.p); break;
-  }
-  ELSIM_CHECK_LAUNCH();
-  ELSIM_TRY(mg_setup(mg, op, d, kap, pse, scale));
-  ELSIM_TRY(mg_vcycle(mg, op, d, (const float*)k32.p, (const float*)s32.p, (const uint16_t*)dv.p, (float)scale, r, z,
-                      nullptr, workspace().partials));
+  ELSIM_TRY(mg_setup(mg, op, d, kap, pse, scale, 0));
+  ELSIM_TRY(mg_presmooth(mg, d, r, nullptr));
+  ELSIM_TRY(mg_vcycle(mg, op, d, r, z, nullptr, workspace().partials, false));
   ELSIM_HIP(hipStreamSynchronize(s));
   return ELSIM_OK;
 }

@@ -191,4 +191,251 @@ __device__ __forceinline__ void for_col(const Dims& d, const ColIdx& ci, F&& f) 
     }                                                                                    \
   }
 
+// ------------------------------------------ FP32 operator, 4 voxels a lane --
+//
+// The FP32 operator of the CG loop with four consecutive d1 voxels per lane:
+// one 16-byte load per row and array (rows start at odd ghosted indices, so
+// these loads are 4-byte aligned, which the hardware serves as one access).
+// A d1 row segment of TW = 64, 128 or 256 voxels takes TW/4 lanes, so a wave
+// covers 256/TW segments, each with its own SJ4 rows; a lane keeps its rows
+// of p and kappa for the planes k and k+1 in registers and marches SKC4
+// planes.  Every face flux f (p_b - p_a) is formed once and used by both
+// voxels it separates: the face coefficient and the difference are the same
+// IEEE operations as in sweep() (the addition commutes) and the six fluxes
+// are combined in the same order, so the results are bit-identical to it.
+// At 256^3 (phP, data from HBM) this takes 44.7 us against 56 us for sweep(),
+// which issues 32 memory instructions per wave and plane (16 of them
+// single-lane d1 edge loads) where this kernel issues 16.
+//
+// mv4_tile walks one tile with the values of p supplied by the caller:
+// ld(q, i, j, k) returns the four values at ghosted index q, the voxels
+// i..i+3 of row j, plane k (j and k may be ghost rows/planes, whose values
+// the homogeneous rule discards); ld1(q, i, j, k) one value (the d1
+// neighbour across a tile edge).  For every interior row of four
+// voxels epi(t, j, k, q, p, Ap) receives the row slot t (0..SJ4-1), its row
+// and plane, its ghosted index and the four values of p and A p.  EDGE: the
+// box has more than one d1 tile, so the first and last lane of a row segment
+// may load their d1 neighbour from the next tile (n1 <= 256 leaves that code,
+// and its registers, out).  Every lane of the block must call it.
+constexpr int SJ4 = 4, SKC4 = 8;
+__host__ __device__ inline int mv4_tw(const Dims& d) { return d.n1 <= 64 ? 64 : (d.n1 <= 128 ? 128 : 256); }
+// usable when n1 is a multiple of 4 and ghosted indices fit 32 bits
+inline bool mv4_ok(const Dims& d) { return d.n1 % 4 == 0 && d.G() < ((int64_t)1 << 32); }
+__host__ __device__ inline TileGrid mv4_tiles(const Dims& d) {
+  const int tw = mv4_tw(d), rows = 4 * (256 / tw) * SJ4;  // d2 rows per block
+  return TileGrid{(d.n1 + tw - 1) / tw, (d.n2 + rows - 1) / rows, (d.n3 + SKC4 - 1) / SKC4};
+}
+__host__ __device__ inline int mv4_ntiles(const Dims& d) {
+  const TileGrid t = mv4_tiles(d);
+  return t.gx * t.gy * t.gz;
+}
+__device__ __forceinline__ float4 ld4(const float* __restrict__ a, uint32_t q) {
+  float4 v;
+  __builtin_memcpy(&v, a + q, 16);
+  return v;
+}
+__device__ __forceinline__ void st4(float* __restrict__ a, uint32_t q, float4 v) { __builtin_memcpy(a + q, &v, 16); }
+__device__ __forceinline__ void unpack4(const float4& v, float (&o)[4]) {
+  o[0] = v.x;
+  o[1] = v.y;
+  o[2] = v.z;
+  o[3] = v.w;
+}
+
+
+template <int OP, bool EDGE, class LD, class LD1, class EPI>
+__device__ __forceinline__ void mv4_tile(const Dims& d, int tile, LD&& ld, LD1&& ld1, const float* __restrict__ kap,
+                                         const float* __restrict__ pse, float scale, EPI&& epi) {
+  constexpr int SJ = SJ4;
+  const TileGrid tg = mv4_tiles(d);
+  const int nt = tg.gx * tg.gy * tg.gz;
+  const int tw = mv4_tw(d), lpr = tw / 4, rpw = 64 / lpr;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int grp = lane / lpr, x4 = lane % lpr;
+  const float half = 0.5f;
+  const uint32_t sz = (uint32_t)d.g12();
+  const int per = nt / 8;
+  int b = tile;
+  if (b < per * 8) b = (b % 8) * per + b / 8;
+  const int tj = b % tg.gy, ti = (b / tg.gy) % tg.gx, tk = b / (tg.gx * tg.gy);
+  const int i0 = ti * tw + 4 * x4 + 1;
+  const bool active = i0 <= d.n1;
+  const int ia = active ? i0 : d.n1 - 3;  // idle lanes read an interior column
+  const bool left_b = i0 == 1, right_b = i0 + 3 == d.n1;
+  const bool ld_l = EDGE && active && x4 == 0 && i0 > 1, ld_r = EDGE && active && x4 == lpr - 1 && i0 + 3 < d.n1;
+  const int j0 = ((tj * 4 + wv) * rpw + grp) * SJ + 1;
+  const int k0 = tk * SKC4 + 1, k1 = min(k0 + SKC4 - 1, d.n3);
+  const bool down = tk & 1;  // odd tiles march downward (see sweep_ld)
+  const int kstart = down ? k1 : k0, kdir = down ? -1 : 1;
+  const uint32_t step = down ? 0u - sz : sz;
+  int jr[SJ];
+  uint32_t rc[SJ];
+#pragma unroll
+  for (int t = 0; t < SJ; t++) {
+    jr[t] = min(j0 + t, d.n2 + 1);
+    rc[t] = (uint32_t)d.gi(ia, jr[t], kstart);
+  }
+  const int jlo = min(j0 - 1, d.n2 + 1), jhi = min(j0 + SJ, d.n2 + 1);
+  const uint32_t clo = (uint32_t)d.gi(ia, jlo, kstart);
+  const uint32_t chi = (uint32_t)d.gi(ia, jhi, kstart);
+  float4 pc[SJ], kc[SJ];
+  float fb[SJ][4];  // flux through the face between plane k and the plane behind it
+  {
+    const bool bnd = down ? kstart == d.n3 : kstart == 1;
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      pc[t] = ld(rc[t], ia, jr[t], kstart);
+      kc[t] = ld4(kap, rc[t]);
+      float pcv[4], kcv[4], pbv[4], kbv[4];
+      unpack4(pc[t], pcv);
+      unpack4(kc[t], kcv);
+      unpack4(ld(rc[t] - step, ia, jr[t], kstart - kdir), pbv);
+      unpack4(ld4(kap, rc[t] - step), kbv);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        // upward: f5 (x - pm); downward: f6 (pz - x); pm / pz = x on the boundary plane
+        const float f = (kbv[q] + kcv[q]) * half;
+        const float nb = bnd ? pcv[q] : pbv[q];
+        fb[t][q] = down ? f * (nb - pcv[q]) : f * (pcv[q] - nb);
+      }
+    }
+  }
+  uint32_t off = 0;
+  const int nk = k1 - k0 + 1;
+  for (int kk = 0; kk < nk; kk++, off += step) {
+    const int k = down ? k1 - kk : k0 + kk;
+    float4 pa[SJ], ka[SJ];
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      pa[t] = ld(rc[t] + off + step, ia, jr[t], k + kdir);
+      ka[t] = ld4(kap, rc[t] + off + step);
+    }
+    const float4 hlp = ld(clo + off, ia, jlo, k), hlk = ld4(kap, clo + off);
+    const float4 hhp = ld(chi + off, ia, jhi, k), hhk = ld4(kap, chi + off);
+    float psv[SJ][4];
+    if (Rule<OP>::diag) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) unpack4(ld4(pse, rc[t] + off), psv[t]);
+    }
+    float elp[SJ], elk[SJ], erp[SJ], erk[SJ];
+#pragma unroll
+    for (int t = 0; t < SJ; t++) elp[t] = elk[t] = erp[t] = erk[t] = 0.0f;
+    if (EDGE && ld_l) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        elp[t] = ld1(rc[t] + off - 1, ia - 1, jr[t], k);
+        elk[t] = kap[rc[t] + off - 1];
+      }
+    }
+    if (EDGE && ld_r) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        erp[t] = ld1(rc[t] + off + 4, ia + 4, jr[t], k);
+        erk[t] = kap[rc[t] + off + 4];
+      }
+    }
+    const bool kbnd = down ? k == 1 : k == d.n3;  // the face ahead is a boundary face
+    // d2 faces: jf[t] lies between rows j0 + t - 1 and j0 + t
+    float jf[SJ + 1][4];
+#pragma unroll
+    for (int t = 0; t <= SJ; t++) {
+      float plv[4], klv[4], puv[4], kuv[4];
+      unpack4(t == 0 ? hlp : pc[t - 1], plv);
+      unpack4(t == 0 ? hlk : kc[t - 1], klv);
+      unpack4(t == SJ ? hhp : pc[t], puv);
+      unpack4(t == SJ ? hhk : kc[t], kuv);
+      const int jl = j0 + t - 1;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float f = (klv[q] + kuv[q]) * half;
+        float lo = plv[q], hi = puv[q];
+        if (jl == 0) lo = Rule<OP>::lo_zero ? 0.0f : hi;     // below row 1
+        if (jl == d.n2) hi = Rule<OP>::hi_zero ? 0.0f : lo;  // above row n2
+        jf[t][q] = f * (hi - lo);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      const int j = j0 + t;
+      float pv[4], kv[4], pav[4], kav[4];
+      unpack4(pc[t], pv);
+      unpack4(kc[t], kv);
+      unpack4(pa[t], pav);
+      unpack4(ka[t], kav);
+      float pL = lane_shift<true>(pv[3]), kL = lane_shift<true>(kv[3]);
+      float pR = lane_shift<false>(pv[0]), kR = lane_shift<false>(kv[0]);
+      if (EDGE && ld_l) {
+        pL = elp[t];
+        kL = elk[t];
+      }
+      if (EDGE && ld_r) {
+        pR = erp[t];
+        kR = erk[t];
+      }
+      // d1 faces: xf[q] lies between voxels q - 1 and q
+      float xf[5];
+#pragma unroll
+      for (int q = 0; q <= 4; q++) {
+        const float kl = q == 0 ? kL : kv[q - 1], kr = q == 4 ? kR : kv[q];
+        float pl = q == 0 ? pL : pv[q - 1], pr = q == 4 ? pR : pv[q];
+        if (q == 0 && left_b) pl = pr;
+        if (q == 4 && right_b) pr = pl;
+        xf[q] = ((kl + kr) * half) * (pr - pl);
+      }
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float xv = pv[q];
+        const float f = (kav[q] + kv[q]) * half;
+        const float nb = kbnd ? xv : pav[q];
+        const float fa = down ? f * (xv - nb) : f * (nb - xv);
+        const float fup = down ? fb[t][q] : fa, flo = down ? fa : fb[t][q];
+        float sv = xf[q + 1] - xf[q];
+        sv = sv + jf[t + 1][q];
+        sv = sv - jf[t][q];
+        sv = sv + fup;
+        sv = sv - flo;
+        float v;
+        if (Rule<OP>::diag)
+          v = psv[t][q] * xv - scale * sv;
+        else
+          v = scale * sv;
+        o[q] = v;
+        fb[t][q] = fa;
+      }
+      if (active && j <= d.n2) epi(t, j, k, rc[t] + off, pv, o);
+    }
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      pc[t] = pa[t];
+      kc[t] = ka[t];
+    }
+  }
+}
+
+// dinv = 1 / diag(A_hom) (FP64), stored as T (uint16_t: BF16).  Mirror
+// faces do not contribute; zero-Dirichlet faces do.  Interior of a ghosted
+// array.
+template <int OP, typename T>
+__global__ __launch_bounds__(256) void k_dinv(Dims d, const double* __restrict__ kap, const double* __restrict__ pse,
+                                              double scale, T* __restrict__ dinv) {
+  const ColIdx ci = col_idx(d);
+  const int64_t sx = d.g1(), sz = d.g12();
+  for_col(d, ci, [&](int64_t c, int64_t, int k) {
+    const double kc = kap[c];
+    double s = 0.0;
+    if (ci.i > 1) s += (kap[c - 1] + kc) * 0.5;
+    if (ci.i < d.n1) s += (kap[c + 1] + kc) * 0.5;
+    if (ci.j > 1 || Rule<OP>::lo_zero) s += (kap[c - sx] + kc) * 0.5;
+    if (ci.j < d.n2 || Rule<OP>::hi_zero) s += (kap[c + sx] + kc) * 0.5;
+    if (k > 1) s += (kap[c - sz] + kc) * 0.5;
+    if (k < d.n3) s += (kap[c + sz] + kc) * 0.5;
+    const double dg = Rule<OP>::diag ? pse[c] + scale * s : -scale * s;
+    if constexpr (std::is_same<T, uint16_t>::value)
+      dinv[c] = f32_to_bf16((float)(1.0 / dg));
+    else
+      dinv[c] = (T)(1.0 / dg);
+  });
+}
+
 }  // namespace elsim

@@ -17,7 +17,7 @@ MODULE elsim_gpu
          d0, tc1, tc2, dab_c0, dab_c1, dab_c2, dab_c3, kap_scale, i0_scale, kfw_cref, bvp, bve, vsr, trgi, &
          cne_target, rtol, crt_e, crt_p, crt_l, outer_tol, inner_rtol
       INTEGER(c_int32_t) :: mass_corr_period, skip_cne_first, stop_rule, max_it_e, max_it_p, max_it_l, &
-         max_outer, ws_mode, cne_rule, reserved_
+         max_outer, ws_mode, cne_rule, ws_order
    END TYPE elsim_sbm_params
 
    TYPE, BIND(C) :: elsim_step_stats

@@ -115,7 +115,9 @@ typedef struct elsim_sbm_params {
                                potentials; 1 = at rtol ||r0||, r0 = the residual of the last step's CnE,
                                i.e. the step's change is resolved to rtol (CnE has no warm start, so this
                                does not depend on a start policy) */
-  int32_t reserved_;        /* zero; keeps the struct free of implicit padding */
+  int32_t ws_order;         /* terms of the extrapolated start of phi_P: 0 = all (linear extrapolation and
+                               the Galerkin steps along the second and third differences of its last
+                               changes), 1 = linear only, 2 = up to the second difference */
 } elsim_sbm_params;
 
 /* Warm start of a step's coupling loop (SBM:624-812), from the third step
@@ -355,6 +357,15 @@ int elsim_dev_matvec_f64(int op, int n1, int n2, int n3, const double* p, const 
                          const double* pse, double scale, double* out, double* dot);
 int elsim_dev_matvec_f32(int op, int n1, int n2, int n3, const float* p, const float* kap,
                          const float* pse, float scale, float* out, double* dot);
+/* Settings of ELSIM_MODE_MIXED_MG, library-wide: the damped-Jacobi weight
+ * omega of its smoother (0 < omega < 1), the weight oc of its coarse
+ * correction (0 < oc <= 4) and delta, the reduction of the FP32 residual
+ * between FP64 reliable updates (0 < delta < 1; also used for the CnE solve
+ * of a step in that mode).  Defaults 0.7, 1.5, 1e-3; a hierarchy is rebuilt
+ * when omega changes.  No counterpart in the reference (its CG is
+ * unpreconditioned). */
+int elsim_mg_config(double omega, double oc, double delta);
+int elsim_mg_config_get(double* omega, double* oc, double* delta);
 /* One V-cycle of the ELSIM_MODE_MIXED_MG preconditioner, z = M^-1 r, on
  * device arrays: kap (and pse for ELSIM_OP_CNE) FP64 ghosted, scale the
  * operator's dt/dx^2 or 1/dx^2; r and z FP32 ghosted (r's ghosts zero, z's are
@@ -370,8 +381,10 @@ int elsim_dev_mg_apply(int op, int n1, int n2, int n3, const double* kap, const 
 enum {
   ELSIM_PROF_OP32 = 0, ELSIM_PROF_UPD32 = 1, ELSIM_PROF_DIR32 = 2,
   ELSIM_PROF_OP64 = 3, ELSIM_PROF_UPD64 = 4, ELSIM_PROF_DIR64 = 5,
-  ELSIM_PROF_MG = 6, /* the V-cycle of ELSIM_MODE_MIXED_MG, all its kernels */
-  ELSIM_PROF_N = 7
+  ELSIM_PROF_MG = 6,      /* the V-cycle of ELSIM_MODE_MIXED_MG, all its kernels */
+  ELSIM_PROF_MG_DOWN = 7, /* its fine-level down-sweep (r - A x1, restricted) */
+  ELSIM_PROF_MG_UP = 8,   /* its fine-level up-sweep (prolongation, post-smoothing) */
+  ELSIM_PROF_N = 9
 };
 typedef struct elsim_kernel_prof {
   int32_t count[ELSIM_PROF_N];

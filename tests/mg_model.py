@@ -8,20 +8,13 @@ zero ghost rule (Dirichlet data); further levels by the same sums; the
 coarsest level (<= 64 cells) solved exactly; damped Jacobi (omega) before and
 after the coarse correction (weight oc).  The fine level uses the CG's own
 operator with sign sigma (A = sigma A_hat: -1 for the potentials, +1 for CnE)
-and the Jacobi inverse rounded to BF16.  FP64 throughout.
+and the Jacobi inverse rounded to FP32.  FP64 throughout otherwise.
 """
 import numpy as np
 
 OP_CNE, OP_PHP, OP_PHE, OP_POT_DD = 0, 1, 2, 3
 RULES = {OP_CNE: (False, False, True), OP_PHP: (False, True, False), OP_PHE: (True, False, False),
          OP_POT_DD: (True, True, False)}  # lo_zero, hi_zero, diag
-
-
-def bf16(x):
-    """Round float32 values to BF16 (nearest even), returned as float64."""
-    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
-    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
-    return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
 def fine_parts(op, kap, pse, scale):
@@ -154,8 +147,8 @@ class VCycle:
         self.scale = scale
         cs, ms = fine_parts(op, kap, pse, scale)
         self.dg0 = diag_of(cs, ms)
-        # the CG's BF16 Jacobi inverse of A (= sigma / dg0)
-        self.dinv = bf16((self.sigma / self.dg0).astype(np.float32))
+        # the FP32 Jacobi inverse of A (= sigma / dg0) that the multigrid path keeps
+        self.dinv = (self.sigma / self.dg0).astype(np.float32).astype(np.float64)
         self.levels = []
         while True:
             cs, ms = coarsen(cs, ms)
