@@ -126,16 +126,17 @@ KERNEL_NAMES = {"op32": "k_matvec4 (7-point operator, 4 voxels a lane, + p.Ap)",
                 "mg_up": "k_mg_up4 (x2 = x1 + oc P x_c, z = x2 + omega dinv (r - A x2))"}
 
 
-def profiled_step(E, s, mode):
-    """One more step with event brackets around the first CG iteration of
+def profiled_step(E, s, mode, nsteps=3):
+    """A few more steps with event brackets around the first CG iteration of
     every batch: mean in-loop duration of the operator, update and direction
-    kernels."""
+    kernels (and the V-cycle and its fine-level sweeps)."""
     L = E.lib()
     P = E._lib
     E.check(L.elsim_prof_enable(1))
     prof = P.KernelProf()
     E.check(L.elsim_prof_read(C.byref(prof)))  # reset
-    s.step(mode)
+    for _ in range(nsteps):
+        s.step(mode)
     E.check(L.elsim_prof_read(C.byref(prof)))
     E.check(L.elsim_prof_enable(0))
     keys = ["op32", "upd32", "dir32", "op64", "upd64", "dir64", "vcycle", "mg_down", "mg_up"]

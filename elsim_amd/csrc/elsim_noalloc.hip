@@ -61,6 +61,10 @@ extern "C" int elsim_dev_solve_pot_noalloc(int n1, int n2, int n3, double dx, co
                                            int* iters, double* relres) {
   using namespace elsim;
   ELSIM_TRY(pot_args_check(n1, n2, n3, dx, kap, rhs, phi, bc_lo, bc_hi, mode, rtol, maxit));
+  if (mode == ELSIM_MODE_MIXED_MG) {
+    set_error("allocation-free solve: the multigrid mode builds its hierarchy in allocated memory (modes 0-2 only)");
+    return ELSIM_ERR_ARG;
+  }
   const Dims d{n1, n2, n3};
   if (d.G() > kMaxG) {
     set_error("allocation-free solve: %lld ghosted elements exceed the maximum of %lld", (long long)d.G(),

@@ -3,9 +3,10 @@ axis within 15 degrees of the through-thickness axis d2) with a random-walk
 tunnel, connectivity clean-up by component labelling (the reference's
 RM_UCNCT_3D step: solid not connected to the current collector and
 electrolyte not connected to the separator are removed), then coupled
-electrochemical steps with the mixed-precision solver and the learned warm
-start.  Reports per-step iterations and times, the device memory footprint
-and the in-loop HBM rates of the CG kernels.
+electrochemical steps with the mixed-precision solver (multigrid-preconditioned,
+or Jacobi-preconditioned with --solver jacobi) and the learned warm start.
+Reports per-step iterations and times, the device memory footprint and the
+in-loop HBM rates of the CG kernels.
 
     python -m elsim_amd.scaleup --n 512 --steps 20 --report scaleup.json
 """
@@ -21,8 +22,10 @@ import elsim_amd as E
 from elsim_amd import nn as NN
 from elsim_amd._lib import F_CNP, KernelProf
 
-# algorithmic bytes per voxel (the mixed solver keeps its Jacobi inverse in BF16)
-KERNEL_BYTES = {"op32": 12, "upd32": 14, "dir32": 22}
+# algorithmic bytes per voxel: the Jacobi-preconditioned solver keeps its
+# inverse in BF16; the multigrid path's update and direction move 20 B, its
+# fine-level sweeps 12.5 and 20.5 B (with the level-1 vectors)
+KERNEL_BYTES = {"op32": 12, "upd32": 14, "dir32": 22, "upd_mg": 20, "dir_mg": 20, "mg_down": 12.5, "mg_up": 20.5}
 
 
 def log(*a):
@@ -70,6 +73,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--ws", choices=["plain", "extrap", "learned"], default="learned")
     ap.add_argument("--weights", default=None)
+    ap.add_argument("--solver", choices=["mg", "jacobi"], default="mg")
     ap.add_argument("--report", default=None)
     a = ap.parse_args(argv)
     E.check(E.lib().elsim_init(0), "elsim_init")
@@ -83,7 +87,10 @@ def main(argv=None):
     log(f"{n}^3: {ns} ellipsoids, solid {frac:.3f}, generated in {t_gen:.1f} s; clean-up {cstats}")
     wsm = {"plain": E.WS_PLAIN, "extrap": E.WS_EXTRAP, "learned": E.WS_LEARNED}[a.ws]
     # as the 256^3 benchmark: the CnE solve resolves the step's change to rtol
-    s = E.Sbm(sdf, -sdf, E.synthetic_params(ws_mode=wsm, cne_rule=1), E.tables())
+    mg = a.solver == "mg"
+    mode = E.MODE_MIXED_MG if mg else E.MODE_MIXED
+    # the multigrid path extrapolates with the second-difference Galerkin step only (as the benchmark)
+    s = E.Sbm(sdf, -sdf, E.synthetic_params(ws_mode=wsm, cne_rule=1, ws_order=2 if mg else 0), E.tables())
     s.set(F_CNP, E.initial_cnp(n, a.seed))
     net = None
     if a.ws == "learned":
@@ -92,7 +99,7 @@ def main(argv=None):
     steps = []
     for k in range(a.steps):
         t1 = time.perf_counter()
-        st = s.step(E.MODE_MIXED)
+        st = s.step(mode)
         E.check(E.lib().elsim_synchronize())
         d = st.as_dict()
         d["wall_s"] = time.perf_counter() - t1
@@ -105,18 +112,22 @@ def main(argv=None):
     prof = KernelProf()
     E.check(L.elsim_prof_enable(1))
     E.check(L.elsim_prof_read(C.byref(prof)))
-    s.step(E.MODE_MIXED)
+    for _ in range(2):
+        s.step(mode)
     E.check(L.elsim_prof_read(C.byref(prof)))
     E.check(L.elsim_prof_enable(0))
     N = n ** 3
     kern = {}
-    for i, k in enumerate(("op32", "upd32", "dir32")):
+    slots = (("op32", 0), ("upd_mg" if mg else "upd32", 1), ("dir_mg" if mg else "dir32", 2), ("mg_down", 7),
+             ("mg_up", 8))
+    for k, i in slots:
         if prof.count[i]:
             us = prof.mean_us(i)
             kern[k] = {"us": us, "GBps": KERNEL_BYTES[k] * N / (us * 1e-6) / 1e9, "launches": int(prof.count[i])}
     log(f"  in-loop kernels: {kern}")
     rep = {"config": {"grid": [n, n, n], "shapes": ns, "solid_fraction": frac, "semi_axes": [8, 24], "cone_deg": 15,
-                      "tunnel_radius": 3, "seed": a.seed, "warm_start": a.ws, "mode": "fp32 jacobi-pcg + fp64 reliable",
+                      "tunnel_radius": 3, "seed": a.seed, "warm_start": a.ws,
+                      "mode": ("fp32 multigrid-pcg" if mg else "fp32 jacobi-pcg") + " + fp64 reliable updates",
                       "cne_rule": "relative to the step's change (||r0||)"},
            "generate_s": t_gen, "cleanup": cstats, "steps": steps,
            "s_per_step_after_first": float(np.mean([x["wall_s"] for x in steps[1:]])) if len(steps) > 1 else None,

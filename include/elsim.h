@@ -333,7 +333,8 @@ int elsim_dev_solve_pot(int n1, int n2, int n3, double dx, const double* kap, co
  * device array of the library instead of allocated buffers.  Same arguments
  * and results (bit-identical).  Grids up to ELSIM_NOALLOC_MAX_G ghosted
  * elements (n1+2)(n2+2)(n3+2), i.e. 64^3 interior; larger grids are
- * ELSIM_ERR_ARG.  elsim_init must have run (it sets the array up; this entry
+ * ELSIM_ERR_ARG, and so is ELSIM_MODE_MIXED_MG (its hierarchy is built in
+ * allocated memory).  elsim_init must have run (it sets the array up; this entry
  * point allocates nothing and does not initialise the library).  The array is
  * one per GPU and shared by all calls on it: calls that overlap on one GPU
  * race. */

@@ -3,7 +3,8 @@
 axis d2) with a random-walk tunnel of radius 3, cleaned of disconnected solid
 and electrolyte by the GPU labeller (the RM_UCNCT_3D step), then coupled
 electrochemical steps (SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90:397-840)
-with FP32 inner iterations, FP64 reliable updates and the learned warm start,
+with the benchmark's fast path (FP32 CG preconditioned by a multigrid V-cycle,
+FP64 reliable updates, the learned warm start),
 the CnE solve resolving each step's change (as the 256^3 benchmark).
 Every solve of every step reaches relres 1e-8 with an FP64 true residual, the
 fields stay finite, Xrf changes only by the step's reaction, and the device
@@ -26,13 +27,13 @@ def test_c4_512_coupled_steps(gpu):
     sdf, ns, frac = E.ellipsoid_pack(N, 3, 8.0, 24.0, cone_deg=15.0, target=0.5, overlap=0.3, dcut=12.0)
     sdf = E.carve_tunnel(sdf, 3, radius=3.0)
     sdf, cstats = scaleup.clean(sdf, 12.0)
-    s = E.Sbm(sdf, -sdf, E.synthetic_params(ws_mode=E.WS_LEARNED, cne_rule=1), E.tables())
+    s = E.Sbm(sdf, -sdf, E.synthetic_params(ws_mode=E.WS_LEARNED, cne_rule=1, ws_order=2), E.tables())
     s.set(F_CNP, E.initial_cnp(N, 3))
     net = NN.WarmStartNet.load(NN.DEFAULT_WEIGHTS)
     s.set_warm_start(net)
     xrf0 = s.scalars()["xrf"]
     for k in range(STEPS):
-        st = s.step(E.MODE_MIXED)
+        st = s.step(E.MODE_MIXED_MG)
         print(f"step {k}: phP {st.it_php} phE {st.it_phe} CnE {st.it_cne} outer {st.n_outer} "
               f"relres {st.relres_php:.1e} {st.relres_phe:.1e} {st.seconds:.2f} s", flush=True)
         assert st.status == 0, st.as_dict()

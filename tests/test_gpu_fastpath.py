@@ -1,5 +1,6 @@
-"""Field error of the benchmark's fast path (FP32 inner iterations with FP64
-reliable updates, extrapolated + learned warm start) on a 128^3 sphere pack
+"""Field error of the benchmark's fast path (FP32 inner iterations preconditioned
+by a multigrid V-cycle, FP64 reliable updates, extrapolated + learned warm
+start) and of the Jacobi-preconditioned mixed path on a 128^3 sphere pack
 (radii 6-16) over six coupled steps.  Every solve ends at relres <= 1e-8 in
 FP64 (the CnE solves relative to the step's change).  The fields are compared
 with a tight FP64 Jacobi-PCG run of the same steps (every solve to relres
@@ -35,10 +36,12 @@ def runs(gpu):
     sdf, _, _ = E.sphere_pack(N, SEED, 6.0, 16.0, target=target, hardcore=0.85, connect=True)
     net = NN.WarmStartNet.load(NN.DEFAULT_WEIGHTS)
     out = {}
-    for key, mode, ws, kw in (("fast", E.MODE_MIXED, E.WS_LEARNED, {}),
-                              ("fp64_learned", E.MODE_FP64_PCG, E.WS_LEARNED, {}),
+    for key, mode, ws, kw in (("fast", E.MODE_MIXED_MG, E.WS_LEARNED, {"ws_order": 2}),
+                              ("fp64_learned", E.MODE_FP64_PCG, E.WS_LEARNED, {"ws_order": 2}),
                               ("fp64_plain", E.MODE_FP64_PCG, E.WS_PLAIN, {}),
                               ("mixed_plain", E.MODE_MIXED, E.WS_PLAIN, {}),
+                              ("mg_plain", E.MODE_MIXED_MG, E.WS_PLAIN, {}),
+                              ("jacobi_learned", E.MODE_MIXED, E.WS_LEARNED, {}),
                               ("tight", E.MODE_FP64_PCG, E.WS_PLAIN, TIGHT)):
         s = E.Sbm(sdf, -sdf, E.synthetic_params(ws_mode=ws, cne_rule=1, **kw), E.tables())
         s.set(F_CNP, E.initial_cnp(N, SEED))
@@ -87,5 +90,9 @@ def test_fast_path_against_converged_fields(runs):
 def test_plain_start_within_the_coupling_rule(runs):
     cvgp = [x.cvgp for x in runs["fp64_plain"][0]]
     assert max(cvgp) > 1e-7  # the loop exits on CvgL with CvgP still this large
-    for key in ("fp64_plain", "mixed_plain"):
+    for key in ("fp64_plain", "mixed_plain", "mg_plain"):
         assert max(errors(runs, key).values()) <= 1e-5
+
+
+def test_jacobi_path_against_converged_fields(runs):
+    assert max(errors(runs, "jacobi_learned").values()) <= 1e-6

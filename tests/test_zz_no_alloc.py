@@ -53,3 +53,10 @@ def test_noalloc_rejects_grids_above_its_maximum(gpu):
     rc, _, _, _ = solve("elsim_dev_solve_pot_noalloc", k2, r2, p2, E.MODE_MIXED)
     assert rc == ERR_ARG
     assert "exceed the maximum" in E.lib().elsim_last_error().decode()
+
+
+def test_noalloc_refuses_the_multigrid_mode(gpu):
+    kap, rhs, phi, _ = E.potential_problem(24, 5, rmin=3.0, rmax=6.0)
+    rc, _, _, _ = solve("elsim_dev_solve_pot_noalloc", kap, rhs, phi, E.MODE_MIXED_MG)
+    assert rc == ERR_ARG
+    assert "multigrid" in E.lib().elsim_last_error().decode()
