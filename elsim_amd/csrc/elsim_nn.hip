@@ -102,6 +102,70 @@ __device__ __forceinline__ void stage_slab(int nq, int rs, int x0, int y0, int z
   }
 }
 
+// Output of a convolution tile.  D[row = 4 lq + i][col = lr] of a wave's
+// MFMA tile is voxel x = 4 lq + i of its row, channel t*16 + lr.  MODE 0
+// (bf16, bias, ReLU) goes through LDS when the tile fits the block's staging
+// buffer (LDS_N elements): written as [voxel][channel], then stored as
+// 16-byte rows of 8 channels instead of 2-byte scattered stores (E0 at 256^3:
+// 1003 -> 844 us).  Every wave must have finished reading the staged input.
+template <int COUT, int MT, int MODE, int LDS_N>
+__device__ __forceinline__ void conv_epilogue(const f4v (&acc)[MT][COUT / 16], int x0, int y0, int z0, int n1,
+                                              int n2, int n3, const float* __restrict__ bias, int relu,
+                                              void* __restrict__ out, View ov, uint16_t* __restrict__ lds) {
+  constexpr int NT = COUT / 16;
+  constexpr int OS = COUT + 8;  // LDS row stride of the output tile (elements)
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  if constexpr (MODE == 0 && CONV_TX * CONV_TY * MT * OS <= LDS_N) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      const int row = wave * MT + m;
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const int co = t * 16 + lr;
+          float v = acc[m][t][i] + bias[co];
+          if (relu) v = fmaxf(v, 0.f);
+          lds[(row * CONV_TX + 4 * lq + i) * OS + co] = f2bf(v);
+        }
+    }
+    __syncthreads();
+    constexpr int CH = COUT / 8;  // 16-byte chunks per voxel
+    for (int e = tid; e < CONV_TX * CONV_TY * MT * CH; e += 256) {
+      const int vt = e / CH, c8 = e % CH;
+      const int x = x0 + vt % CONV_TX, row = vt / CONV_TX, y = y0 + row % CONV_TY, z = z0 + row / CONV_TY;
+      if (x >= n1 || y >= n2 || z >= n3) continue;
+      const int64_t vox = x + (int64_t)n1 * (y + (int64_t)n2 * z);
+      *(uint4*)((uint16_t*)out + vox * ov.cs + ov.co + 8 * c8) = *(const uint4*)(lds + vt * OS + 8 * c8);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      const int row = wave * MT + m, y = y0 + row % CONV_TY, z = z0 + row / CONV_TY;
+      if (y >= n2 || z >= n3) continue;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int x = x0 + 4 * lq + i;
+        if (x >= n1) continue;
+        const int64_t vox = x + (int64_t)n1 * (y + (int64_t)n2 * z);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const int co = t * 16 + lr;
+          if (MODE == 0) {
+            float v = acc[m][t][i] + bias[co];
+            if (relu) v = fmaxf(v, 0.f);
+            ((uint16_t*)out)[vox * ov.cs + ov.co + co] = f2bf(v);
+          } else {
+            ((float*)out)[vox * ov.cs + ov.co + co] = acc[m][t][i];
+          }
+        }
+      }
+    }
+  }
+}
+
 // MODE 0: out bf16 = relu?(acc + bias); MODE 1: out FP32 = acc.
 template <int COUT, int MT, int MODE>
 __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
@@ -171,29 +235,7 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
       }
     }
   }
-  // D[row = 4 lq + i][col = lr]: voxel x = 4 lq + i of the wave's row, channel t*16 + lr
-#pragma unroll
-  for (int m = 0; m < MT; m++) {
-    const int row = wave * MT + m, y = y0 + row % CONV_TY, z = z0 + row / CONV_TY;
-    if (y >= n2 || z >= n3) continue;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int x = x0 + 4 * lq + i;
-      if (x >= n1) continue;
-      const int64_t vox = x + (int64_t)n1 * (y + (int64_t)n2 * z);
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const int co = t * 16 + lr;
-        if (MODE == 0) {
-          float v = acc[m][t][i] + bias[co];
-          if (relu) v = fmaxf(v, 0.f);
-          ((uint16_t*)out)[vox * ov.cs + ov.co + co] = f2bf(v);
-        } else {
-          ((float*)out)[vox * ov.cs + ov.co + co] = acc[m][t][i];
-        }
-      }
-    }
-  }
+  conv_epilogue<COUT, MT, MODE, HN*(CS_MAX + LDS_PAD)>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds);
 }
 
 // The same convolution for 16-channel slabs (input channels not a multiple
@@ -257,28 +299,7 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
         for (int m = 0; m < MT; m++) acc[m][t] = mfma32(a[m], bw[pr][t], acc[m][t]);
     }
   }
-#pragma unroll
-  for (int m = 0; m < MT; m++) {
-    const int row = wave * MT + m, y = y0 + row % CONV_TY, z = z0 + row / CONV_TY;
-    if (y >= n2 || z >= n3) continue;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int x = x0 + 4 * lq + i;
-      if (x >= n1) continue;
-      const int64_t vox = x + (int64_t)n1 * (y + (int64_t)n2 * z);
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const int co = t * 16 + lr;
-        if (MODE == 0) {
-          float v = acc[m][t][i] + bias[co];
-          if (relu) v = fmaxf(v, 0.f);
-          ((uint16_t*)out)[vox * ov.cs + ov.co + co] = f2bf(v);
-        } else {
-          ((float*)out)[vox * ov.cs + ov.co + co] = acc[m][t][i];
-        }
-      }
-    }
-  }
+  conv_epilogue<COUT, MT, MODE, HN * RS>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds);
 }
 
 template <int MODE, int COUT, int MT>
@@ -427,21 +448,33 @@ __global__ void k_colsum_reduce(int ng, int C, const float* __restrict__ part, f
 // ------------------------------------------------- pooling and upsampling --
 
 // coarse[v][c] = bf16(mean of the 8 children), children from a channel view.
+// One thread per coarse voxel and 8-channel chunk: eight 16-byte child loads
+// and one 16-byte store (C, the view's stride and offset are multiples of 8);
+// the children are summed in the same order for every channel.
 __global__ void k_pool(int n1, int n2, int n3, int C, const uint16_t* __restrict__ fine, View fv,
                        uint16_t* __restrict__ coarse) {
-  const int m1 = n1 / 2, m2 = n2 / 2, m3 = n3 / 2;
-  const int64_t n = (int64_t)m1 * m2 * m3 * C;
+  const int m1 = n1 / 2, m2 = n2 / 2, m3 = n3 / 2, C8 = C / 8;
+  const int64_t n = (int64_t)m1 * m2 * m3 * C8;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    const int64_t vc = e / C;
+    const int c8 = (int)(e % C8);
+    const int64_t vc = e / C8;
     const int x = (int)(vc % m1), y = (int)((vc / m1) % m2), z = (int)(vc / ((int64_t)m1 * m2));
-    float s = 0.f;
+    uint4 ch[8];
+#pragma unroll
     for (int q = 0; q < 8; q++) {
       const int fx = 2 * x + (q & 1), fy = 2 * y + ((q >> 1) & 1), fz = 2 * z + (q >> 2);
       const int64_t vf = fx + (int64_t)n1 * (fy + (int64_t)n2 * fz);
-      s += bf2f(fine[vf * fv.cs + fv.co + c]);
+      ch[q] = *(const uint4*)(fine + vf * fv.cs + fv.co + 8 * c8);
     }
-    coarse[vc * C + c] = f2bf(s * 0.125f);
+    uint16_t r[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      float s = 0.f;
+#pragma unroll
+      for (int q = 0; q < 8; q++) s += bf2f(((const uint16_t*)&ch[q])[k]);
+      r[k] = f2bf(s * 0.125f);
+    }
+    *(uint4*)(coarse + vc * C + 8 * c8) = *(const uint4*)r;
   }
 }
 
@@ -738,27 +771,34 @@ struct NnNet {
     const int64_t N0 = (int64_t)a1 * a2 * a3, N1 = N0 / 8, N2 = N1 / 8, N3 = N2 / 8;
     uint16_t *C0 = (uint16_t*)cat0.p, *C1 = (uint16_t*)cat1.p, *C2 = (uint16_t*)cat2.p;
     const float* B = P();
-    // encoder
-    ELSIM_TRY(conv<0>(a1, a2, a3, x0, 16, View{16, 0}, WT(L_E0), B + b_off[L_E0], 1, 16, C0, View{48, 32}, s));
-    k_pool<<<nblk_cap(N1 * 16), 256, 0, s>>>(a1, a2, a3, 16, C0, View{48, 32}, (uint16_t*)p1.p);
-    ELSIM_TRY(conv<0>(m1, m2, m3, (uint16_t*)p1.p, 16, View{16, 0}, WT(L_E1), B + b_off[L_E1], 1, 32, C1,
-                      View{96, 64}, s));
-    k_pool<<<nblk_cap(N2 * 32), 256, 0, s>>>(m1, m2, m3, 32, C1, View{96, 64}, (uint16_t*)p2.p);
-    ELSIM_TRY(conv<0>(q1, q2, q3, (uint16_t*)p2.p, 32, View{32, 0}, WT(L_E2), B + b_off[L_E2], 1, 64, C2,
-                      View{128, 64}, s));
-    k_pool<<<nblk_cap(N3 * 64), 256, 0, s>>>(q1, q2, q3, 64, C2, View{128, 64}, (uint16_t*)p3.p);
+    // encoder.  The skip halves of the decoder inputs live in cat0/1/2 after
+    // the upsampled halves (channels-last rows of 48, 96, 128); at inference
+    // the upsampled halves are read at half resolution and never stored, so
+    // the skip tensors are kept compact (rows of 16, 32, 64: every cache line
+    // the encoder writes and the pooling and the decoder read is all skip
+    // data) and the decoder addresses them with a negative channel offset.
+    const View s0 = infer ? View{16, 0} : View{48, 32}, s1 = infer ? View{32, 0} : View{96, 64},
+               s2 = infer ? View{64, 0} : View{128, 64};
+    const View i0 = infer ? View{16, -32} : View{48, 0}, i1 = infer ? View{32, -64} : View{96, 0},
+               i2 = infer ? View{64, -64} : View{128, 0};
+    ELSIM_TRY(conv<0>(a1, a2, a3, x0, 16, View{16, 0}, WT(L_E0), B + b_off[L_E0], 1, 16, C0, s0, s));
+    k_pool<<<nblk_cap(N1 * 2), 256, 0, s>>>(a1, a2, a3, 16, C0, s0, (uint16_t*)p1.p);
+    ELSIM_TRY(conv<0>(m1, m2, m3, (uint16_t*)p1.p, 16, View{16, 0}, WT(L_E1), B + b_off[L_E1], 1, 32, C1, s1, s));
+    k_pool<<<nblk_cap(N2 * 4), 256, 0, s>>>(m1, m2, m3, 32, C1, s1, (uint16_t*)p2.p);
+    ELSIM_TRY(conv<0>(q1, q2, q3, (uint16_t*)p2.p, 32, View{32, 0}, WT(L_E2), B + b_off[L_E2], 1, 64, C2, s2, s));
+    k_pool<<<nblk_cap(N3 * 8), 256, 0, s>>>(q1, q2, q3, 64, C2, s2, (uint16_t*)p3.p);
     ELSIM_TRY(conv<0>(r1, r2, r3, (uint16_t*)p3.p, 64, View{64, 0}, WT(L_E3), B + b_off[L_E3], 1, 64, e3.p,
                       View{64, 0}, s));
     // decoder
     const Coarse none{nullptr, 0, View{0, 0}};
     if (!infer) k_up<<<nblk_cap(N2 * 8), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
-    ELSIM_TRY(conv<0>(q1, q2, q3, C2, 128, View{128, 0}, WT(L_D2), B + b_off[L_D2], 1, 64, d2.p, View{64, 0}, s,
+    ELSIM_TRY(conv<0>(q1, q2, q3, C2, 128, i2, WT(L_D2), B + b_off[L_D2], 1, 64, d2.p, View{64, 0}, s,
                       infer ? Coarse{(uint16_t*)e3.p, 64, View{64, 0}} : none));
     if (!infer) k_up<<<nblk_cap(N1 * 8), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
-    ELSIM_TRY(conv<0>(m1, m2, m3, C1, 96, View{96, 0}, WT(L_D1), B + b_off[L_D1], 1, 32, d1.p, View{32, 0}, s,
+    ELSIM_TRY(conv<0>(m1, m2, m3, C1, 96, i1, WT(L_D1), B + b_off[L_D1], 1, 32, d1.p, View{32, 0}, s,
                       infer ? Coarse{(uint16_t*)d2.p, 64, View{64, 0}} : none));
     if (!infer) k_up<<<nblk_cap(N0 * 4), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
-    ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, View{48, 0}, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s,
+    ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, i0, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s,
                       infer ? Coarse{(uint16_t*)d1.p, 32, View{32, 0}} : none));
     const int ob = nblk(N0);
     if (target && ob > kMaxRedBlocks) {
