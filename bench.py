@@ -153,15 +153,16 @@ def mg_settings(E):
 
 def nn_forward_time(E, net, n, reps=5):
     """Seconds per forward pass of the warm-start network at n^3 (HIP
-    events on the library stream), on zero input."""
+    events on the library stream), on zero input, reading input channels
+    0..7 as the SBM warm start does."""
     L = E.lib()
     x = E.DeviceArray((n ** 3 * 16,), np.uint16)
     E.check(L.elsim_memset(x.ptr, 0, x.nbytes))
     pred = E.DeviceArray((n ** 3,), np.float32)
-    net.forward((n, n, n), x, pred)
+    net.forward((n, n, n), x, pred, in_used=8)
     E.check(L.elsim_timer_start())
     for _ in range(reps):
-        net.forward((n, n, n), x, pred)
+        net.forward((n, n, n), x, pred, in_used=8)
     ms = C.c_float()
     E.check(L.elsim_timer_stop(C.byref(ms)))
     return ms.value * 1e-3 / reps

@@ -132,6 +132,7 @@ _SIGS = {
     "elsim_nn_get_params": (I, [C.POINTER(C.c_int64), P]),
     "elsim_nn_get_grads": (I, [C.POINTER(C.c_int64), P]),
     "elsim_nn_forward": (I, [C.POINTER(C.c_int64), I, I, I, P, P]),
+    "elsim_nn_forward_used": (I, [C.POINTER(C.c_int64), I, I, I, P, I, P]),
     "elsim_nn_loss_grad": (I, [C.POINTER(C.c_int64), I, I, I, P, P, P, DP]),
     "elsim_nn_adam": (I, [C.POINTER(C.c_int64), C.c_float]),
     "elsim_nn_train_forward": (I, [C.POINTER(C.c_int64), I, I, I, P, P]),

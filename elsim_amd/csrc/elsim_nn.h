@@ -23,8 +23,9 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 struct NnNet;
 int64_t nn_param_count();
 // pred[v] (FP32) for the input features x (bf16 [voxel][16]); grid dims
-// multiples of 8.
-int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred);
+// multiples of 8.  in_used 8: channels 8..15 of x are zero and are not read
+// (16: all are read).
+int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred, int in_used);
 // The network of a handle issued by elsim_nn_create (nullptr if unknown or
 // destroyed).
 NnNet* nn_lookup(int64_t handle);

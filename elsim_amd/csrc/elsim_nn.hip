@@ -238,20 +238,28 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
   conv_epilogue<COUT, MT, MODE, HN*(CS_MAX + LDS_PAD)>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds);
 }
 
-// The same convolution for 16-channel slabs (input channels not a multiple
-// of 32) and at most 32 output channels: each wave loads the weight
-// fragments of all 27 taps of a slab into registers at once (14 x NT loads in
-// flight, issued before the slab is staged), instead of one load per tap
-// whose latency the tap's few MFMAs cannot hide; two taps per MFMA.
-template <int COUT, int MT, int MODE>
+// The same convolution for 16- or 8-channel slabs (CS; input channels not a
+// multiple of 32, or a layer whose used input channels are 8 of its 16) and
+// at most 32 output channels: each wave loads the weight fragments of all 27
+// taps of a slab into registers at once (14 x NT loads in flight, issued
+// before the slab is staged), instead of one load per tap whose latency the
+// tap's few MFMAs cannot hide.  One 16x16x32 MFMA covers 32 / CS taps: lane
+// group lq holds tap (32 / CS) pair + lq / (CS / 8), channels
+// 8 (lq % (CS / 8)) .. + 7; the taps past the 27th have zero weights.  The
+// slabs cover channels [0, cused) of cin (the weight rows' length); the LDS
+// row stride is 24 elements in both cases, which keeps a 16-lane ds_read_b128
+// free of bank conflicts.
+template <int COUT, int MT, int MODE, int CS>
 __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
-                                                   View iv, Coarse lo, const uint16_t* __restrict__ wt,
+                                                   int cused, View iv, Coarse lo, const uint16_t* __restrict__ wt,
                                                    const float* __restrict__ bias, int relu, void* __restrict__ out,
                                                    View ov) {
-  constexpr int NT = COUT / 16, CS = 16;
+  constexpr int NT = COUT / 16;
   static_assert(NT <= 2, "weight fragments of 27 taps for at most two output tiles");
+  static_assert(CS == 16 || CS == 8, "slabs of 16 or 8 channels");
+  constexpr int TPM = 32 / CS, NP = (27 + TPM - 1) / TPM;  // taps per MFMA, MFMAs per slab
   constexpr int HX = CONV_TX + 2, HY = CONV_TY + 2, HZ = MT + 2, HN = HX * HY * HZ;
-  constexpr int RS = CS + LDS_PAD;
+  constexpr int RS = 24;
   __shared__ __align__(16) uint16_t lds[HN * RS];
   const int tx = (n1 + CONV_TX - 1) / CONV_TX, ty = (n2 + CONV_TY - 1) / CONV_TY;
   const int b = blockIdx.x;
@@ -265,26 +273,23 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[m][t] = f4v{0.f, 0.f, 0.f, 0.f};
 
-  // taps in pairs, one 16x16x32 MFMA per pair: lane group lq holds tap
-  // 2 pair + (lq >> 1), channels 8 (lq & 1) .. + 7; the 28th tap has zero
-  // weights
-  const int tsub = lq >> 1, csub = 8 * (lq & 1);
-  for (int slab = 0; slab < cin; slab += CS) {
-    uint4 bw[14][NT];
+  const int tsub = lq / (CS / 8), csub = 8 * (lq % (CS / 8));
+  for (int slab = 0; slab < cused; slab += CS) {
+    uint4 bw[NP][NT];
 #pragma unroll
-    for (int pr = 0; pr < 14; pr++) {
-      const int tap = 2 * pr + tsub;
+    for (int pr = 0; pr < NP; pr++) {
+      const int tap = TPM * pr + tsub;
 #pragma unroll
       for (int t = 0; t < NT; t++)
         bw[pr][t] = tap < 27 ? *(const uint4*)(wt + (int64_t)(t * 16 + lr) * K + tap * cin + slab + csub)
                              : make_uint4(0, 0, 0, 0);
     }
     __syncthreads();
-    stage_slab<HN, 2>(2, RS, x0, y0, z0, n1, n2, n3, in, iv, lo, slab, lds);
+    stage_slab<HN, CS / 8>(CS / 8, RS, x0, y0, z0, n1, n2, n3, in, iv, lo, slab, lds);
     __syncthreads();
 #pragma unroll
-    for (int pr = 0; pr < 14; pr++) {
-      const int tap = min(2 * pr + tsub, 26);  // the 28th tap reads tap 26's values against zero weights
+    for (int pr = 0; pr < NP; pr++) {
+      const int tap = min(TPM * pr + tsub, 26);  // taps past the 27th read tap 26's values against zero weights
       const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
       uint4 a[MT];
 #pragma unroll
@@ -303,23 +308,38 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
 }
 
 template <int MODE, int COUT, int MT>
-static void conv_launch(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, Coarse lo, const uint16_t* w,
-                        const float* bias, int relu, void* out, View ov, hipStream_t s) {
+static void conv_launch(int n1, int n2, int n3, const uint16_t* in, int cin, int cused, View iv, Coarse lo,
+                        const uint16_t* w, const float* bias, int relu, void* out, View ov, hipStream_t s) {
   const int nb = ((n1 + CONV_TX - 1) / CONV_TX) * ((n2 + CONV_TY - 1) / CONV_TY) * ((n3 + MT - 1) / MT);
   if constexpr (COUT <= 32) {
+    if (cused % 16 != 0) {
+      k_conv3_w27<COUT, MT, MODE, 8><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov);
+      return;
+    }
     if (cin % 32 != 0) {
-      k_conv3_w27<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov);
+      k_conv3_w27<COUT, MT, MODE, 16><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov);
       return;
     }
   }
   k_conv3<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov);
 }
 
+// Input channels: a multiple of 16, or 8.  cused < cin: only input channels
+// [0, cused) are read (the others are known to be zero, their weights
+// contribute nothing).  8 used channels (cin 8, or 8 of 16) run the 8-channel
+// slab kernel, which takes at most 32 output channels.
 template <int MODE>
 static int conv(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, const uint16_t* w, const float* bias,
-                int relu, int cout, void* out, View ov, hipStream_t s, Coarse lo = Coarse{nullptr, 0, View{0, 0}}) {
-  if (cin % 16 || cin <= 0 || iv.cs % 8 || iv.co % 8) {
-    set_error("convolution: input channels %d (stride %d, offset %d) must be multiples of 16 (8)", cin, iv.cs, iv.co);
+                int relu, int cout, void* out, View ov, hipStream_t s, Coarse lo = Coarse{nullptr, 0, View{0, 0}},
+                int cused = 0) {
+  if ((cin % 16 && cin != 8) || cin <= 0 || iv.cs % 8 || iv.co % 8) {
+    set_error("convolution: input channels %d (stride %d, offset %d) must be 8 or multiples of 16 (8)", cin, iv.cs,
+              iv.co);
+    return ELSIM_ERR_ARG;
+  }
+  if (cused == 0) cused = cin;
+  if ((cused != cin && !(cused == 8 && cin == 16)) || (cused == 8 && (cout > 32 || lo.ch))) {
+    set_error("convolution: %d of %d input channels: only 8 of 16, and 8 with at most 32 outputs", cused, cin);
     return ELSIM_ERR_ARG;
   }
   const int cs = cin % 32 == 0 ? 32 : 16;
@@ -328,12 +348,12 @@ static int conv(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, co
     return ELSIM_ERR_ARG;
   }
   switch (cout) {
-    case 16: conv_launch<MODE, 16, 4>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
-    case 32: conv_launch<MODE, 32, 4>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
-    case 48: conv_launch<MODE, 48, 4>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
-    case 64: conv_launch<MODE, 64, 4>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
-    case 96: conv_launch<MODE, 96, 2>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
-    case 128: conv_launch<MODE, 128, 2>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, s); break;
+    case 16: conv_launch<MODE, 16, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
+    case 32: conv_launch<MODE, 32, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
+    case 48: conv_launch<MODE, 48, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
+    case 64: conv_launch<MODE, 64, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
+    case 96: conv_launch<MODE, 96, 2>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
+    case 128: conv_launch<MODE, 128, 2>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
     default: set_error("convolution: %d output channels not supported", cout); return ELSIM_ERR_ARG;
   }
   ELSIM_CHECK_LAUNCH();
@@ -762,7 +782,10 @@ struct NnNet {
   // infer: the activations are not kept for a backward pass, so the
   // decoder reads its upsampled halves at half resolution while staging
   // (no full-resolution copies; the same values, so the same results).
-  int forward(const uint16_t* x0, float* pred, const float* target, double* loss_host, bool infer = false) {
+  // in_used: input channels [in_used, 16) are known to be zero (8: the first
+  // layer runs on 8-channel slabs).
+  int forward(const uint16_t* x0, float* pred, const float* target, double* loss_host, bool infer = false,
+              int in_used = NN_IN_CH) {
     ELSIM_TRY(pack());
     hipStream_t s = stream();
     const int a1 = n1, a2 = n2, a3 = n3;
@@ -781,7 +804,8 @@ struct NnNet {
                s2 = infer ? View{64, 0} : View{128, 64};
     const View i0 = infer ? View{16, -32} : View{48, 0}, i1 = infer ? View{32, -64} : View{96, 0},
                i2 = infer ? View{64, -64} : View{128, 0};
-    ELSIM_TRY(conv<0>(a1, a2, a3, x0, 16, View{16, 0}, WT(L_E0), B + b_off[L_E0], 1, 16, C0, s0, s));
+    ELSIM_TRY(conv<0>(a1, a2, a3, x0, 16, View{16, 0}, WT(L_E0), B + b_off[L_E0], 1, 16, C0, s0, s,
+                      Coarse{nullptr, 0, View{0, 0}}, in_used));
     k_pool<<<nblk_cap(N1 * 2), 256, 0, s>>>(a1, a2, a3, 16, C0, s0, (uint16_t*)p1.p);
     ELSIM_TRY(conv<0>(m1, m2, m3, (uint16_t*)p1.p, 16, View{16, 0}, WT(L_E1), B + b_off[L_E1], 1, 32, C1, s1, s));
     k_pool<<<nblk_cap(N2 * 4), 256, 0, s>>>(m1, m2, m3, 32, C1, s1, (uint16_t*)p2.p);
@@ -927,9 +951,9 @@ struct NnNet {
   }
 };
 
-int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred) {
+int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred, int in_used) {
   ELSIM_TRY(net->ensure(n1, n2, n3, false));
-  return net->forward(x, pred, nullptr, nullptr, true);
+  return net->forward(x, pred, nullptr, nullptr, true, in_used);
 }
 
 }  // namespace elsim
@@ -1020,7 +1044,18 @@ int elsim_nn_get_grads(const int64_t* handle, float* host) {
 int elsim_nn_forward(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, float* pred) {
   NnNet* n = net_of(handle);
   if (!n || !x || !pred) return ELSIM_ERR_ARG;
-  return nn_forward(n, n1, n2, n3, x, pred);
+  return nn_forward(n, n1, n2, n3, x, pred, NN_IN_CH);
+}
+
+int elsim_nn_forward_used(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, int in_used,
+                          float* pred) {
+  NnNet* n = net_of(handle);
+  if (!n || !x || !pred) return ELSIM_ERR_ARG;
+  if (in_used != 8 && in_used != NN_IN_CH) {
+    set_error("elsim_nn_forward_used: in_used %d (8 or 16)", in_used);
+    return ELSIM_ERR_ARG;
+  }
+  return nn_forward(n, n1, n2, n3, x, pred, in_used);
 }
 
 int elsim_nn_loss_grad(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, const float* target,

@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void k_bv(Dims d, const double* __restrict__ p
 __global__ __launch_bounds__(256) void k_feat_pack(Dims d, const double* __restrict__ psp,
                                                    const double* __restrict__ r0, const double* __restrict__ kap,
                                                    const double* __restrict__ dphi, double ir, double idp,
-                                                   uint16_t* __restrict__ x) {
+                                                   int full, uint16_t* __restrict__ x) {
   const Col ci = colx(d);
   const float depth = (float)(2.0 * (ci.j - 0.5) / d.n2 - 1.0);
   each(d, ci, [&](int64_t c, int64_t n, int) {
@@ -382,8 +382,9 @@ __global__ __launch_bounds__(256) void k_feat_pack(Dims d, const double* __restr
     uint4* o = (uint4*)(x + n * 16);
     o[0] = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
                       h[6] | (uint32_t)h[7] << 16);
-    o[1] = make_uint4(h[8] | (uint32_t)h[9] << 16, h[10] | (uint32_t)h[11] << 16, h[12] | (uint32_t)h[13] << 16,
-                      h[14] | (uint32_t)h[15] << 16);
+    if (full)
+      o[1] = make_uint4(h[8] | (uint32_t)h[9] << 16, h[10] | (uint32_t)h[11] << 16, h[12] | (uint32_t)h[13] << 16,
+                        h[14] | (uint32_t)h[15] << 16);
   });
 }
 
@@ -645,7 +646,9 @@ double host_sum(const double* partials, int nb) {
 
 // The network input at the first phP solve: the start's residual r0
 // (interior, rms rms_r) and the last change of phi_P (ws_dir, ghosted).
-int ws_features(SbmState& s, const double* r0, double rms_r, uint16_t* x) {
+// full: all 16 channels written (a recorded sample); else channels 0..7 only,
+// which is all the first layer reads of them (channels 5..15 are zero).
+int ws_features(SbmState& s, const double* r0, double rms_r, uint16_t* x, bool full) {
   const Dims& d = s.d;
   const dim3 grid = col_grid(d), blk(TX, TY, 1);
   k_sum_sq_g<<<grid, blk, 0, stream()>>>(d, s.g(s.ws_dir), workspace().partials);
@@ -653,7 +656,8 @@ int ws_features(SbmState& s, const double* r0, double rms_r, uint16_t* x) {
   const double sd = host_sum(workspace().partials, blocks_of(grid));
   const double rms_d = std::sqrt(sd / (double)d.N());
   k_feat_pack<<<grid, blk, 0, stream()>>>(d, s.g(s.psp), r0, s.g(s.kapp), s.g(s.ws_dir),
-                                          rms_r > 0 ? 1.0 / rms_r : 0.0, rms_d > 0 ? 1.0 / rms_d : 0.0, x);
+                                          rms_r > 0 ? 1.0 / rms_r : 0.0, rms_d > 0 ? 1.0 / rms_d : 0.0, full ? 1 : 0,
+                                          x);
   ELSIM_CHECK_LAUNCH();
   return ELSIM_OK;
 }
@@ -851,7 +855,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
       z.ws_res_plain = std::sqrt(rr);
       z.ws_res_nn = z.ws_res_plain;
       uint16_t* xin = s.rec_x ? s.rec_x : (uint16_t*)s.nn_x.p;
-      ELSIM_TRY(ws_features(s, r0, rms_r, xin));
+      ELSIM_TRY(ws_features(s, r0, rms_r, xin, s.rec_x != nullptr));
       if (s.rec_start) {
         k_interior<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.rec_start);
         ELSIM_CHECK_LAUNCH();
@@ -859,7 +863,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
       if (s.ws_net && rms_r > 0.0) {
         {
           const Range range("ws_network");
-          ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p));
+          ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p, 8));
         }
         double* pd = s.g(s.ws_cand);  // correction direction, zero ghosts
         ELSIM_HIP(hipMemsetAsync(pd, 0, (size_t)d.G() * sizeof(double), stv));

@@ -100,9 +100,14 @@ class WarmStartNet:
         check(lib().elsim_nn_get_grads(C.byref(self.h), v.ctypes.data), "elsim_nn_get_grads")
         return v
 
-    def forward(self, dims, x, pred):
-        """x: device bf16 [N][16]; pred: device FP32 [N]."""
-        check(lib().elsim_nn_forward(C.byref(self.h), *dims, x.ptr, pred.ptr), "elsim_nn_forward")
+    def forward(self, dims, x, pred, in_used=IN_CH):
+        """x: device bf16 [N][16]; pred: device FP32 [N].  in_used 8: channels
+        8..15 of x are zero and not read (the first layer on 8-channel slabs)."""
+        if in_used == IN_CH:
+            check(lib().elsim_nn_forward(C.byref(self.h), *dims, x.ptr, pred.ptr), "elsim_nn_forward")
+        else:
+            check(lib().elsim_nn_forward_used(C.byref(self.h), *dims, x.ptr, in_used, pred.ptr),
+                  "elsim_nn_forward_used")
 
     def loss_grad(self, dims, x, target, pred):
         loss = C.c_double()

@@ -418,6 +418,11 @@ int elsim_nn_get_params(const int64_t* handle, float* host_params);
 int elsim_nn_get_grads(const int64_t* handle, float* host_grads);
 /* pred[voxel] (FP32, device) of the input features x (device). */
 int elsim_nn_forward(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, float* pred);
+/* The same reading only input channels [0, in_used) (in_used 8 or 16): for
+ * inputs whose channels in_used..15 are zero, as the SBM warm start's
+ * features (channels 0..4), the first layer then runs on 8-channel slabs. */
+int elsim_nn_forward_used(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, int in_used,
+                          float* pred);
 /* Forward, loss = mean((pred - target)^2) (host double) and the gradients of
  * every parameter (kept on device, read with elsim_nn_get_grads). */
 int elsim_nn_loss_grad(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, const float* target,
@@ -450,7 +455,8 @@ int elsim_sbm_ws_record(const int64_t* sbm_handle, uint16_t* x_dev, double* star
  * bf16(relu?(b[co] + sum_{tap,ci} wt[co][tap cin + ci] x[v + o_tap][ci]));
  * bwd_data: dx[v][ci] (FP32) = sum_{tap,co} wb[ci][tap cout + co] dz[v + o_tap][co]
  * with wb the tap-flipped, transposed weights; bwd_weight: dw[co][tap][ci]
- * (FP32) = sum_v dz[v][co] x[v + o_tap][ci].  Zero padding, stride 1. */
+ * (FP32) = sum_v dz[v][co] x[v + o_tap][ci].  Zero padding, stride 1.
+ * conv3: cin 8 (then cout <= 32) or a multiple of 16, cout <= 64. */
 int elsim_nn_conv3(int n1, int n2, int n3, const uint16_t* x, int cin, const uint16_t* wt, const float* bias,
                    int cout, int relu, uint16_t* out);
 int elsim_nn_conv3_bwd_data(int n1, int n2, int n3, const uint16_t* dz, int cout, const uint16_t* wb, int cin,
