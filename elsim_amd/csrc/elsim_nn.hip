@@ -360,6 +360,217 @@ static int conv(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, co
   return ELSIM_OK;
 }
 
+// ------------------------------------- last decoder layer at inference --
+// The d0 convolution (48 -> 16 channels: 0..31 nearest-upsampled from d1,
+// 32..47 the skip tensor) with its ReLU and the 1x1x1 output layer, for the
+// inference forward.  The upsampled half is convolved where it lies, at d1's
+// resolution: for a fine voxel of parity (px, py, pz) the 27 taps fall on
+// 2 x 2 x 2 coarse voxels (along one axis, parity 0 sends tap 0 to coarse
+// offset -1 and taps 1, 2 to 0; parity 1 sends taps 0, 1 to 0 and tap 2 to
+// +1), so its 27 weight matrices fold into 8 per parity (wf, summed in FP32
+// from the bf16 weights and rounded to bf16): 8 MFMAs of 32 channels in
+// place of 27.  Each of the 8 waves of a block owns one parity class: its
+// MFMA rows are 16 fine voxels two apart in x, and the staged skip tile is
+// split by x parity so that these rows read consecutive LDS rows.  Blocks are
+// persistent (one per CU): both weight sets stay in registers, and the next
+// tile's loads are in flight while the current one is computed.  The output
+// layer reads the bf16-rounded ReLU output as the unfused path does.
+constexpr int D0_CC = 32, D0_SC = 16, D0_CIN = D0_CC + D0_SC;
+constexpr int D0_TX = 16, D0_TY = 2, D0_TZ = 2;  // coarse tile (fine: 32 x 4 x 4)
+constexpr int D0_CHX = D0_TX + 2, D0_CHY = D0_TY + 2, D0_CHZ = D0_TZ + 2;
+constexpr int D0_CHN = D0_CHX * D0_CHY * D0_CHZ;  // staged coarse voxels
+constexpr int D0_CRS = D0_CC + 8;                 // their LDS row (elements)
+constexpr int D0_SHX = 2 * D0_TX + 2, D0_SHY = 2 * D0_TY + 2, D0_SHZ = 2 * D0_TZ + 2;
+constexpr int D0_SHN = D0_SHX * D0_SHY * D0_SHZ;  // staged fine skip voxels
+constexpr int D0_SPR = D0_SHX / 2;                // voxels per x-parity row
+constexpr int D0_SRS = D0_SC + 8;
+constexpr int D0_FN = 8 * D0_TX * D0_TY * D0_TZ;  // fine voxels per tile
+constexpr int D0_ORS = 24;
+constexpr int D0_NUC = (D0_CHN * (D0_CC / 8) + 511) / 512, D0_NUS = (D0_SHN * (D0_SC / 8) + 511) / 512;
+static_assert(D0_FN == 512 && D0_CC == 32 && D0_SC == 16, "one fine voxel per thread; 4 + 2 chunks per staged voxel");
+
+// wf[p][co][o][ci] = sum over the taps that parity p sends to coarse offset
+// o of wt[co][tap][ci], ci < 32 (p = px + 2 py + 4 pz, o likewise).
+__global__ void k_fold_d0(const uint16_t* __restrict__ wt, uint16_t* __restrict__ wf) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 8 * 16 * 8 * D0_CC) return;
+  const int ci = e % D0_CC, o = (e / D0_CC) % 8, co = (e / (D0_CC * 8)) % 16, p = e / (D0_CC * 8 * 16);
+  // taps [lo, hi] of parity pp that land on coarse offset oo
+  auto lo = [](int pp, int oo) { return pp == 0 ? (oo == 0 ? 0 : 1) : (oo == 0 ? 0 : 2); };
+  auto hi = [](int pp, int oo) { return pp == 0 ? (oo == 0 ? 0 : 2) : (oo == 0 ? 1 : 2); };
+  const int px = p & 1, py = (p >> 1) & 1, pz = p >> 2, ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+  float sum = 0.f;
+  for (int dz = lo(pz, oz); dz <= hi(pz, oz); dz++)
+    for (int dy = lo(py, oy); dy <= hi(py, oy); dy++)
+      for (int dx = lo(px, ox); dx <= hi(px, ox); dx++)
+        sum += bf2f(wt[(co * 27 + dx + 3 * (dy + 3 * dz)) * D0_CIN + ci]);
+  wf[e] = f2bf(sum);
+}
+
+__global__ __launch_bounds__(512) void k_dec0_out(int n1, int n2, int n3, const uint16_t* __restrict__ skip, View sv,
+                                                  const uint16_t* __restrict__ lo, View lv,
+                                                  const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wf,
+                                                  const float* __restrict__ bias, const float* __restrict__ ow,
+                                                  float* __restrict__ pred) {
+  __shared__ __align__(16) uint16_t lds_c[D0_CHN * D0_CRS];
+  __shared__ __align__(16) uint16_t lds_s[D0_SHN * D0_SRS];
+  __shared__ __align__(16) uint16_t lds_o[D0_FN * D0_ORS];
+  const int m1 = n1 / 2, m2 = n2 / 2, m3 = n3 / 2;
+  const int tx = (m1 + D0_TX - 1) / D0_TX, ty = (m2 + D0_TY - 1) / D0_TY, tz = (m3 + D0_TZ - 1) / D0_TZ;
+  const int ntiles = tx * ty * tz;
+  const int tid = threadIdx.x, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int px = wave & 1, py = (wave >> 1) & 1, pz = wave >> 2;
+  const int tsub = lq >> 1, csub = 8 * (lq & 1);
+  // Skip-half tap pairs of one MFMA (lanes 0..31: the first tap, 32..63 the
+  // second, one step along z, y or x): pairs 0..8 (dx, dy, 0) + z, 9..11
+  // (dx, 0, 2) + y, 12 (0, 2, 2) + x, 13 (2, 2, 2) alone.  The second tap's
+  // staged address is the first's plus a per-lane constant of its direction.
+  auto pair_tap = [](int k, int& dx, int& dy, int& dz, int& dir) {
+    if (k < 9) { dx = k % 3; dy = k / 3; dz = 0; dir = 2; }
+    else if (k < 12) { dx = k - 9; dy = 0; dz = 2; dir = 1; }
+    else if (k == 12) { dx = 0; dy = 2; dz = 2; dir = 0; }
+    else { dx = 2; dy = 2; dz = 2; dir = 3; }
+  };
+  uint4 bc[8], bs[14];
+#pragma unroll
+  for (int o = 0; o < 8; o++) bc[o] = *(const uint4*)(wf + ((wave * 16 + lr) * 8 + o) * D0_CC + 8 * lq);
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    int dx, dy, dz, dir;
+    pair_tap(k, dx, dy, dz, dir);
+    if (tsub) {
+      dx += dir == 0;
+      dy += dir == 1;
+      dz += dir == 2;
+    }
+    const int tap = dx + 3 * (dy + 3 * dz);
+    bs[k] = (tsub && dir == 3) ? make_uint4(0, 0, 0, 0)
+                               : *(const uint4*)(wt + (lr * 27 + tap) * D0_CIN + D0_CC + csub);
+  }
+  const float bco = bias[lr];
+  // LDS element offsets: coarse rows of this lane and wave parity; skip rows
+  // (plus the second tap's step along z, y and x for lanes 32..63)
+  const int cbase = ((pz * D0_CHY + py) * D0_CHX + lr + px) * D0_CRS + 8 * lq;
+  const int sbase = (((py + D0_SHY * pz) * 2 + (px & 1)) * D0_SPR + lr) * D0_SRS + csub;
+  const int sdz = tsub * D0_SHY * 2 * D0_SPR * D0_SRS, sdy = tsub * 2 * D0_SPR * D0_SRS;
+  const int sdx = tsub * (px == 0 ? D0_SPR : 1 - D0_SPR) * D0_SRS;  // pair 12 has dx = 0
+
+  uint4 vc[D0_NUC], vs[D0_NUS];
+  auto load = [&](int t) {
+    const int X0 = (t % tx) * D0_TX, Y0 = ((t / tx) % ty) * D0_TY, Z0 = (t / (tx * ty)) * D0_TZ;
+#pragma unroll
+    for (int u = 0; u < D0_NUC; u++) {
+      const int e = tid + 512 * u, h = e >> 2, q = e & 3;
+      const int gx = X0 - 1 + h % D0_CHX, gy = Y0 - 1 + (h / D0_CHX) % D0_CHY, gz = Z0 - 1 + h / (D0_CHX * D0_CHY);
+      vc[u] = make_uint4(0, 0, 0, 0);
+      if (e < D0_CHN * 4 && gx >= 0 && gx < m1 && gy >= 0 && gy < m2 && gz >= 0 && gz < m3)
+        vc[u] = *(const uint4*)(lo + (gx + (int64_t)m1 * (gy + (int64_t)m2 * gz)) * lv.cs + lv.co + 8 * q);
+    }
+#pragma unroll
+    for (int u = 0; u < D0_NUS; u++) {
+      const int e = tid + 512 * u, h = e >> 1, q = e & 1;
+      const int gx = 2 * X0 - 1 + h % D0_SHX, gy = 2 * Y0 - 1 + (h / D0_SHX) % D0_SHY,
+                gz = 2 * Z0 - 1 + h / (D0_SHX * D0_SHY);
+      vs[u] = make_uint4(0, 0, 0, 0);
+      if (e < D0_SHN * 2 && gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3)
+        vs[u] = *(const uint4*)(skip + (gx + (int64_t)n1 * (gy + (int64_t)n2 * gz)) * sv.cs + sv.co + 8 * q);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int u = 0; u < D0_NUC; u++) {
+      const int e = tid + 512 * u;
+      if (e < D0_CHN * 4) *(uint4*)(lds_c + (e >> 2) * D0_CRS + 8 * (e & 3)) = vc[u];
+    }
+#pragma unroll
+    for (int u = 0; u < D0_NUS; u++) {
+      const int e = tid + 512 * u, h = e >> 1;
+      if (e < D0_SHN * 2) {
+        const int hx = h % D0_SHX, hyz = h / D0_SHX;  // hyz = hy + D0_SHY hz
+        *(uint4*)(lds_s + ((hyz * 2 + (hx & 1)) * D0_SPR + (hx >> 1)) * D0_SRS + 8 * (e & 1)) = vs[u];
+      }
+    }
+  };
+
+  int t = blockIdx.x;
+  if (t < ntiles) load(t);
+  for (; t < ntiles; t += gridDim.x) {
+    __syncthreads();  // every wave is done with the previous tile's LDS
+    store();
+    if (t + (int)gridDim.x < ntiles) load(t + gridDim.x);
+    __syncthreads();
+    f4v acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) acc[m] = f4v{0.f, 0.f, 0.f, 0.f};
+    // 22 steps of 4 MFMAs (one per coarse row): 8 coarse offsets o of the
+    // upsampled half, then 14 tap pairs of the skip half.  The A fragments of
+    // step k + 1 are read from LDS before the MFMAs of step k are issued.
+    // Upsampled half: coarse voxel (X + px - 1 + ox, ...) is staged at
+    // (lr + px + ox, ...); skip half: fine voxel (2 X + px + dx - 1, ...) at
+    // staged x 2 lr + px + dx.
+    auto lda = [&](int k, uint4 (&a)[4]) {
+      if (k < 8) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const int h = (((m >> 1) + oz) * D0_CHY + ((m & 1) + oy)) * D0_CHX + ox;
+          a[m] = *(const uint4*)(lds_c + cbase + h * D0_CRS);
+        }
+      } else {
+        int dx, dy, dz, dir;
+        pair_tap(k - 8, dx, dy, dz, dir);
+        const int step = dir == 2 ? sdz : dir == 1 ? sdy : dir == 0 ? sdx : 0;
+        // staged x of the first tap: 2 lr + px + dx (px's parity is in sbase)
+        const int xo = (px + dx) / 2 * D0_SRS + ((px + dx) % 2 - px) * D0_SPR * D0_SRS;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const int hyz = (2 * (m & 1) + dy) + D0_SHY * (2 * (m >> 1) + dz);
+          a[m] = *(const uint4*)(lds_s + sbase + step + xo + hyz * 2 * D0_SPR * D0_SRS);
+        }
+      }
+    };
+    uint4 a0[4], a1[4];
+    lda(0, a0);
+#pragma unroll
+    for (int k = 0; k < 22; k += 2) {
+      lda(k + 1, a1);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int m = 0; m < 4; m++) acc[m] = mfma32(a0[m], k < 8 ? bc[k] : bs[k - 8], acc[m]);
+      __builtin_amdgcn_sched_barrier(0);
+      if (k + 2 < 22) lda(k + 2, a0);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int m = 0; m < 4; m++) acc[m] = mfma32(a1[m], k + 1 < 8 ? bc[k + 1] : bs[k - 7], acc[m]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // ReLU output of voxel X = 4 lq + i, channel lr, as bf16 into the tile
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const int fy = 2 * (m & 1) + py, fz = 2 * (m >> 1) + pz;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int fx = 2 * (4 * lq + i) + px;
+        lds_o[((fz * 2 * D0_TY + fy) * 2 * D0_TX + fx) * D0_ORS + lr] = f2bf(fmaxf(acc[m][i] + bco, 0.f));
+      }
+    }
+    __syncthreads();
+    {
+      const int X0 = (t % tx) * D0_TX, Y0 = ((t / tx) % ty) * D0_TY, Z0 = (t / (tx * ty)) * D0_TZ;
+      const int fx = tid % (2 * D0_TX), fy = (tid / (2 * D0_TX)) % (2 * D0_TY), fz = tid / (4 * D0_TX * D0_TY);
+      const int gx = 2 * X0 + fx, gy = 2 * Y0 + fy, gz = 2 * Z0 + fz;
+      if (gx < n1 && gy < n2 && gz < n3) {
+        float sum = 0.f;
+#pragma unroll
+        for (int c = 0; c < 16; c++) sum += ow[c] * bf2f(lds_o[tid * D0_ORS + c]);
+        sum += ow[16];
+        pred[gx + (int64_t)n1 * (gy + (int64_t)n2 * gz)] = sum;
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------- weight gradient --
 
 constexpr int WG_VOX = 4096;  // voxels per block of the weight-gradient kernel
@@ -660,6 +871,15 @@ __global__ void k_pack(int cin, int cout, const float* __restrict__ W, uint16_t*
 namespace {
 
 inline int nblk(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+// compute units of the current device (persistent grids)
+int cu_count() {
+  static int cus[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus[dev] = 256;
+  return cus[dev] > 0 ? cus[dev] : 256;
+}
 // Element kernels over voxel x channel index spaces loop over a capped grid:
 // a dispatch holds at most 2^32 - 1 work-items per dimension, which one
 // thread per element would exceed at 512^3 x 32 channels.
@@ -682,6 +902,7 @@ int64_t nn_param_count() {
 struct NnNet {
   DevBuf params, grads, m, v;  // FP32, nn_param_count()
   DevBuf wt, wb;               // bf16 packed convolution weights
+  DevBuf wf;                   // d0's upsampled half folded per parity (inference)
   int64_t w_off[NN_NLAYERS], b_off[NN_NLAYERS], wt_off[NN_NLAYERS], out_off = 0;
   int t = 0;
   bool packed = false;
@@ -719,6 +940,7 @@ struct NnNet {
     }
     ELSIM_TRY(dev_alloc(wt, nw() * 2));
     ELSIM_TRY(dev_alloc(wb, nw() * 2));
+    ELSIM_TRY(dev_alloc(wf, 8 * 16 * 8 * D0_CC * 2));
     ELSIM_TRY(dev_alloc(loss, sizeof(double)));
     return ELSIM_OK;
   }
@@ -728,6 +950,7 @@ struct NnNet {
       const int64_t n = (int64_t)27 * kLayers[l].cin * kLayers[l].cout;
       k_pack<<<nblk(n), 256, 0, stream()>>>(kLayers[l].cin, kLayers[l].cout, P() + w_off[l], WT(l), WB(l));
     }
+    k_fold_d0<<<nblk(8 * 16 * 8 * D0_CC), 256, 0, stream()>>>(WT(L_D0), (uint16_t*)wf.p);
     ELSIM_CHECK_LAUNCH();
     packed = true;
     return ELSIM_OK;
@@ -821,6 +1044,16 @@ struct NnNet {
     if (!infer) k_up<<<nblk_cap(N1 * 8), 256, 0, s>>>(m1, m2, m3, 64, (uint16_t*)d2.p, C1, View{96, 0});
     ELSIM_TRY(conv<0>(m1, m2, m3, C1, 96, i1, WT(L_D1), B + b_off[L_D1], 1, 32, d1.p, View{32, 0}, s,
                       infer ? Coarse{(uint16_t*)d2.p, 64, View{64, 0}} : none));
+    if (infer && !target) {
+      // d0, its ReLU and the output layer in one pass, the upsampled half
+      // convolved at d1's resolution with the folded weights
+      const int nt = ((m1 + D0_TX - 1) / D0_TX) * ((m2 + D0_TY - 1) / D0_TY) * ((m3 + D0_TZ - 1) / D0_TZ);
+      k_dec0_out<<<std::min(nt, cu_count()), 512, 0, s>>>(a1, a2, a3, C0, View{16, 0}, (uint16_t*)d1.p, View{32, 0},
+                                                          WT(L_D0), (uint16_t*)wf.p, B + b_off[L_D0], B + out_off,
+                                                          pred);
+      ELSIM_CHECK_LAUNCH();
+      return ELSIM_OK;
+    }
     if (!infer) k_up<<<nblk_cap(N0 * 4), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
     ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, i0, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s,
                       infer ? Coarse{(uint16_t*)d1.p, 32, View{32, 0}} : none));
@@ -944,7 +1177,7 @@ struct NnNet {
   }
 
   void release() {
-    for (DevBuf* b : {&params, &grads, &m, &v, &wt, &wb, &cat0, &p1, &cat1, &p2, &cat2, &p3, &e3, &d2, &d1, &d0,
+    for (DevBuf* b : {&params, &grads, &m, &v, &wt, &wb, &wf, &cat0, &p1, &cat1, &p2, &cat2, &p3, &e3, &d2, &d1, &d0,
                       &dpred, &dd0, &dcat0, &dd1, &dcat1, &dd2, &dcat2, &de3, &dp3, &dp2, &dp1, &dz, &wpart, &cpart,
                       &opart, &loss, &eg_dg, &eg_w})
       dev_free(*b);

@@ -105,15 +105,21 @@ def test_network_forward_and_gradients(gpu, shape):
     g = N.unflatten(net.grads())
     for k in g_ref:
         assert rel_l2(g[k], g_ref[k]) < 5e-2, k
-    # the forward alone gives the same prediction
+    # The inference forward: the same prediction up to its last decoder
+    # layer, which convolves the upsampled half at half resolution with the
+    # weights folded per voxel parity (sums of bf16 weights rounded to bf16)
+    # and sums in another order.
     pd2 = DeviceArray((n1 * n2 * n3,), np.float32)
     net.forward((n1, n2, n3), xd, pd2)
-    assert np.array_equal(pd2.to_host(), pd.to_host())
+    e16 = rel_l2(pd2.to_host(), pd.to_host())
+    assert rel_l2(pd2.to_host().reshape(n3, n2, n1), pred_ref) < 2e-2
     # channels 5..15 are zero: the forward reading channels 0..7 only (first
     # layer on 8-channel slabs) differs by FP32 summation order alone
     net.forward((n1, n2, n3), xd, pd2, in_used=8)
+    e8 = rel_l2(pd2.to_host(), pd.to_host())
+    print("inference forward vs training forward: rel l2 %.2e (16 channels), %.2e (8)" % (e16, e8))
+    assert e16 < 5e-3 and e8 < 5e-3
     assert rel_l2(pd2.to_host().reshape(n3, n2, n1), pred_ref) < 2e-2
-    assert rel_l2(pd2.to_host(), pd.to_host()) < 1e-2
     net.close()
 
 
