@@ -156,7 +156,7 @@ def nn_forward_time(E, net, n, reps=5):
     events on the library stream), on zero input, reading input channels
     0..7 as the SBM warm start does."""
     L = E.lib()
-    x = E.DeviceArray((n ** 3 * 16,), np.uint16)
+    x = E.DeviceArray((n ** 3 * 8,), np.uint16)
     E.check(L.elsim_memset(x.ptr, 0, x.nbytes))
     pred = E.DeviceArray((n ** 3,), np.float32)
     net.forward((n, n, n), x, pred, in_used=8)

@@ -22,9 +22,9 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 
 struct NnNet;
 int64_t nn_param_count();
-// pred[v] (FP32) for the input features x (bf16 [voxel][16]); grid dims
-// multiples of 8.  in_used 8: channels 8..15 of x are zero and are not read
-// (16: all are read).
+// pred[v] (FP32) for the input features x (bf16 [voxel][in_used], in_used
+// 16 or 8; with 8, the network's input channels 8..15 are zero); grid dims
+// multiples of 8.
 int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred, int in_used);
 // The network of a handle issued by elsim_nn_create (nullptr if unknown or
 // destroyed).

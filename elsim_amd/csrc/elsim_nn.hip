@@ -108,10 +108,14 @@ __device__ __forceinline__ void stage_slab(int nq, int rs, int x0, int y0, int z
 // buffer (LDS_N elements): written as [voxel][channel], then stored as
 // 16-byte rows of 8 channels instead of 2-byte scattered stores (E0 at 256^3:
 // 1003 -> 844 us).  Every wave must have finished reading the staged input.
+// pout (MODE 0 through LDS only): also the 2x2x2 average pooling of the tile
+// (compact rows of COUT channels at half resolution), summed from the stored
+// bf16 values in k_pool's order, so the pooled tensor is the same.
 template <int COUT, int MT, int MODE, int LDS_N>
 __device__ __forceinline__ void conv_epilogue(const f4v (&acc)[MT][COUT / 16], int x0, int y0, int z0, int n1,
                                               int n2, int n3, const float* __restrict__ bias, int relu,
-                                              void* __restrict__ out, View ov, uint16_t* __restrict__ lds) {
+                                              void* __restrict__ out, View ov, uint16_t* __restrict__ lds,
+                                              uint16_t* __restrict__ pout) {
   constexpr int NT = COUT / 16;
   constexpr int OS = COUT + 8;  // LDS row stride of the output tile (elements)
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -139,6 +143,32 @@ __device__ __forceinline__ void conv_epilogue(const f4v (&acc)[MT][COUT / 16], i
       if (x >= n1 || y >= n2 || z >= n3) continue;
       const int64_t vox = x + (int64_t)n1 * (y + (int64_t)n2 * z);
       *(uint4*)((uint16_t*)out + vox * ov.cs + ov.co + 8 * c8) = *(const uint4*)(lds + vt * OS + 8 * c8);
+    }
+    static_assert(CONV_TX % 2 == 0 && CONV_TY % 2 == 0, "pooling of whole 2x2x2 blocks");
+    if (MT % 2 == 0 && pout) {  // the host launches an odd MT without pooling only
+      constexpr int PX = CONV_TX / 2, PY = CONV_TY / 2, PN = PX * PY * (MT / 2 > 0 ? MT / 2 : 1);
+      const int m1 = n1 / 2, m2 = n2 / 2, m3 = n3 / 2;
+      for (int e = tid; e < PN * CH; e += 256) {
+        const int pv = e / CH, c8 = e % CH;
+        const int px = pv % PX, py = (pv / PX) % PY, pz = pv / (PX * PY);
+        const int cx = x0 / 2 + px, cy = y0 / 2 + py, cz = z0 / 2 + pz;
+        if (cx >= m1 || cy >= m2 || cz >= m3) continue;
+        uint4 ch[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int vt = (2 * px + (q & 1)) + CONV_TX * ((2 * py + ((q >> 1) & 1)) + CONV_TY * (2 * pz + (q >> 2)));
+          ch[q] = *(const uint4*)(lds + vt * OS + 8 * c8);
+        }
+        uint16_t r[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          float sum = 0.f;
+#pragma unroll
+          for (int q = 0; q < 8; q++) sum += bf2f(((const uint16_t*)&ch[q])[k]);
+          r[k] = f2bf(sum * 0.125f);
+        }
+        *(uint4*)(pout + (cx + (int64_t)m1 * (cy + (int64_t)m2 * cz)) * COUT + 8 * c8) = *(const uint4*)r;
+      }
     }
   } else {
 #pragma unroll
@@ -171,7 +201,7 @@ template <int COUT, int MT, int MODE>
 __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
                                                View iv, Coarse lo, const uint16_t* __restrict__ wt,
                                                const float* __restrict__ bias, int relu, void* __restrict__ out,
-                                               View ov) {
+                                               View ov, uint16_t* __restrict__ pout) {
   constexpr int NT = COUT / 16;
   constexpr int HX = CONV_TX + 2, HY = CONV_TY + 2, HZ = MT + 2, HN = HX * HY * HZ;
   __shared__ __align__(16) uint16_t lds[HN * (CS_MAX + LDS_PAD)];
@@ -235,7 +265,7 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
       }
     }
   }
-  conv_epilogue<COUT, MT, MODE, HN*(CS_MAX + LDS_PAD)>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds);
+  conv_epilogue<COUT, MT, MODE, HN*(CS_MAX + LDS_PAD)>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds, pout);
 }
 
 // The same convolution for 16- or 8-channel slabs (CS; input channels not a
@@ -253,7 +283,7 @@ template <int COUT, int MT, int MODE, int CS>
 __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
                                                    int cused, View iv, Coarse lo, const uint16_t* __restrict__ wt,
                                                    const float* __restrict__ bias, int relu, void* __restrict__ out,
-                                                   View ov) {
+                                                   View ov, uint16_t* __restrict__ pout) {
   constexpr int NT = COUT / 16;
   static_assert(NT <= 2, "weight fragments of 27 taps for at most two output tiles");
   static_assert(CS == 16 || CS == 8, "slabs of 16 or 8 channels");
@@ -304,34 +334,44 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
         for (int m = 0; m < MT; m++) acc[m][t] = mfma32(a[m], bw[pr][t], acc[m][t]);
     }
   }
-  conv_epilogue<COUT, MT, MODE, HN * RS>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds);
+  conv_epilogue<COUT, MT, MODE, HN * RS>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds, pout);
 }
 
 template <int MODE, int COUT, int MT>
 static void conv_launch(int n1, int n2, int n3, const uint16_t* in, int cin, int cused, View iv, Coarse lo,
-                        const uint16_t* w, const float* bias, int relu, void* out, View ov, hipStream_t s) {
+                        const uint16_t* w, const float* bias, int relu, void* out, View ov, hipStream_t s,
+                        uint16_t* pout) {
   const int nb = ((n1 + CONV_TX - 1) / CONV_TX) * ((n2 + CONV_TY - 1) / CONV_TY) * ((n3 + MT - 1) / MT);
   if constexpr (COUT <= 32) {
     if (cused % 16 != 0) {
-      k_conv3_w27<COUT, MT, MODE, 8><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov);
+      k_conv3_w27<COUT, MT, MODE, 8><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, pout);
       return;
     }
-    if (cin % 32 != 0) {
-      k_conv3_w27<COUT, MT, MODE, 16><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov);
+    // 16-channel slabs with all 27 taps' weights in registers pay for 16
+    // output channels only: for 32, the 112 registers of weights leave one
+    // wave per SIMD (E1 at 128^3: 278 us against 213 us for k_conv3)
+    if (cin % 32 != 0 && COUT == 16) {
+      k_conv3_w27<COUT, MT, MODE, 16><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, pout);
       return;
     }
   }
-  k_conv3<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov);
+  k_conv3<COUT, MT, MODE><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, iv, lo, w, bias, relu, out, ov, pout);
 }
 
 // Input channels: a multiple of 16, or 8.  cused < cin: only input channels
 // [0, cused) are read (the others are known to be zero, their weights
 // contribute nothing).  8 used channels (cin 8, or 8 of 16) run the 8-channel
-// slab kernel, which takes at most 32 output channels.
+// slab kernel, which takes at most 32 output channels.  pout: also the
+// output's 2x2x2 average pooling (MODE 0, at most 64 output channels, even
+// grid), as k_pool computes it.
 template <int MODE>
 static int conv(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, const uint16_t* w, const float* bias,
                 int relu, int cout, void* out, View ov, hipStream_t s, Coarse lo = Coarse{nullptr, 0, View{0, 0}},
-                int cused = 0) {
+                int cused = 0, uint16_t* pout = nullptr) {
+  if (pout && (MODE != 0 || cout > 64 || n1 % 2 || n2 % 2 || n3 % 2)) {
+    set_error("convolution: pooled output needs bf16 output, at most 64 channels and an even grid");
+    return ELSIM_ERR_ARG;
+  }
   if ((cin % 16 && cin != 8) || cin <= 0 || iv.cs % 8 || iv.co % 8) {
     set_error("convolution: input channels %d (stride %d, offset %d) must be 8 or multiples of 16 (8)", cin, iv.cs,
               iv.co);
@@ -348,12 +388,12 @@ static int conv(int n1, int n2, int n3, const uint16_t* in, int cin, View iv, co
     return ELSIM_ERR_ARG;
   }
   switch (cout) {
-    case 16: conv_launch<MODE, 16, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
-    case 32: conv_launch<MODE, 32, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
-    case 48: conv_launch<MODE, 48, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
-    case 64: conv_launch<MODE, 64, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
-    case 96: conv_launch<MODE, 96, 2>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
-    case 128: conv_launch<MODE, 128, 2>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s); break;
+    case 16: conv_launch<MODE, 16, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s, pout); break;
+    case 32: conv_launch<MODE, 32, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s, pout); break;
+    case 48: conv_launch<MODE, 48, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s, pout); break;
+    case 64: conv_launch<MODE, 64, 4>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s, pout); break;
+    case 96: conv_launch<MODE, 96, 2>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s, pout); break;
+    case 128: conv_launch<MODE, 128, 2>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, s, pout); break;
     default: set_error("convolution: %d output channels not supported", cout); return ELSIM_ERR_ARG;
   }
   ELSIM_CHECK_LAUNCH();
@@ -1005,8 +1045,9 @@ struct NnNet {
   // infer: the activations are not kept for a backward pass, so the
   // decoder reads its upsampled halves at half resolution while staging
   // (no full-resolution copies; the same values, so the same results).
-  // in_used: input channels [in_used, 16) are known to be zero (8: the first
-  // layer runs on 8-channel slabs).
+  // in_used: x holds rows of in_used channels, the network's input
+  // channels [in_used, 16) being zero (8: the first layer runs on 8-channel
+  // slabs).
   int forward(const uint16_t* x0, float* pred, const float* target, double* loss_host, bool infer = false,
               int in_used = NN_IN_CH) {
     ELSIM_TRY(pack());
@@ -1027,17 +1068,17 @@ struct NnNet {
                s2 = infer ? View{64, 0} : View{128, 64};
     const View i0 = infer ? View{16, -32} : View{48, 0}, i1 = infer ? View{32, -64} : View{96, 0},
                i2 = infer ? View{64, -64} : View{128, 0};
-    ELSIM_TRY(conv<0>(a1, a2, a3, x0, 16, View{16, 0}, WT(L_E0), B + b_off[L_E0], 1, 16, C0, s0, s,
-                      Coarse{nullptr, 0, View{0, 0}}, in_used));
-    k_pool<<<nblk_cap(N1 * 2), 256, 0, s>>>(a1, a2, a3, 16, C0, s0, (uint16_t*)p1.p);
-    ELSIM_TRY(conv<0>(m1, m2, m3, (uint16_t*)p1.p, 16, View{16, 0}, WT(L_E1), B + b_off[L_E1], 1, 32, C1, s1, s));
-    k_pool<<<nblk_cap(N2 * 4), 256, 0, s>>>(m1, m2, m3, 32, C1, s1, (uint16_t*)p2.p);
-    ELSIM_TRY(conv<0>(q1, q2, q3, (uint16_t*)p2.p, 32, View{32, 0}, WT(L_E2), B + b_off[L_E2], 1, 64, C2, s2, s));
-    k_pool<<<nblk_cap(N3 * 8), 256, 0, s>>>(q1, q2, q3, 64, C2, s2, (uint16_t*)p3.p);
+    // each encoder convolution also writes its 2x2x2-pooled output
+    const Coarse none{nullptr, 0, View{0, 0}};
+    ELSIM_TRY(conv<0>(a1, a2, a3, x0, 16, View{in_used, 0}, WT(L_E0), B + b_off[L_E0], 1, 16, C0, s0, s, none,
+                      in_used, (uint16_t*)p1.p));
+    ELSIM_TRY(conv<0>(m1, m2, m3, (uint16_t*)p1.p, 16, View{16, 0}, WT(L_E1), B + b_off[L_E1], 1, 32, C1, s1, s, none,
+                      0, (uint16_t*)p2.p));
+    ELSIM_TRY(conv<0>(q1, q2, q3, (uint16_t*)p2.p, 32, View{32, 0}, WT(L_E2), B + b_off[L_E2], 1, 64, C2, s2, s, none,
+                      0, (uint16_t*)p3.p));
     ELSIM_TRY(conv<0>(r1, r2, r3, (uint16_t*)p3.p, 64, View{64, 0}, WT(L_E3), B + b_off[L_E3], 1, 64, e3.p,
                       View{64, 0}, s));
     // decoder
-    const Coarse none{nullptr, 0, View{0, 0}};
     if (!infer) k_up<<<nblk_cap(N2 * 8), 256, 0, s>>>(q1, q2, q3, 64, (uint16_t*)e3.p, C2, View{128, 0});
     ELSIM_TRY(conv<0>(q1, q2, q3, C2, 128, i2, WT(L_D2), B + b_off[L_D2], 1, 64, d2.p, View{64, 0}, s,
                       infer ? Coarse{(uint16_t*)e3.p, 64, View{64, 0}} : none));

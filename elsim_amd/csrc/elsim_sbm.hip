@@ -359,7 +359,8 @@ __global__ __launch_bounds__(256) void k_bv(Dims d, const double* __restrict__ p
 
 // ------------------------------------------------ learned warm start --
 
-// Network input x[voxel][16] (bf16) at the first phP solve of a step:
+// Network input x[voxel][16 or 8] (bf16, full or not) at the first phP
+// solve of a step:
 // psi_P, the FP64 residual of the extrapolated start in units of its rms,
 // the through-thickness coordinate (d2, the axis of the boundary
 // conditions) in (-1, 1), log10 of the phP coefficient over 7 (its range is
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(256) void k_feat_pack(Dims d, const double* __restr
     h[3] = f2bf((float)(log10(fmax(kap[c], 1e-30)) / 7.0));
     h[4] = f2bf((float)(dphi[c] * idp));
     for (int q = 5; q < 16; q++) h[q] = 0;
-    uint4* o = (uint4*)(x + n * 16);
+    uint4* o = (uint4*)(x + n * (full ? 16 : 8));
     o[0] = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
                       h[6] | (uint32_t)h[7] << 16);
     if (full)
@@ -646,8 +647,8 @@ double host_sum(const double* partials, int nb) {
 
 // The network input at the first phP solve: the start's residual r0
 // (interior, rms rms_r) and the last change of phi_P (ws_dir, ghosted).
-// full: all 16 channels written (a recorded sample); else channels 0..7 only,
-// which is all the first layer reads of them (channels 5..15 are zero).
+// full: rows of all 16 channels (a recorded sample); else rows of channels
+// 0..7, the network's input with channels 8..15 zero.
 int ws_features(SbmState& s, const double* r0, double rms_r, uint16_t* x, bool full) {
   const Dims& d = s.d;
   const dim3 grid = col_grid(d), blk(TX, TY, 1);
@@ -863,7 +864,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
       if (s.ws_net && rms_r > 0.0) {
         {
           const Range range("ws_network");
-          ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p, 8));
+          ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p, s.rec_x ? 16 : 8));
         }
         double* pd = s.g(s.ws_cand);  // correction direction, zero ghosts
         ELSIM_HIP(hipMemsetAsync(pd, 0, (size_t)d.G() * sizeof(double), stv));

@@ -101,8 +101,9 @@ class WarmStartNet:
         return v
 
     def forward(self, dims, x, pred, in_used=IN_CH):
-        """x: device bf16 [N][16]; pred: device FP32 [N].  in_used 8: channels
-        8..15 of x are zero and not read (the first layer on 8-channel slabs)."""
+        """x: device bf16 [N][in_used] (in_used 16, or 8 when input channels
+        8..15 are zero: the first layer then runs on 8-channel slabs); pred:
+        device FP32 [N]."""
         if in_used == IN_CH:
             check(lib().elsim_nn_forward(C.byref(self.h), *dims, x.ptr, pred.ptr), "elsim_nn_forward")
         else:

@@ -418,9 +418,9 @@ int elsim_nn_get_params(const int64_t* handle, float* host_params);
 int elsim_nn_get_grads(const int64_t* handle, float* host_grads);
 /* pred[voxel] (FP32, device) of the input features x (device). */
 int elsim_nn_forward(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, float* pred);
-/* The same reading only input channels [0, in_used) (in_used 8 or 16): for
- * inputs whose channels in_used..15 are zero, as the SBM warm start's
- * features (channels 0..4), the first layer then runs on 8-channel slabs. */
+/* The same with x[voxel][in_used] (in_used 8 or 16), the network's input
+ * channels in_used..15 being zero: with 8 (the SBM warm start's features
+ * occupy channels 0..4) the first layer runs on 8-channel slabs. */
 int elsim_nn_forward_used(const int64_t* handle, int n1, int n2, int n3, const uint16_t* x, int in_used,
                           float* pred);
 /* Forward, loss = mean((pred - target)^2) (host double) and the gradients of
