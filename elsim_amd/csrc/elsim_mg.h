@@ -79,4 +79,14 @@ int mg_vcycle(MgWork& m, int op, const Dims& d, const float* r32, float* z, cons
               bool timed);
 int mg_partial_blocks(const Dims& d);
 
+// The CG iteration's residual update and the V-cycle in one pass over the
+// fine level (boxes whose n1 is a multiple of 4): r_out = r_in - alpha w
+// (alpha from cg; use_alpha false: r = r_in, r_out may be null, as at a start
+// or restart), z = M^-1 r, the pre-smoothed iterate formed where it is used.
+// r_out must not alias r_in or w; its ghosts stay zero.  Partials as
+// mg_vcycle (r.r of the new residual, r.z).
+bool mg_fused_ok(const Dims& d);
+int mg_vcycle_fused(MgWork& m, int op, const Dims& d, const float* r_in, const float* w32, bool use_alpha,
+                    float* r_out, float* z, const CgDev* cg, double* partials, bool timed);
+
 }  // namespace elsim

@@ -919,10 +919,23 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     const bool mv4 = mv4_ok(d), mv4_edge = mv4_tiles(d).gx > 1;
     const int nbm = mv4 ? mv4_ntiles(d) : nb32;
     const int fbd = (int)((G2 + (int64_t)FLAT_T * DIR_P - 1) / ((int64_t)FLAT_T * DIR_P));
-    auto precondition = [&]() { return mg_vcycle(m, OP, d, r32, w32, cg, P, false); };
+    // Fused form (n1 a multiple of 4): the residual update runs inside the
+    // V-cycle's down-sweep, which writes the new residual to the other of two
+    // buffers (r32, and the work space's pre-smoothing buffer, unused then);
+    // par names the current one.  A restart writes r32 and resets par.  An
+    // iteration that the device skips (solve done, or a reliable update
+    // due) leaves par stale only until that exit or restart.
+    const bool fused = mg_fused_ok(d);
+    float* rb[2] = {r32, m.x1};
+    int par = 0;
+    auto precondition = [&]() {
+      par = 0;
+      if (fused) return mg_vcycle_fused(m, OP, d, r32, r32, false, nullptr, w32, cg, P, false);
+      ELSIM_TRY(mg_presmooth(m, d, r32, cg));
+      return mg_vcycle(m, OP, d, r32, w32, cg, P, false);
+    };
     k_mg_start<<<fbu, FLAT_T, 0, s>>>(G2, (const double2*)r0, (float2*)r32, (float2*)p32, (float2*)y32,
                                       (double2*)e64);
-    ELSIM_TRY(mg_presmooth(m, d, r32, cg));
     ELSIM_TRY(precondition());
     fin2(FIN_MG_START, nbv);
     k_mg_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)w32, cg);
@@ -944,9 +957,16 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
         }));
         fin1(FIN_ALPHA_RZ, nbm);
         int rc = ELSIM_OK;
-        ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] { rc = mg_update(m, d, r32, w32, cg); }));
-        ELSIM_TRY(rc);
-        ELSIM_TRY(timed(t, ELSIM_PROF_MG, s, [&] { rc = mg_vcycle(m, OP, d, r32, w32, cg, P, t); }));
+        if (fused) {
+          ELSIM_TRY(timed(t, ELSIM_PROF_MG, s, [&] {
+            rc = mg_vcycle_fused(m, OP, d, rb[par], w32, true, rb[par ^ 1], w32, cg, P, t);
+          }));
+          par ^= 1;
+        } else {
+          ELSIM_TRY(timed(t, ELSIM_PROF_UPD32, s, [&] { rc = mg_update(m, d, r32, w32, cg); }));
+          ELSIM_TRY(rc);
+          ELSIM_TRY(timed(t, ELSIM_PROF_MG, s, [&] { rc = mg_vcycle(m, OP, d, r32, w32, cg, P, t); }));
+        }
         ELSIM_TRY(rc);
         fin2(FIN_MG_UPDATE, nbv);
         ELSIM_TRY(timed(t, ELSIM_PROF_DIR32, s, [&] {
@@ -969,7 +989,6 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
         k_mx_fold<<<fb, FLAT_T, 0, s>>>(G, e64, y32, cg);
         k_true_resid<OP, false><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, r32, nullptr, cg, P);
         fin2(FIN_TRUE_RELIABLE, nb64);
-        ELSIM_TRY(mg_presmooth(m, d, r32, cg));
         ELSIM_TRY(precondition());
         fin2(FIN_MG_START, nbv);
         k_mg_dir3<<<fbd, FLAT_T, 0, s>>>(G2, (float2*)y32, (float2*)p32, (const float2*)w32, cg);
