@@ -45,6 +45,16 @@ __device__ __forceinline__ f4v mfma32(uint4 a, uint4 b, f4v c) {
 
 // ----------------------------------------------------------- convolution --
 
+// compute units of the current device (persistent grids)
+static int cu_count() {
+  static int cus[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus[dev] = 256;
+  return cus[dev] > 0 ? cus[dev] : 256;
+}
+
 constexpr int CONV_TX = 16, CONV_TY = 4;  // voxel tile in d1, d2 (d3: MT)
 constexpr int LDS_PAD = 8;                // bf16 elements of padding per staged voxel
 constexpr int CS_MAX = 32;                // channels staged per slab
@@ -337,11 +347,86 @@ __global__ __launch_bounds__(256) void k_conv3_w27(int n1, int n2, int n3, const
   conv_epilogue<COUT, MT, MODE, HN * RS>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds, pout);
 }
 
+// The 8-channel form for 16 bf16 outputs (the first layer) as a persistent
+// kernel: a block keeps the 7 weight fragments in registers for all of its
+// tiles and loads the next tile's input while it computes the current one.
+template <int MT>
+__global__ __launch_bounds__(256) void k_conv3_p8(int n1, int n2, int n3, const uint16_t* __restrict__ in, int cin,
+                                                  View iv, const uint16_t* __restrict__ wt,
+                                                  const float* __restrict__ bias, int relu, void* __restrict__ out,
+                                                  View ov, uint16_t* __restrict__ pout) {
+  constexpr int HX = CONV_TX + 2, HY = CONV_TY + 2, HZ = MT + 2, HN = HX * HY * HZ;
+  constexpr int RS = 24, NU = (HN + 255) / 256;
+  __shared__ __align__(16) uint16_t lds[HN * RS];
+  const int tx = (n1 + CONV_TX - 1) / CONV_TX, ty = (n2 + CONV_TY - 1) / CONV_TY, tz = (n3 + MT - 1) / MT;
+  const int ntiles = tx * ty * tz;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  uint4 bw[7];
+#pragma unroll
+  for (int pr = 0; pr < 7; pr++) {
+    const int tap = 4 * pr + lq;
+    bw[pr] = tap < 27 ? *(const uint4*)(wt + (int64_t)lr * 27 * cin + tap * cin) : make_uint4(0, 0, 0, 0);
+  }
+  uint4 v[NU];
+  auto load = [&](int t) {
+    const int x0 = (t % tx) * CONV_TX, y0 = ((t / tx) % ty) * CONV_TY, z0 = (t / (tx * ty)) * MT;
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+      const int h = tid + 256 * u;
+      const int gx = x0 - 1 + h % HX, gy = y0 - 1 + (h / HX) % HY, gz = z0 - 1 + h / (HX * HY);
+      v[u] = make_uint4(0, 0, 0, 0);
+      if (h < HN && gx >= 0 && gx < n1 && gy >= 0 && gy < n2 && gz >= 0 && gz < n3)
+        v[u] = *(const uint4*)(in + (gx + (int64_t)n1 * (gy + (int64_t)n2 * gz)) * iv.cs + iv.co);
+    }
+  };
+  int t = blockIdx.x;
+  if (t < ntiles) load(t);
+  for (; t < ntiles; t += gridDim.x) {
+    const int x0 = (t % tx) * CONV_TX, y0 = ((t / tx) % ty) * CONV_TY, z0 = (t / (tx * ty)) * MT;
+    __syncthreads();  // the previous tile's output is stored
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+      const int h = tid + 256 * u;
+      if (h < HN) *(uint4*)(lds + h * RS) = v[u];
+    }
+    if (t + (int)gridDim.x < ntiles) load(t + gridDim.x);
+    __syncthreads();
+    f4v acc[MT][1];
+#pragma unroll
+    for (int m = 0; m < MT; m++) acc[m][0] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int pr = 0; pr < 7; pr++) {
+      const int tap = min(4 * pr + lq, 26);  // taps past the 27th read tap 26 against zero weights
+      const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
+#pragma unroll
+      for (int m = 0; m < MT; m++) {
+        const int row = wave * MT + m, y = row % CONV_TY, z = row / CONV_TY;
+        const int h = ((z + dz) * HY + (y + dy)) * HX + (lr + dx);
+        acc[m][0] = mfma32(*(const uint4*)(lds + h * RS), bw[pr], acc[m][0]);
+      }
+    }
+    conv_epilogue<16, MT, 0, HN * RS>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds, pout);
+  }
+}
+
 template <int MODE, int COUT, int MT>
 static void conv_launch(int n1, int n2, int n3, const uint16_t* in, int cin, int cused, View iv, Coarse lo,
                         const uint16_t* w, const float* bias, int relu, void* out, View ov, hipStream_t s,
                         uint16_t* pout) {
   const int nb = ((n1 + CONV_TX - 1) / CONV_TX) * ((n2 + CONV_TY - 1) / CONV_TY) * ((n3 + MT - 1) / MT);
+  if constexpr (COUT == 16 && MODE == 0) {
+    if (cused == 8 && lo.ch == 0) {
+      const int nt = ((n1 + CONV_TX - 1) / CONV_TX) * ((n2 + CONV_TY - 1) / CONV_TY) * ((n3 + MT - 1) / MT);
+      static int per_cu = 0;  // resident blocks per CU
+      if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3_p8<MT>, 256, 0) != hipSuccess ||
+                      per_cu < 1))
+        per_cu = 1;
+      k_conv3_p8<MT><<<std::min(nt, per_cu * cu_count()), 256, 0, s>>>(n1, n2, n3, in, cin, iv, w, bias, relu, out,
+                                                                      ov, pout);
+      return;
+    }
+  }
   if constexpr (COUT <= 32) {
     if (cused % 16 != 0) {
       k_conv3_w27<COUT, MT, MODE, 8><<<nb, 256, 0, s>>>(n1, n2, n3, in, cin, cused, iv, lo, w, bias, relu, out, ov, pout);
@@ -911,15 +996,6 @@ __global__ void k_pack(int cin, int cout, const float* __restrict__ W, uint16_t*
 namespace {
 
 inline int nblk(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
-// compute units of the current device (persistent grids)
-int cu_count() {
-  static int cus[64] = {};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    cus[dev] = 256;
-  return cus[dev] > 0 ? cus[dev] : 256;
-}
 // Element kernels over voxel x channel index spaces loop over a capped grid:
 // a dispatch holds at most 2^32 - 1 work-items per dimension, which one
 // thread per element would exceed at 512^3 x 32 channels.
