@@ -262,6 +262,31 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevel f, MgLevel c, const CgD
   down_child(f, c, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
 }
 
+// The same pass with the children laid out along the lanes: a block is a
+// row of 32 coarse cells (64 fine cells in d1, one lane each) and its four
+// waves the four (d2, d3) child rows, so that every load of the level's
+// arrays is one contiguous run per wave.  Children are summed in pairs along
+// d1 (shuffle), then over the four rows (LDS).
+__global__ __launch_bounds__(256) void k_mg_down_rows(MgLevel f, MgLevel c, const CgDev* cg) {
+  if (mg_skip(cg)) return;
+  __shared__ float part[4][32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nbx = (c.m1 + 31) / 32;
+  const int bx = blockIdx.x % nbx, cj = (blockIdx.x / nbx) % c.m2, ck = blockIdx.x / (nbx * c.m2);
+  const Ijk F{64 * bx + lane, 2 * cj + (w & 1), 2 * ck + (w >> 1)};
+  float v = 0.0f;
+  if (F.i < f.m1 && F.j < f.m2 && F.k < f.m3) {
+    const int64_t p = lidx(f, F.i, F.j, F.k);
+    auto x1 = [&](int64_t q, int, int, int) { return f.wd[q] * f.b[q]; };
+    v = f.b[p] - apply_lvl(f, F, p, x1(p, 0, 0, 0), x1);
+  }
+  v += __shfl_xor(v, 1, 64);
+  if ((lane & 1) == 0) part[w][lane >> 1] = v;
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < 32 && 32 * bx + t < c.m1) c.b[lidx(c, 32 * bx + t, cj, ck)] = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+}
+
 __global__ __launch_bounds__(256) void k_mg_up(MgLevel f, MgLevel c, float oc, const CgDev* cg) {
   if (mg_skip(cg)) return;
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -627,6 +652,15 @@ __global__ __launch_bounds__(256) void k_mg_up1(Dims d, const float* __restrict_
 
 inline int blocks256(int64_t n) { return (int)((n + 255) / 256); }
 
+// down-passes from level 1 to level lb (one kernel each)
+inline void down_levels(MgWork& m, const CgDev* cg, hipStream_t s) {
+  MgLevel* lv = m.L.lv;
+  for (int l = 1; l < m.lb; l++) {
+    const MgLevel& c = lv[l + 1];
+    k_mg_down_rows<<<((c.m1 + 31) / 32) * c.m2 * c.m3, 256, 0, s>>>(lv[l], c, cg);
+  }
+}
+
 template <int OP>
 int setup_t(MgWork& m, const Dims& d, const double* kap, const double* pse, double scale) {
   hipStream_t s = stream();
@@ -661,7 +695,7 @@ int vcycle_t(MgWork& m, const Dims& d, const float* r32, float* z, const CgDev* 
     k_mg_restrict1<<<blocks256(lvl_n(lv[1])), 256, 0, s>>>(d, m.t, lv[1], cg);
   }
   if (tm) ELSIM_TRY(prof_event(ELSIM_PROF_MG_DOWN, 1));
-  for (int l = 1; l < m.lb; l++) k_mg_down<<<blocks256(8 * lvl_n(lv[l + 1])), 256, 0, s>>>(lv[l], lv[l + 1], cg);
+  down_levels(m, cg, s);
   k_mg_bottom<<<1, 1024, 0, s>>>(m.dlv, m.lb, m.nlev, m.ainv, g.oc, cg);
   for (int l = m.lb - 1; l >= 1; l--) k_mg_up<<<blocks256(lvl_n(lv[l])), 256, 0, s>>>(lv[l], lv[l + 1], g.oc, cg);
   if (tm) ELSIM_TRY(prof_event(ELSIM_PROF_MG_UP, 0));
@@ -700,7 +734,7 @@ int vcycle_fused_t(MgWork& m, const Dims& d, const float* r_in, const float* w32
     k_mg_down4f<OP, false><<<nt, 256, 0, s>>>(d, r_in, w32, m.dinv, use_alpha, m.omega, r_out, m.kap32, m.pse32,
                                               sc, lv[1], cg, partials);
   if (tm) ELSIM_TRY(prof_event(ELSIM_PROF_MG_DOWN, 1));
-  for (int l = 1; l < m.lb; l++) k_mg_down<<<blocks256(8 * lvl_n(lv[l + 1])), 256, 0, s>>>(lv[l], lv[l + 1], cg);
+  down_levels(m, cg, s);
   k_mg_bottom<<<1, 1024, 0, s>>>(m.dlv, m.lb, m.nlev, m.ainv, g.oc, cg);
   for (int l = m.lb - 1; l >= 1; l--) k_mg_up<<<blocks256(lvl_n(lv[l])), 256, 0, s>>>(lv[l], lv[l + 1], g.oc, cg);
   const float* r = r_out ? r_out : r_in;
