@@ -672,13 +672,7 @@ int galerkin_step(SbmState& s, int op, double* x, const double* kap, const doubl
   const Dims& d = s.d;
   const double sc = 1.0 / (s.P.dx * s.P.dx);
   double num = 0.0, den = 0.0;
-  double* scr = (double*)scratch(1, (size_t)d.N() * sizeof(double) + sizeof(double));
-  if (!scr) return ELSIM_ERR_ALLOC;
-  double* ddot = scr + d.N();
-  ELSIM_TRY(launch_dot_resid(op, d, x, kap, nullptr, sc, rhs, dv, &num));
-  ELSIM_TRY(launch_matvec_f64(op, d, dv, kap, nullptr, sc, scr, ddot, true));
-  ELSIM_HIP(hipMemcpyAsync(&den, ddot, sizeof(double), hipMemcpyDeviceToHost, stream()));
-  ELSIM_HIP(hipStreamSynchronize(stream()));
+  ELSIM_TRY(launch_galerkin_dots(op, d, x, kap, sc, rhs, dv, &num, &den));
   const double alpha = (den < 0.0 && std::isfinite(num / den)) ? num / den : 0.0;
   if (alpha != 0.0) {
     k_ws_axpy<<<col_grid(d), dim3(TX, TY, 1), 0, stream()>>>(d, x, dv, alpha);
@@ -871,13 +865,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
         k_ws_apply<<<grid, blk, 0, stv>>>(d, pd, (const float*)s.nn_pred.p, 1.0, pd);
         ELSIM_CHECK_LAUNCH();
         double num = 0.0, den = 0.0;
-        double* scr = (double*)scratch(1, (size_t)d.N() * sizeof(double) + sizeof(double));
-        if (!scr) return ELSIM_ERR_ALLOC;
-        double* ddot = scr + d.N();
-        ELSIM_TRY(launch_dot_resid(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), pd, &num));
-        ELSIM_TRY(launch_matvec_f64(ELSIM_OP_PHP, d, pd, s.g(s.kapp), nullptr, sc, scr, ddot, true));
-        ELSIM_HIP(hipMemcpyAsync(&den, ddot, sizeof(double), hipMemcpyDeviceToHost, stv));
-        ELSIM_HIP(hipStreamSynchronize(stv));
+        ELSIM_TRY(launch_galerkin_dots(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), sc, s.g(s.rhsp), pd, &num, &den));
         const double alpha = (den < 0.0 && std::isfinite(num / den)) ? num / den : 0.0;  // den = p.L p < 0
         if (alpha != 0.0) {
           k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), pd, alpha);

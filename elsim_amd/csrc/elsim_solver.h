@@ -103,6 +103,11 @@ int launch_resid_store(int op, const Dims& d, const double* x, const double* kap
 // sum over the interior of dv * (rhs - A x), x's ghosts as stored (host result).
 int launch_dot_resid(int op, const Dims& d, const double* x, const double* kap, const double* pse, double scale,
                      const double* rhs, const double* dv, double* out);
+// The Galerkin step's dot products along dv (phP or phE operator, host
+// results): num = dv.(rhs - A x), x's ghosts as stored; den = dv.(A dv) with
+// the homogeneous rule.
+int launch_galerkin_dots(int op, const Dims& d, const double* x, const double* kap, double scale, const double* rhs,
+                         const double* dv, double* num, double* den);
 
 static_assert(sizeof(CgDev) <= 256, "CgDev must fit the 256 bytes a bound work space reserves for it");
 

@@ -247,6 +247,29 @@ __global__ __launch_bounds__(256) void k_dot_resid(Dims d, const double* __restr
   block_partials<1>(acc, partials);
 }
 
+// num = dv.(rhs - A x) (x's ghosts as stored) and den = dv.(A dv) (the
+// homogeneous rule): the two dot products of a Galerkin step along dv, in
+// one kernel -- per tile a sweep over x, then one over dv, which finds the
+// coefficient in cache -- instead of k_dot_resid and a k_matvec whose output
+// only fed its dot product.  Same per-tile sums as those two kernels.
+// Partials: slot 0 num, slot 1 den.
+template <int OP>
+__global__ __launch_bounds__(256) void k_galerkin_dots(Dims d, const double* __restrict__ x,
+                                                       const double* __restrict__ kap, double scale,
+                                                       const double* __restrict__ rhs, const double* __restrict__ dv,
+                                                       double* partials) {
+  double acc[2] = {0.0, 0.0};
+  const int nt = stencil_ntiles<double>(d);
+  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    const StIdx si = stencil_idx<Rows<double>::SJ>(d, tile);
+    sweep<double, OP, false>(d, si, x, kap, nullptr, scale,
+                             [&](int64_t c, int64_t n, double, double v) { acc[0] += dv[c] * (rhs[n] - v); });
+    sweep<double, OP, true>(d, si, dv, kap, nullptr, scale,
+                            [&](int64_t, int64_t, double pc, double v) { acc[1] += pc * v; });
+  }
+  block_partials<2>(acc, partials);
+}
+
 // ------------------------------------------------------------- Jacobi --
 
 
@@ -1157,6 +1180,35 @@ int launch_dot_resid(int op, const Dims& d, const double* x, const double* kap, 
   double t = 0.0;
   for (double v : h) t += v;
   *out = t;
+  return ELSIM_OK;
+}
+
+int launch_galerkin_dots(int op, const Dims& d, const double* x, const double* kap, double scale, const double* rhs,
+                         const double* dv, double* num, double* den) {
+  const int nb = stencil_ntiles<double>(d);
+  if (nb >= kMaxRedBlocks) {
+    set_error("grid too large for the reduction workspace");
+    return ELSIM_ERR_ARG;
+  }
+  double* P = workspace().partials;
+  hipStream_t s = stream();
+  switch (op) {
+    case ELSIM_OP_PHP: k_galerkin_dots<ELSIM_OP_PHP><<<nb, TX, 0, s>>>(d, x, kap, scale, rhs, dv, P); break;
+    case ELSIM_OP_PHE: k_galerkin_dots<ELSIM_OP_PHE><<<nb, TX, 0, s>>>(d, x, kap, scale, rhs, dv, P); break;
+    default: set_error("Galerkin step: operator %d (phP or phE)", op); return ELSIM_ERR_ARG;
+  }
+  // den summed as the operator's dot product is (one-block finaliser, into
+  // the unused end of slot 0), num on the host in block order, as before
+  double* dden = P + kMaxRedBlocks - 1;
+  k_fin<1><<<1, FIN_T, 0, s>>>(FIN_DOT, nb, P + kMaxRedBlocks, nullptr, dden, 0.0);
+  ELSIM_CHECK_LAUNCH();
+  std::vector<double> h(nb);
+  ELSIM_HIP(hipMemcpyAsync(h.data(), P, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipMemcpyAsync(den, dden, sizeof(double), hipMemcpyDeviceToHost, s));
+  ELSIM_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double v : h) t += v;
+  *num = t;
   return ELSIM_OK;
 }
 
