@@ -108,31 +108,33 @@ def matvec_roofline(n, reps=20):
 # direction: r, p, dinv in; p out.  The mixed solver moves the solution
 # update into the direction kernel: update r, w, dinv in, r out; direction y,
 # r, p, dinv in, y, p out.  Its dinv is BF16 (2 B), the FP64 solver's FP64.
-# The multigrid-preconditioned solver's update reads r, w, dinv and writes r
-# and the pre-smoothed iterate x1 (20 B); its direction kernel reads y, p, z
-# and writes y, p (20 B).  The V-cycle's fine-level down-sweep reads x1, kappa,
-# r and writes the level-1 right-hand side (12 + 0.5 B per voxel); the
-# up-sweep reads x1, kappa, r, dinv and the level-1 iterate and writes z
-# (20 + 0.5 B); the coarse levels move about 1/7 of the fine bytes again
-# (not counted in "vcycle").
+# The multigrid-preconditioned solver (n1 a multiple of 4) folds the residual
+# update into the V-cycle's fine down-sweep: it reads r, w, dinv, kappa and
+# writes the new r (to a second buffer) and the level-1 right-hand side
+# (16 + 4 + 0.5 B per voxel); the up-sweep reads r, dinv, kappa and the
+# level-1 iterate and writes z (12 + 0.5 + 4 B); the direction kernel reads
+# y, p, z and writes y, p (20 B).  The coarse levels move about 1/7 of the
+# fine bytes again (not counted in "vcycle").
 KERNEL_BYTES = {"op32": 12, "upd32": 14, "dir32": 22, "op64": 24, "upd64": 56, "dir64": 32, "upd_mg": 20,
-                "dir_mg": 20, "vcycle": 33, "mg_down": 12.5, "mg_up": 20.5}
+                "dir_mg": 20, "vcycle": 37, "mg_down": 20.5, "mg_up": 16.5}
 KERNEL_NAMES = {"op32": "k_matvec4 (7-point operator, 4 voxels a lane, + p.Ap)", "upd32": "k_mx_update (r -= a w, r.z)",
                 "dir32": "k_mx_dir (y += a p, p = z + b p)", "op64": "k_matvec<double>", "upd64": "k_pcg64_update",
                 "dir64": "k_pcg64_dir", "upd_mg": "k_mg_update (r -= a w, x1 = omega dinv r)",
                 "dir_mg": "k_mg_dir3 (y += a p, p = z + b p)",
                 "vcycle": "V-cycle (fine-level bytes; all levels' kernels)",
-                "mg_down": "k_mg_down4 (x1 -> sigma (r - A x1), restricted in registers)",
-                "mg_up": "k_mg_up4 (x2 = x1 + oc P x_c, z = x2 + omega dinv (r - A x2))"}
+                "mg_down": "k_mg_down4f (r = r_in - alpha w, x1 = omega dinv r formed in the loaders, "
+                           "sigma (r - A x1) restricted in registers)",
+                "mg_up": "k_mg_up4f (x2 = omega dinv r + oc P x_c in the loaders, z = x2 + omega dinv (r - A x2))"}
 
 
 def profiled_step(E, s, mode, nsteps=3):
     """A few more steps with event brackets around the first CG iteration of
-    every batch: mean in-loop duration of the operator, update and direction
-    kernels (and the V-cycle and its fine-level sweeps)."""
+    every batch of the phP solves: mean in-loop duration of the operator,
+    update and direction kernels (and the V-cycle and its fine-level
+    sweeps)."""
     L = E.lib()
     P = E._lib
-    E.check(L.elsim_prof_enable(1))
+    E.check(L.elsim_prof_enable(2 + E_OP_PHP))
     prof = P.KernelProf()
     E.check(L.elsim_prof_read(C.byref(prof)))  # reset
     for _ in range(nsteps):

@@ -235,23 +235,41 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
     __syncthreads();
     if (cs == 32) {
       // one 16x16x32 MFMA per output tile and tap: lane group lq holds
-      // channels 8 lq .. 8 lq + 7 of the slab
-      for (int tap = 0; tap < 27; tap++) {
+      // channels 8 lq .. 8 lq + 7 of the slab.  The next tap's weight
+      // fragments (global) and input fragments (LDS) are loaded before this
+      // tap's MFMAs are issued: with one or two waves per SIMD nothing else
+      // would cover those latencies.
+      auto lda = [&](int tap, uint4 (&a)[MT]) {
         const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
-        uint4 a[MT];
 #pragma unroll
         for (int m = 0; m < MT; m++) {
           const int row = wave * MT + m, y = row % CONV_TY, z = row / CONV_TY;
           const int h = ((z + dz) * HY + (y + dy)) * HX + (lr + dx);
           a[m] = *(const uint4*)(lds + h * rs + 8 * lq);
         }
+      };
+      auto ldb = [&](int tap, uint4 (&bw)[NT]) {
         const int k = tap * cin + slab + 8 * lq;
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const uint4 bw = *(const uint4*)(wt + (int64_t)(t * 16 + lr) * K + k);
+        for (int t = 0; t < NT; t++) bw[t] = *(const uint4*)(wt + (int64_t)(t * 16 + lr) * K + k);
+      };
+      uint4 an[MT], bn[NT];
+      lda(0, an);
+      ldb(0, bn);
+      for (int tap = 0; tap < 27; tap++) {
+        uint4 a[MT], bw[NT];
 #pragma unroll
-          for (int m = 0; m < MT; m++) acc[m][t] = mfma32(a[m], bw, acc[m][t]);
+        for (int m = 0; m < MT; m++) a[m] = an[m];
+#pragma unroll
+        for (int t = 0; t < NT; t++) bw[t] = bn[t];
+        if (tap + 1 < 27) {
+          lda(tap + 1, an);
+          ldb(tap + 1, bn);
         }
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int m = 0; m < MT; m++) acc[m][t] = mfma32(a[m], bw[t], acc[m][t]);
       }
       continue;
     }

@@ -410,14 +410,27 @@ __global__ __launch_bounds__(256) void k_ws_apply(Dims d, const double* base, co
   each(d, ci, [&](int64_t c, int64_t n, int) { out[c] = base[c] + scale * (double)pred[n]; });
 }
 
-// out = a - b over the whole ghosted box.
-__global__ __launch_bounds__(256) void k_ws_diff(int64_t G, const double* __restrict__ a, const double* __restrict__ b,
-                                                 double* __restrict__ out) {
+// out = a - prev, then prev = a, over the whole ghosted box (the last
+// step's change of a field and the copy kept for the next one, in one pass).
+__global__ __launch_bounds__(256) void k_ws_diff_keep(int64_t G, const double* __restrict__ a,
+                                                      double* __restrict__ prev, double* __restrict__ out) {
   const int64_t e = (int64_t)blockIdx.x * (FLAT_T * FLAT_U) + threadIdx.x;
+  double av[FLAT_U], bv[FLAT_U];
 #pragma unroll
   for (int u = 0; u < FLAT_U; u++) {
     const int64_t q = e + u * FLAT_T;
-    if (q < G) out[q] = a[q] - b[q];
+    if (q < G) {
+      av[u] = a[q];
+      bv[u] = prev[q];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < FLAT_U; u++) {
+    const int64_t q = e + u * FLAT_T;
+    if (q < G) {
+      out[q] = av[u] - bv[u];
+      prev[q] = av[u];
+    }
   }
 }
 
@@ -722,14 +735,15 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
     ELSIM_TRY(ensure_ws_buffers(s));
     std::swap(s.ws_dir3, s.ws_dir2);
     std::swap(s.ws_dir2, s.ws_dir);
-    k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.php), s.g(s.php_prev), s.g(s.ws_dir));
-    k_ws_diff<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.phe), s.g(s.phe_prev), s.g(s.ws_dir_e));
+    k_ws_diff_keep<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.php), s.g(s.php_prev), s.g(s.ws_dir));
+    k_ws_diff_keep<<<flat_blocks(d.G()), FLAT_T, 0, stv>>>(d.G(), s.g(s.phe), s.g(s.phe_prev), s.g(s.ws_dir_e));
     ELSIM_CHECK_LAUNCH();
     ws_hist = s.ws_p_hist;
+  } else {
+    ELSIM_HIP(hipMemcpyAsync(s.php_prev.p, s.php.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
+    ELSIM_HIP(hipMemcpyAsync(s.phe_prev.p, s.phe.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
   }
   s.ws_p_hist = ws_on ? std::min(ws_hist + 1, 2) : 0;
-  ELSIM_HIP(hipMemcpyAsync(s.php_prev.p, s.php.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
-  ELSIM_HIP(hipMemcpyAsync(s.phe_prev.p, s.phe.p, (size_t)d.G() * sizeof(double), hipMemcpyDeviceToDevice, stv));
   s.nprev++;
 
   // 1. Cahn-Hilliard (SBM:397-443)

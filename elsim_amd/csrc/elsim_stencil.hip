@@ -766,6 +766,8 @@ namespace {
 // Optional event brackets around the first CG iteration of each batch.
 struct Prof {
   bool on = false, ready = false;
+  int op = -1;  // time the solves of this operator only (-1: every solve)
+  bool rec[ELSIM_PROF_N] = {};  // slots bracketed since the last collection
   hipEvent_t ev[2 * ELSIM_PROF_N];
   elsim_kernel_prof acc{};
 };
@@ -775,17 +777,22 @@ template <class F>
 int timed(bool on, int slot, hipStream_t s, F&& launch) {
   if (on) ELSIM_HIP(hipEventRecord(g_prof.ev[2 * slot], s));
   launch();
-  if (on) ELSIM_HIP(hipEventRecord(g_prof.ev[2 * slot + 1], s));
+  if (on) {
+    ELSIM_HIP(hipEventRecord(g_prof.ev[2 * slot + 1], s));
+    g_prof.rec[slot] = true;
+  }
   return ELSIM_OK;
 }
 
 // After a synchronisation: add the bracketed durations of the slots.
 int prof_collect(int first, int last) {
   for (int sl = first; sl <= last; sl++) {
+    if (!g_prof.rec[sl]) continue;
     float ms = 0.f;
     ELSIM_HIP(hipEventElapsedTime(&ms, g_prof.ev[2 * sl], g_prof.ev[2 * sl + 1]));
     g_prof.acc.count[sl] += 1;
     g_prof.acc.ms_total[sl] += ms;
+    g_prof.rec[sl] = false;
   }
   return ELSIM_OK;
 }
@@ -882,7 +889,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     fin1(FIN_RZ, fb);
     int it_prev = 0;
     for (;;) {
-      const bool pr = g_prof.on;
+      const bool pr = g_prof.on && (g_prof.op < 0 || g_prof.op == OP);
       for (int b = 0; b < batch; b++) {
         const bool t = pr && b == 0;
         ELSIM_TRY(timed(t, ELSIM_PROF_OP64, s, [&] {
@@ -967,7 +974,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     BatchPlan plan;
     int nrun = 2;  // a few iterations before the first poll: warm-started solves often need only these
     for (;;) {
-      const bool pr = g_prof.on;
+      const bool pr = g_prof.on && (g_prof.op < 0 || g_prof.op == OP);
       for (int b = 0; b < nrun; b++) {
         const bool t = pr && b == 0;
         ELSIM_TRY(timed(t, ELSIM_PROF_OP32, s, [&] {
@@ -1040,7 +1047,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     BatchPlan plan;
     int nrun = batch;
     for (;;) {
-      const bool pr = g_prof.on;
+      const bool pr = g_prof.on && (g_prof.op < 0 || g_prof.op == OP);
       for (int b = 0; b < nrun; b++) {
         const bool t = pr && b == 0;
         ELSIM_TRY(timed(t, ELSIM_PROF_OP32, s, [&] {
@@ -1104,7 +1111,10 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
 }  // namespace
 
 int prof_event(int slot, int end) {
-  if (g_prof.on) ELSIM_HIP(hipEventRecord(g_prof.ev[2 * slot + (end ? 1 : 0)], stream()));
+  if (g_prof.on) {
+    ELSIM_HIP(hipEventRecord(g_prof.ev[2 * slot + (end ? 1 : 0)], stream()));
+    if (end) g_prof.rec[slot] = true;
+  }
   return ELSIM_OK;
 }
 
@@ -1368,7 +1378,9 @@ int elsim_prof_enable(int on) {
     for (hipEvent_t& e : g_prof.ev) ELSIM_HIP(hipEventCreate(&e));
     g_prof.ready = true;
   }
+  if (on < 0 || on > 2 + ELSIM_OP_POT_DD) return ELSIM_ERR_ARG;
   g_prof.on = on != 0;
+  g_prof.op = on >= 2 ? on - 2 : -1;
   return ELSIM_OK;
 }
 

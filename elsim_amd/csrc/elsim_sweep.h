@@ -216,7 +216,11 @@ __device__ __forceinline__ void for_col(const Dims& d, const ColIdx& ci, F&& f) 
 // and plane, its ghosted index and the four values of p and A p.  EDGE: the
 // box has more than one d1 tile, so the first and last lane of a row segment
 // may load their d1 neighbour from the next tile (n1 <= 256 leaves that code,
-// and its registers, out).  Every lane of the block must call it.
+// and its registers, out).  Every lane of a 256-thread block must call it,
+// and the same number of times (a block barrier per plane: the row groups'
+// boundary rows go through LDS to the neighbouring groups, so only the
+// block's outer halo rows are loaded -- the FP32 down-sweep fetched 1.96
+// times its algorithmic bytes when every group loaded its own).
 constexpr int SJ4 = 4, SKC4 = 8;
 __host__ __device__ inline int mv4_tw(const Dims& d) { return d.n1 <= 64 ? 64 : (d.n1 <= 128 ? 128 : 256); }
 // usable when n1 is a multiple of 4 and ghosted indices fit 32 bits
@@ -247,6 +251,7 @@ template <int OP, bool EDGE, class LD, class LD1, class EPI>
 __device__ __forceinline__ void mv4_tile(const Dims& d, int tile, LD&& ld, LD1&& ld1, const float* __restrict__ kap,
                                          const float* __restrict__ pse, float scale, EPI&& epi) {
   constexpr int SJ = SJ4;
+  __shared__ float4 xch[2][256][4];  // row exchange: first / last row of p and kappa per lane
   const TileGrid tg = mv4_tiles(d);
   const int nt = tg.gx * tg.gy * tg.gz;
   const int tw = mv4_tw(d), lpr = tw / 4, rpw = 64 / lpr;
@@ -310,8 +315,34 @@ __device__ __forceinline__ void mv4_tile(const Dims& d, int tile, LD&& ld, LD1&&
       pa[t] = ld(rc[t] + off + step, ia, jr[t], k + kdir);
       ka[t] = ld4(kap, rc[t] + off + step);
     }
-    const float4 hlp = ld(clo + off, ia, jlo, k), hlk = ld4(kap, clo + off);
-    const float4 hhp = ld(chi + off, ia, jhi, k), hhk = ld4(kap, chi + off);
+    // d2 halo rows of this plane: the last row of the row group below and
+    // the first row of the group above, which those groups' lanes hold in
+    // registers -- exchanged through LDS (double-buffered, one barrier per
+    // plane); only the block's first and last groups load them.
+    const int tid = threadIdx.x, bsel = kk & 1;
+    xch[bsel][tid][0] = pc[0];
+    xch[bsel][tid][1] = kc[0];
+    xch[bsel][tid][2] = pc[SJ - 1];
+    xch[bsel][tid][3] = kc[SJ - 1];
+    const bool glo = tid < lpr, ghi = tid >= 256 - lpr;
+    float4 hlp, hlk, hhp, hhk;
+    if (glo) {
+      hlp = ld(clo + off, ia, jlo, k);
+      hlk = ld4(kap, clo + off);
+    }
+    if (ghi) {
+      hhp = ld(chi + off, ia, jhi, k);
+      hhk = ld4(kap, chi + off);
+    }
+    __syncthreads();
+    if (!glo) {
+      hlp = xch[bsel][tid - lpr][2];
+      hlk = xch[bsel][tid - lpr][3];
+    }
+    if (!ghi) {
+      hhp = xch[bsel][tid + lpr][0];
+      hhk = xch[bsel][tid + lpr][1];
+    }
     float psv[SJ][4];
     if (Rule<OP>::diag) {
 #pragma unroll

@@ -378,12 +378,15 @@ int elsim_dev_mg_apply(int op, int n1, int n2, int n3, const double* kap, const 
 /* Kernel timing inside the CG loops.  When enabled, the first iteration of
  * every batch of CG iterations is bracketed by HIP events on the library
  * stream (operator, update, direction kernels, V-cycle); read-out averages them over
- * the batches whose iteration ran, then resets.  Slots: ELSIM_PROF_*. */
+ * the batches whose iteration ran, then resets.  Slots: ELSIM_PROF_*.
+ * elsim_prof_enable(on): 0 off, 1 every solve, 2 + op the solves of
+ * operator op (ELSIM_OP_*) only. */
 enum {
   ELSIM_PROF_OP32 = 0, ELSIM_PROF_UPD32 = 1, ELSIM_PROF_DIR32 = 2,
   ELSIM_PROF_OP64 = 3, ELSIM_PROF_UPD64 = 4, ELSIM_PROF_DIR64 = 5,
   ELSIM_PROF_MG = 6,      /* the V-cycle of ELSIM_MODE_MIXED_MG, all its kernels */
-  ELSIM_PROF_MG_DOWN = 7, /* its fine-level down-sweep (r - A x1, restricted) */
+  ELSIM_PROF_MG_DOWN = 7, /* its fine-level down-sweep (r - A x1, restricted; with the
+                             residual update when n1 is a multiple of 4) */
   ELSIM_PROF_MG_UP = 8,   /* its fine-level up-sweep (prolongation, post-smoothing) */
   ELSIM_PROF_N = 9
 };
