@@ -2,11 +2,13 @@
 
 One step = CH update -> CnE solve -> outer {Butler-Volmer -> phP solve -> phE
 solve} -> current control (SBMES_3D_MPI_CH_Gr_CONJG_VD_T05_01_BQ_hx3.f90:397-840),
-every CG to relative residual 1e-8.  Two paths run on the same synthetic
+every CG to relative residual 1e-8.  The paths run on the same synthetic
 microstructure from the same initial state:
-  fp64  - FP64 Jacobi-preconditioned CG (the same preconditioner, all FP64);
-  fast  - FP32 Jacobi-PCG with FP64 reliable updates, plus the learned warm
-          start when a trained network is available;
+  fast  - FP32 CG preconditioned by one multigrid V-cycle, FP64 reliable
+          updates, extrapolated start plus the learned correction (the
+          packaged network); also with the extrapolated and the plain start;
+  jacobi - the same mixed solver with Jacobi preconditioning;
+  fp64  - FP64 Jacobi-preconditioned CG, the reference's plain start;
   ref   - the reference CG (unpreconditioned FP64): cost per iteration only,
           since its solves do not reach 1e-8 within the reference's caps.
 `value` is the fast path's seconds per step (lower is better).  Inputs are
@@ -50,14 +52,13 @@ E_OP_PHP = 1
 # The flat kernels sweep the whole ghosted box, (n+2)^3 elements.
 PMC_BYTES_PER_GHOSTED = {"upd32": (83892.2 * 2 + 67354.1) * 1024 / 258 ** 3,
                          "dir32": (117411.2 * 2 + 134168.1) * 1024 / 258 ** 3}
-# the phP operator covers the interior: bytes per interior voxel, same runs
-PMC_BYTES_PER_VOXEL_OP32 = (94073.9 * 2 + 66625.1) * 1024 / 256 ** 3
 # the multigrid path's kernels, same method (profiles/mg_kernels_256_pmc.md):
-# flat kernels over the ghosted box, the fine-level sweeps over the interior
-PMC_BYTES_PER_GHOSTED.update({"upd_mg": (100642.7 * 2 + 134168.1) * 1024 / 258 ** 3,
-                              "dir_mg": (100640.7 * 2 + 134168.1) * 1024 / 258 ** 3})
-PMC_BYTES_PER_VOXEL_MG = {"mg_up": (182570.4 * 2 + 67035.2) * 1024 / 256 ** 3,
-                          "mg_down": (126173.2 * 2 + 8208.0) * 1024 / 256 ** 3}
+# flat kernels over the ghosted box, the operator and the fine-level sweeps
+# over the interior (bytes per interior voxel)
+PMC_BYTES_PER_VOXEL_OP32 = (78082.8 * 2 + 67073.5) * 1024 / 256 ** 3
+PMC_BYTES_PER_GHOSTED.update({"dir_mg": (100640.7 * 2 + 134168.1) * 1024 / 258 ** 3})
+PMC_BYTES_PER_VOXEL_MG = {"mg_up": (191912.1 * 2 + 68654.6) * 1024 / 256 ** 3,
+                          "mg_down": (231135.0 * 2 + 76764.8) * 1024 / 256 ** 3}
 
 
 def log(*a):
@@ -395,7 +396,7 @@ def main():
         roof = {"bound": "hbm", "achieved": kernels[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": kernels[dom]["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                 "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per launch, profiles/mg_kernels_256_pmc.md "
-                                  "(multigrid kernels) and profiles/cg_kernels_256_pmc.md",
+                                  "(operator and multigrid kernels) and profiles/cg_kernels_256_pmc.md",
                 "kernel": kernels[dom]["kernel"],
                 "kernel_us": kernels[dom]["us"], "bytes_per_launch": KERNEL_BYTES[dom] * N,
                 "stencil": {"kernel": kernels["op32"]["kernel"], "us": kernels["op32"]["us"],

@@ -26,16 +26,18 @@
 //   * Coarsest level (at most 64 cells): a dense inverse, formed once per
 //     hierarchy by Gauss-Jordan elimination in FP64 in one block.
 //
-// The fine level is the CG's own FP32 operator.  The CG's residual update
-// also writes the pre-smoothed iterate x1 = omega D^-1 r; the down-sweep
-// forms r - A x1 with the four-voxels-a-lane tile walk of the CG operator
-// (mv4_tile) and sums it over 2 x 2 x 2 blocks in registers (a lane's four d1
-// voxels, its four rows and the tile's eight planes hold whole blocks), so
-// the fine residual is never stored; the up-sweep forms x2 = x1 + oc P x_c in
-// the loader and writes z = x2 + omega D^-1 (r - A x2).  Levels 1 .. lb-1 run
-// one kernel per level and direction (a thread per child cell, the 8 children
-// of a coarse cell in adjacent lanes, summed by three butterfly shuffles);
-// the levels of at most kMgBottomMax cells run in one single-block kernel.
+// The fine level is the CG's own FP32 operator.  The down-sweep forms
+// r - A x1 (x1 = omega D^-1 r, the pre-smoothed iterate) with the
+// four-voxels-a-lane tile walk of the CG operator (mv4_tile) and sums it
+// over 2 x 2 x 2 blocks in registers (a lane's four d1 voxels, its two rows
+// and the tile's eight planes hold whole blocks), so the fine residual is
+// never stored; the up-sweep forms x2 = x1 + oc P x_c in the loader and
+// writes z = x2 + omega D^-1 (r - A x2).  On boxes whose n1 is a multiple of
+// 4 the CG's residual update r -= alpha w runs inside the down-sweep and x1
+// is formed in both sweeps' loaders, never stored.  Levels 1 .. lb-1 run one
+// kernel per level and direction (down: a block per row of 32 coarse cells,
+// the children along the lanes; up: a thread per cell); the levels of at
+// most kMgBottomMax cells run in one single-block kernel.
 //
 // The operators are negative definite for the potentials (A = -A_hat) and
 // positive for CnE; the coarse levels hold A_hat = sigma A, sigma = -1 / +1,
@@ -362,9 +364,10 @@ __global__ __launch_bounds__(256) void k_mg_presmooth(int64_t G2, const float2* 
 
 // Down-sweep, four voxels a lane: c.b = restriction of sigma (r - A x1) to
 // level 1, summed in registers; partials of r.r (slot 0).  A lane's four
-// voxels are two level-1 cells in d1, its four rows two in d2, and the
-// tile's eight planes (aligned to 8) four in d3, so every level-1 cell is
-// summed by one lane, over its planes in the tile's marching order.
+// voxels are two level-1 cells in d1, its rows (SJ4, even, from an odd row)
+// whole level-1 rows in d2, and the tile's eight planes (aligned to 8) four
+// in d3, so every level-1 cell is summed by one lane, over its planes in the
+// tile's marching order.
 template <int OP, bool EDGE>
 __global__ __launch_bounds__(256) void k_mg_down4(Dims d, const float* __restrict__ x1, const float* __restrict__ r,
                                                   const float* __restrict__ kap, const float* __restrict__ pse,
