@@ -814,9 +814,13 @@ struct BatchPlan {
   double rr_base = -1.0;
   int it_base = 0;
   // after a poll: learn the rate from the iterations since the last one
+  // (also when the batch ended in a reliable-update request: h.rr is then
+  // the FP32 residual of the iteration that made it, h.iters counts it)
+  // (the first poll measures from the start: r.r in rmax2, iteration 0)
   void observe(const CgDev& h) {
-    if (rr_base > 0.0 && h.iters > it_base && h.rr > 0.0 && !h.need_reliable)
-      rate = std::pow(h.rr / rr_base, 1.0 / (h.iters - it_base));
+    const double base = rr_base > 0.0 ? rr_base : h.rmax2;
+    const int it0 = rr_base > 0.0 ? it_base : 0;
+    if (base > 0.0 && h.iters > it0 && h.rr > 0.0) rate = std::pow(h.rr / base, 1.0 / (h.iters - it0));
   }
   // the next batch from the state after the poll (and a reliable update,
   // which leaves the true residual in rmax2)
@@ -1045,7 +1049,10 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     const int fbd = (int)((G2 + (int64_t)FLAT_T * DIR_P - 1) / ((int64_t)FLAT_T * DIR_P));
     int it_prev = 0;
     BatchPlan plan;
-    int nrun = batch;
+    // two iterations before the first poll, then batches sized from the
+    // observed rate: the CnE solve converges a segment in about two
+    // iterations, and a batch sized past that runs only skipped launches
+    int nrun = 2;
     for (;;) {
       const bool pr = g_prof.on && (g_prof.op < 0 || g_prof.op == OP);
       for (int b = 0; b < nrun; b++) {
