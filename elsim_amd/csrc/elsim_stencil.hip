@@ -113,23 +113,37 @@ __global__ __launch_bounds__(256) void k_resid64(Dims d, const double* __restric
   block_partials<1>(acc, partials);
 }
 
-// ||b_eff||^2 with b_eff = r0 + A_hom x (the residual of a zero interior with
-// the boundary data of x); sets tol2 and, for the relative rule, done if the
-// initial residual already meets it.
+// k_resid64 and ||b_eff||^2 in one kernel, b_eff = r0 + A_hom x (the
+// residual of a zero interior with the boundary data of x): per tile the
+// residual sweep, then the homogeneous sweep of the same x (which finds x and
+// the coefficient in cache) reading back the r0 the first one wrote.
+// Partials: slot 0 r.r, slot 1 ||b_eff||^2 (FIN_BEFF then sets tol2 and, for
+// the relative rule, done if the start already meets it).
 template <int OP>
-__global__ __launch_bounds__(256) void k_beff(Dims d, const double* __restrict__ x, const double* __restrict__ kap,
-                                              const double* __restrict__ pse, double scale,
-                                              const double* __restrict__ r0, double rtol, CgDev* cg,
-                                              double* partials) {
-  double acc[1] = {0.0};
+__global__ __launch_bounds__(256) void k_resid64_beff(Dims d, const double* __restrict__ x,
+                                                      const double* __restrict__ kap, const double* __restrict__ pse,
+                                                      double scale, const double* __restrict__ rhs,
+                                                      double* __restrict__ res, double* __restrict__ r0,
+                                                      double* __restrict__ prj, double* __restrict__ x0,
+                                                      double* partials) {
+  double acc[2] = {0.0, 0.0};
   const int nt = stencil_ntiles<double>(d);
-  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x)
-    sweep<double, OP, true>(d, stencil_idx<Rows<double>::SJ>(d, tile), x, kap, pse, scale,
-                            [&](int64_t c, int64_t, double, double v) {
-      const double b = r0[c] + v;
-      acc[0] += b * b;
+  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    const StIdx si = stencil_idx<Rows<double>::SJ>(d, tile);
+    sweep<double, OP, false>(d, si, x, kap, pse, scale, [&](int64_t c, int64_t n, double xc, double v) {
+      const double r = rhs[n] - v;
+      if (res) res[c] = r;
+      r0[c] = r;
+      if (prj) prj[c] = r;
+      x0[c] = xc;
+      acc[0] += r * r;
     });
-  block_partials<1>(acc, partials);
+    sweep<double, OP, true>(d, si, x, kap, pse, scale, [&](int64_t c, int64_t, double, double v) {
+      const double b = r0[c] + v;
+      acc[1] += b * b;
+    });
+  }
+  block_partials<2>(acc, partials);
 }
 
 // FP64 reference CG update (SBM:533-547, 688-705, 769-786), flat over the box.
@@ -871,13 +885,19 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   auto fin1 = [&](int op, int nb) { k_fin<1><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
   auto fin2 = [&](int op, int nb) { k_fin<2><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
   const bool fp32_inner = mode == ELSIM_MODE_MIXED || mode == ELSIM_MODE_MIXED_MG;
-  k_resid64<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, fp32_inner ? nullptr : res, r0,
-                                     fp32_inner ? nullptr : prj, x0, cg, P);
-  fin1(FIN_RESID, nb64);
   // b_eff's norm is not used when the rule is relative to ||r0|| (CnE's
   // increment rule): the finaliser then only sets the tolerance
-  if (!a.rel_r0) k_beff<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, r0, a.rtol, cg, P);
-  fin1(FIN_BEFF, a.rel_r0 ? 0 : nb64);
+  if (a.rel_r0) {
+    k_resid64<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, fp32_inner ? nullptr : res, r0,
+                                       fp32_inner ? nullptr : prj, x0, cg, P);
+    fin1(FIN_RESID, nb64);
+    fin1(FIN_BEFF, 0);
+  } else {
+    k_resid64_beff<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, fp32_inner ? nullptr : res, r0,
+                                            fp32_inner ? nullptr : prj, x0, P);
+    fin1(FIN_RESID, nb64);
+    k_fin<1><<<1, FIN_T, 0, s>>>(FIN_BEFF, nb64, P + kMaxRedBlocks, cg, nullptr, a.rtol);  // slot 1
+  }
   ELSIM_CHECK_LAUNCH();
   const int batch = kMaxBatch;
   if (mode == ELSIM_MODE_FP64_PCG) {
