@@ -900,6 +900,14 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   }
   ELSIM_CHECK_LAUNCH();
   const int batch = kMaxBatch;
+  // Multigrid path: a start that already meets the tolerance (phE in most
+  // steps of a warm-started run) needs no hierarchy, so one poll decides
+  // before the coefficient's set-up and the first V-cycle.
+  bool mg_start_done = false;
+  if (mode == ELSIM_MODE_MIXED_MG) {
+    ELSIM_TRY(poll(w, s));
+    mg_start_done = w.host->done != 0;
+  }
   if (mode == ELSIM_MODE_FP64_PCG) {
     double* dinv64 = (double*)w.dinv64.p;
     k_dinv<OP, double><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv64);
@@ -954,6 +962,8 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
     k_diff<<<gc, blk, 0, s>>>(d, a.x, x0, e64);
     k_true_resid<OP, true><<<nb64, blk, 0, s>>>(d, e64, a.kap, a.pse, a.scale, r0, nullptr, nullptr, cg, P);
     fin2(FIN_TRUE_FINAL, nb64);
+  } else if (mode == ELSIM_MODE_MIXED_MG && mg_start_done) {
+    // nothing to correct: x stays as it came in
   } else if (mode == ELSIM_MODE_MIXED_MG) {
     // FP32 CG preconditioned by one multigrid V-cycle, FP64 reliable updates
     // as in the Jacobi-preconditioned mixed path below.  Per iteration: the
