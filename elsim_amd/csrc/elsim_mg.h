@@ -21,6 +21,7 @@ struct MgLevel {
 constexpr int kMgMaxLev = 24;
 constexpr int kMgCoarseMax = 64;    // cells of the directly solved level
 constexpr int kMgBottomMax = 512;   // levels this small run in one single-block kernel
+constexpr int kMgBottomCells = 640; // ... with at most this many cells together (staged in LDS)
 
 struct MgLevels {
   MgLevel lv[kMgMaxLev + 1];
