@@ -53,12 +53,18 @@ __global__ __launch_bounds__(256) void k_matvec(Dims d, const T* __restrict__ p,
                                                 double* dot, CgDev* cg, int num_is_rz, double* partials) {
   if (cg && (cg->done || cg->need_reliable)) return;
   double acc[1] = {0.0};
-  const int nt = stencil_ntiles<T>(d);
-  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x)
-    sweep<T, OP, HOM>(d, stencil_idx<Rows<T>::SJ>(d, tile), p, kap, pse, scale, [&](int64_t c, int64_t n, T pc, T v) {
-      out[OUTG ? c : n] = v;
-      acc[0] += (double)pc * (double)v;
-    });
+  auto epi = [&](int64_t c, int64_t n, T pc, T v) {
+    out[OUTG ? c : n] = v;
+    acc[0] += (double)pc * (double)v;
+  };
+  if constexpr (std::is_same<T, double>::value) {
+    const int nt = ntiles64(d);
+    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) tile64<OP, HOM>(d, tile, p, kap, pse, scale, epi);
+  } else {
+    const int nt = stencil_ntiles<T>(d);
+    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x)
+      sweep<T, OP, HOM>(d, stencil_idx<Rows<T>::SJ>(d, tile), p, kap, pse, scale, epi);
+  }
   if (dot || cg) block_partials<1>(acc, partials);
 }
 
@@ -857,11 +863,13 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   hipStream_t s = stream();
   const dim3 gc = col_grid(d), blk(TX, TY, 1);
   const int nb64 = stencil_ntiles<double>(d), nb32 = stencil_ntiles<float>(d);
+  const int nbo64 = ntiles64(d);  // the FP64 operator's walk
   const int64_t G = d.G();
   const int fb = flat_blocks(G);
   const int64_t G2 = (G + 1) / 2;  // element pairs of the FP32 pair kernels
   const int fbu = (int)((G2 + (int64_t)FLAT_T * UPD_P - 1) / ((int64_t)FLAT_T * UPD_P));
-  if (nb64 > kMaxRedBlocks || nb32 > kMaxRedBlocks || fb > kMaxRedBlocks || mv4_ntiles(d) > kMaxRedBlocks) {
+  if (nb64 > kMaxRedBlocks || nbo64 > kMaxRedBlocks || nb32 > kMaxRedBlocks || fb > kMaxRedBlocks ||
+      mv4_ntiles(d) > kMaxRedBlocks) {
     set_error("grid too large for the reduction workspace");
     return ELSIM_ERR_ARG;
   }
@@ -925,9 +933,10 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
       for (int b = 0; b < batch; b++) {
         const bool t = pr && b == 0;
         ELSIM_TRY(timed(t, ELSIM_PROF_OP64, s, [&] {
-          k_matvec<double, OP, true, true><<<nb64, blk, 0, s>>>(d, prj, a.kap, a.pse, a.scale, wmx, nullptr, cg, 1, P);
+          k_matvec<double, OP, true, true><<<nbo64, blk, 0, s>>>(d, prj, a.kap, a.pse, a.scale, wmx, nullptr, cg, 1,
+                                                                 P);
         }));
-        fin1(FIN_ALPHA_RZ, nb64);
+        fin1(FIN_ALPHA_RZ, nbo64);
         ELSIM_TRY(timed(t, ELSIM_PROF_UPD64, s, [&] {
           k_pcg64_update<<<fb, FLAT_T, 0, s>>>(G, e64, prj, res, wmx, dinv64, cg, P);
         }));
@@ -948,8 +957,8 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   } else if (mode == ELSIM_MODE_FP64) {
     for (;;) {
       for (int b = 0; b < batch; b++) {
-        k_matvec<double, OP, true, true><<<nb64, blk, 0, s>>>(d, prj, a.kap, a.pse, a.scale, wmx, nullptr, cg, 0, P);
-        fin1(FIN_ALPHA_RR, nb64);
+        k_matvec<double, OP, true, true><<<nbo64, blk, 0, s>>>(d, prj, a.kap, a.pse, a.scale, wmx, nullptr, cg, 0, P);
+        fin1(FIN_ALPHA_RR, nbo64);
         k_cg64_update<OP><<<fb, FLAT_T, 0, s>>>(G, a.x, prj, res, wmx, a.w8, cg, P);
         fin2(FIN_CG64, fb);
         k_cg64_dir<<<fb, FLAT_T, 0, s>>>(G, prj, res, cg);
@@ -1297,7 +1306,7 @@ static int launch_mv(int op, const Dims& d, const T* p, const T* kap, const T* p
                      bool hom) {
   ELSIM_TRY(ensure_init());
   double* P = workspace().partials;
-  const int grid = stencil_ntiles<T>(d);
+  const int grid = std::is_same<T, double>::value ? ntiles64(d) : stencil_ntiles<T>(d);
   const dim3 blk(TX, TY, 1);
   if (grid > kMaxRedBlocks) {
     set_error("grid too large for the reduction workspace");

@@ -446,6 +446,269 @@ __device__ __forceinline__ void mv4_tile(const Dims& d, int tile, LD&& ld, LD1&&
   }
 }
 
+// ------------------------------------------ FP64 operator, 2 voxels a lane --
+//
+// The FP64 7-point sweep with two consecutive d1 voxels per lane (one 16-byte
+// load per row and array, 8-byte aligned) and the tile walk of mv4_tile:
+// row groups exchange their boundary rows through LDS, a lane marches SKC2
+// planes with its SJ2 rows of p and kappa for planes k and k+1 in registers,
+// and every face flux is formed once.  HOM: the homogeneous ghost rule of
+// sweep<HOM = true>; otherwise the stored ghosts, as sweep<HOM = false>
+// (lanes at the box's d1 edges load the ghost column).  Per voxel the same
+// IEEE operations in the same order as sweep(), so the same values.
+// epi(c, n, x, v) for every interior voxel (ghosted and interior index, p and
+// A p there), as sweep's.  A segment of TW = 64 or 128 d1 voxels takes TW/2
+// lanes.  Launch: mv2_ntiles(d) blocks of 256 threads, every lane calling it
+// the same number of times (a block barrier per plane).
+constexpr int SJ2 = 2, SKC2 = 8;
+__host__ __device__ inline int mv2_tw(const Dims& d) { return d.n1 <= 64 ? 64 : 128; }
+// usable when n1 is even and ghosted indices fit 32 bits
+__host__ __device__ inline bool mv2_ok(const Dims& d) { return d.n1 % 2 == 0 && d.G() < ((int64_t)1 << 32); }
+__host__ __device__ inline TileGrid mv2_tiles(const Dims& d) {
+  const int tw = mv2_tw(d), rows = 4 * (128 / tw) * SJ2;  // d2 rows per block
+  return TileGrid{(d.n1 + tw - 1) / tw, (d.n2 + rows - 1) / rows, (d.n3 + SKC2 - 1) / SKC2};
+}
+__host__ __device__ inline int mv2_ntiles(const Dims& d) {
+  const TileGrid t = mv2_tiles(d);
+  return t.gx * t.gy * t.gz;
+}
+__device__ __forceinline__ void ld2(const double* __restrict__ a, uint32_t q, double (&o)[2]) {
+  __builtin_memcpy(o, a + q, 16);
+}
+// The row-exchange buffer of mv2_tile: one array for all of a kernel's walks
+// (a kernel may walk with and without the homogeneous rule, or both edge
+// variants), 2 x 256 lanes x (p, kappa of the first and the last row).
+typedef double Xch64[256][8];
+__device__ __forceinline__ Xch64* xch64() {
+  __shared__ double buf[2][256][8];
+  return buf;
+}
+
+template <int OP, bool HOM, bool EDGE, class EPI>
+__device__ __forceinline__ void mv2_tile(const Dims& d, int tile, const double* __restrict__ p,
+                                         const double* __restrict__ kap, const double* __restrict__ pse,
+                                         double scale, EPI&& epi) {
+  constexpr int SJ = SJ2;
+  Xch64* xch = xch64();
+  const TileGrid tg = mv2_tiles(d);
+  const int nt = tg.gx * tg.gy * tg.gz;
+  const int tw = mv2_tw(d), lpr = tw / 2, rpw = 64 / lpr;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int grp = lane / lpr, x2 = lane % lpr;
+  const double half = 0.5;
+  const uint32_t sz = (uint32_t)d.g12();
+  const int per = nt / 8;
+  int b = tile;
+  if (b < per * 8) b = (b % 8) * per + b / 8;
+  const int tj = b % tg.gy, ti = (b / tg.gy) % tg.gx, tk = b / (tg.gx * tg.gy);
+  const int i0 = ti * tw + 2 * x2 + 1;
+  const bool active = i0 <= d.n1;
+  const int ia = active ? i0 : d.n1 - 1;  // idle lanes read an interior column
+  const bool left_b = i0 == 1, right_b = i0 + 1 == d.n1;
+  // the d1 neighbour across the lane's ends: from the next tile (EDGE) or,
+  // with stored ghosts, the ghost column; a lane shift otherwise
+  const bool ld_l = active && ((EDGE && x2 == 0 && i0 > 1) || (!HOM && left_b));
+  const bool ld_r = active && ((EDGE && x2 == lpr - 1 && i0 + 1 < d.n1) || (!HOM && right_b));
+  const int j0 = ((tj * 4 + wv) * rpw + grp) * SJ + 1;
+  const int k0 = tk * SKC2 + 1, k1 = min(k0 + SKC2 - 1, d.n3);
+  const bool down = tk & 1;  // odd tiles march downward (see sweep_ld)
+  const int kstart = down ? k1 : k0, kdir = down ? -1 : 1;
+  const uint32_t step = down ? 0u - sz : sz;
+  int jr[SJ];
+  uint32_t rc[SJ];
+#pragma unroll
+  for (int t = 0; t < SJ; t++) {
+    jr[t] = min(j0 + t, d.n2 + 1);
+    rc[t] = (uint32_t)d.gi(ia, jr[t], kstart);
+  }
+  const int jlo = min(j0 - 1, d.n2 + 1), jhi = min(j0 + SJ, d.n2 + 1);
+  const uint32_t clo = (uint32_t)d.gi(ia, jlo, kstart);
+  const uint32_t chi = (uint32_t)d.gi(ia, jhi, kstart);
+  double pc[SJ][2], kc[SJ][2];
+  double fb[SJ][2];  // flux through the face between plane k and the plane behind it
+  {
+    const bool bnd = down ? kstart == d.n3 : kstart == 1;
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      ld2(p, rc[t], pc[t]);
+      ld2(kap, rc[t], kc[t]);
+      double pbv[2], kbv[2];
+      ld2(p, rc[t] - step, pbv);
+      ld2(kap, rc[t] - step, kbv);
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        // upward: f5 (x - pm); downward: f6 (pz - x); pm / pz = x on the boundary plane (HOM)
+        const double f = (kbv[q] + kc[t][q]) * half;
+        const double nb = (HOM && bnd) ? pc[t][q] : pbv[q];
+        fb[t][q] = down ? f * (nb - pc[t][q]) : f * (pc[t][q] - nb);
+      }
+    }
+  }
+  uint32_t off = 0;
+  const int64_t nz = (int64_t)d.n1 * d.n2;
+  const int nk = k1 - k0 + 1;
+  for (int kk = 0; kk < nk; kk++, off += step) {
+    const int k = down ? k1 - kk : k0 + kk;
+    double pa[SJ][2], ka[SJ][2];
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      ld2(p, rc[t] + off + step, pa[t]);
+      ld2(kap, rc[t] + off + step, ka[t]);
+    }
+    // d2 halo rows: the neighbouring row groups' boundary rows through LDS,
+    // the block's outer ones from memory
+    const int bsel = kk & 1;
+    xch[bsel][tid][0] = pc[0][0];
+    xch[bsel][tid][1] = pc[0][1];
+    xch[bsel][tid][2] = kc[0][0];
+    xch[bsel][tid][3] = kc[0][1];
+    xch[bsel][tid][4] = pc[SJ - 1][0];
+    xch[bsel][tid][5] = pc[SJ - 1][1];
+    xch[bsel][tid][6] = kc[SJ - 1][0];
+    xch[bsel][tid][7] = kc[SJ - 1][1];
+    const bool glo = tid < lpr, ghi = tid >= 256 - lpr;
+    double hlp[2], hlk[2], hhp[2], hhk[2];
+    if (glo) {
+      ld2(p, clo + off, hlp);
+      ld2(kap, clo + off, hlk);
+    }
+    if (ghi) {
+      ld2(p, chi + off, hhp);
+      ld2(kap, chi + off, hhk);
+    }
+    double psv[SJ][2];
+    if (Rule<OP>::diag) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) ld2(pse, rc[t] + off, psv[t]);
+    }
+    double elp[SJ], elk[SJ], erp[SJ], erk[SJ];
+#pragma unroll
+    for (int t = 0; t < SJ; t++) elp[t] = elk[t] = erp[t] = erk[t] = 0.0;
+    if (ld_l) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        elp[t] = p[rc[t] + off - 1];
+        elk[t] = kap[rc[t] + off - 1];
+      }
+    }
+    if (ld_r) {
+#pragma unroll
+      for (int t = 0; t < SJ; t++) {
+        erp[t] = p[rc[t] + off + 2];
+        erk[t] = kap[rc[t] + off + 2];
+      }
+    }
+    __syncthreads();
+    if (!glo) {
+      hlp[0] = xch[bsel][tid - lpr][4];
+      hlp[1] = xch[bsel][tid - lpr][5];
+      hlk[0] = xch[bsel][tid - lpr][6];
+      hlk[1] = xch[bsel][tid - lpr][7];
+    }
+    if (!ghi) {
+      hhp[0] = xch[bsel][tid + lpr][0];
+      hhp[1] = xch[bsel][tid + lpr][1];
+      hhk[0] = xch[bsel][tid + lpr][2];
+      hhk[1] = xch[bsel][tid + lpr][3];
+    }
+    const bool kbnd = down ? k == 1 : k == d.n3;  // the face ahead is a boundary face
+    // d2 faces: jf[t] lies between rows j0 + t - 1 and j0 + t
+    double jf[SJ + 1][2];
+#pragma unroll
+    for (int t = 0; t <= SJ; t++) {
+      const int jl = j0 + t - 1;
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const double plq = t == 0 ? hlp[q] : pc[t - 1][q], klq = t == 0 ? hlk[q] : kc[t - 1][q];
+        const double puq = t == SJ ? hhp[q] : pc[t][q], kuq = t == SJ ? hhk[q] : kc[t][q];
+        const double f = (klq + kuq) * half;
+        double lo = plq, hi = puq;
+        if (HOM && jl == 0) lo = Rule<OP>::lo_zero ? 0.0 : hi;     // below row 1
+        if (HOM && jl == d.n2) hi = Rule<OP>::hi_zero ? 0.0 : lo;  // above row n2
+        jf[t][q] = f * (hi - lo);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < SJ; t++) {
+      const int j = j0 + t;
+      double pL = lane_shift<true>(pc[t][1]), kL = lane_shift<true>(kc[t][1]);
+      double pR = lane_shift<false>(pc[t][0]), kR = lane_shift<false>(kc[t][0]);
+      if (ld_l) {
+        pL = elp[t];
+        kL = elk[t];
+      }
+      if (ld_r) {
+        pR = erp[t];
+        kR = erk[t];
+      }
+      // d1 faces: xf[q] lies between voxels q - 1 and q
+      double xf[3];
+#pragma unroll
+      for (int q = 0; q <= 2; q++) {
+        const double kl = q == 0 ? kL : kc[t][q - 1], kr = q == 2 ? kR : kc[t][q];
+        double pl = q == 0 ? pL : pc[t][q - 1], pr = q == 2 ? pR : pc[t][q];
+        if (HOM && q == 0 && left_b) pl = pr;
+        if (HOM && q == 2 && right_b) pr = pl;
+        xf[q] = ((kl + kr) * half) * (pr - pl);
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const double xv = pc[t][q];
+        const double f = (ka[t][q] + kc[t][q]) * half;
+        const double nb = (HOM && kbnd) ? xv : pa[t][q];
+        const double fa = down ? f * (xv - nb) : f * (nb - xv);
+        const double fup = down ? fb[t][q] : fa, flo = down ? fa : fb[t][q];
+        double sv = xf[q + 1] - xf[q];
+        sv = sv + jf[t + 1][q];
+        sv = sv - jf[t][q];
+        sv = sv + fup;
+        sv = sv - flo;
+        double v;
+        if (Rule<OP>::diag)
+          v = psv[t][q] * xv - scale * sv;
+        else
+          v = scale * sv;
+        fb[t][q] = fa;
+        if (active && j <= d.n2)
+          epi((int64_t)(rc[t] + off) + q, (int64_t)(ia - 1 + q) + (int64_t)d.n1 * (j - 1) + nz * (k - 1), xv, v);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < SJ; t++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        pc[t][q] = pa[t][q];
+        kc[t][q] = ka[t][q];
+      }
+  }
+}
+
+// Tiles of the FP64 operator (k_matvec<double>): the two-voxels-a-lane walk
+// where the box allows it, the one-voxel sweep() otherwise.  ntiles64(d)
+// blocks of 256 threads; tile64 calls epi(c, n, x, v) per interior voxel
+// either way.  The FP64 kernels of the fast path's set-up keep sweep(): the
+// walk is 8-18 % faster there too, but it sums their reductions in another
+// order, and the coupled, controlled step amplifies such last-bit changes
+// into another trajectory (at 256^3, seed 1, 48 steps: 11.3 phP iterations a
+// step against 9.7; seeds 2 and 3 unchanged), which a benchmark window of
+// fixed length cannot average out.
+__host__ __device__ inline int ntiles64(const Dims& d) {
+  return mv2_ok(d) ? mv2_ntiles(d) : stencil_ntiles<double>(d);
+}
+template <int OP, bool HOM, class EPI>
+__device__ __forceinline__ void tile64(const Dims& d, int tile, const double* __restrict__ p,
+                                       const double* __restrict__ kap, const double* __restrict__ pse, double scale,
+                                       EPI&& epi) {
+  if (mv2_ok(d)) {
+    if (mv2_tiles(d).gx > 1)
+      mv2_tile<OP, HOM, true>(d, tile, p, kap, pse, scale, epi);
+    else
+      mv2_tile<OP, HOM, false>(d, tile, p, kap, pse, scale, epi);
+  } else {
+    sweep<double, OP, HOM>(d, stencil_idx<Rows<double>::SJ>(d, tile), p, kap, pse, scale, epi);
+  }
+}
+
 // dinv = 1 / diag(A_hom) (FP64), stored as T (uint16_t: BF16).  Mirror
 // faces do not contribute; zero-Dirichlet faces do.  Interior of a ghosted
 // array.
