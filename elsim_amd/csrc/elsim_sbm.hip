@@ -113,14 +113,17 @@ __global__ __launch_bounds__(256) void k_red_gw(Dims d, const double* __restrict
   block_partials<1>(acc, partials);
 }
 
-// sum over the interior of (x - x0)^2 w (all ghosted)
+// sum over the interior of (x - x0)^2 w (all ghosted); dx (optional): the
+// interior of x - x0 as well (the iteration's correction, for the next
+// coupling iteration's Galerkin step)
 __global__ __launch_bounds__(256) void k_red_cvg(Dims d, const double* __restrict__ x, const double* __restrict__ x0,
                                                  const double* __restrict__ w, int op, SbmScal* s, elsim_sbm_params P,
-                                                 double* partials) {
+                                                 double* partials, double* __restrict__ dx) {
   const Col ci = colx(d);
   double acc[1] = {0.0};
   each(d, ci, [&](int64_t c, int64_t, int) {
     const double t = x[c] - x0[c];
+    if (dx) dx[c] = t;
     acc[0] += t * t * w[c];
   });
   block_partials<1>(acc, partials);
@@ -925,15 +928,12 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
     pbc.hi = 1;
     pbc.hi_v = P.bvp;
     ELSIM_TRY(bc(s, s.g(s.php), pbc));
+    const bool keep_dx = P.ws_mode != ELSIM_WS_PLAIN;  // the corrections, for the next iteration's steps
+    if (keep_dx) ELSIM_TRY(ensure_ws_buffers(s));
     k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.psp), R_CVGP, s.S(), P,
-                                     ws.partials);
+                                     ws.partials, keep_dx ? s.g(s.outer_dp) : nullptr);
     ELSIM_CHECK_LAUNCH();
     ELSIM_TRY(fin_sbm(s, R_CVGP));
-    if (P.ws_mode != ELSIM_WS_PLAIN) {
-      ELSIM_TRY(ensure_ws_buffers(s));
-      k_diff_interior<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.outer_dp));
-      ELSIM_CHECK_LAUNCH();
-    }
     ELSIM_TRY(solve(s, ELSIM_OP_PHE, mode, &r));
     z.it_phe += r.iters;
     z.relres_phe = r.relres;
@@ -941,13 +941,9 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
     if (r.status) status = r.status;
     ELSIM_TRY(bc(s, s.g(s.phe), ebc));
     k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.phe), (const double*)s.cg.x0.p, s.g(s.pse), R_CVGL, s.S(), P,
-                                     ws.partials);
+                                     ws.partials, keep_dx ? s.g(s.outer_de) : nullptr);
     ELSIM_CHECK_LAUNCH();
     ELSIM_TRY(fin_sbm(s, R_CVGL));
-    if (P.ws_mode != ELSIM_WS_PLAIN) {
-      k_diff_interior<<<grid, blk, 0, stv>>>(d, s.g(s.phe), (const double*)s.cg.x0.p, s.g(s.outer_de));
-      ELSIM_CHECK_LAUNCH();
-    }
     ELSIM_TRY(sync_scal(s));
     cvgp = s.hs->cvgp;
     cvgl = s.hs->cvgl;
