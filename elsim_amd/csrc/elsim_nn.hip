@@ -273,24 +273,40 @@ __global__ __launch_bounds__(256) void k_conv3(int n1, int n2, int n3, const uin
       }
       continue;
     }
-    for (int tap = 0; tap < 27; tap++) {
+    // 16-channel slabs: one 16x16x16 MFMA per output tile and tap, the next
+    // tap's fragments loaded before this tap's MFMAs (as above)
+    auto lda16 = [&](int tap, s4v (&a)[MT]) {
       const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;  // halo offsets 0..2 (= -1..1 + 1)
-      for (int cc = 0; cc < cs; cc += 16) {
-        s4v a[MT];
 #pragma unroll
-        for (int m = 0; m < MT; m++) {
-          const int row = wave * MT + m, y = row % CONV_TY, z = row / CONV_TY;
-          const int h = ((z + dz) * HY + (y + dy)) * HX + (lr + dx);
-          a[m] = *(const s4v*)(lds + h * rs + cc + 4 * lq);
-        }
-        const int k = tap * cin + slab + cc + 4 * lq;
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const s4v bw = *(const s4v*)(wt + (int64_t)(t * 16 + lr) * K + k);
-#pragma unroll
-          for (int m = 0; m < MT; m++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bw, acc[m][t], 0, 0, 0);
-        }
+      for (int m = 0; m < MT; m++) {
+        const int row = wave * MT + m, y = row % CONV_TY, z = row / CONV_TY;
+        const int h = ((z + dz) * HY + (y + dy)) * HX + (lr + dx);
+        a[m] = *(const s4v*)(lds + h * rs + 4 * lq);
       }
+    };
+    auto ldb16 = [&](int tap, s4v (&bw)[NT]) {
+      const int k = tap * cin + slab + 4 * lq;
+#pragma unroll
+      for (int t = 0; t < NT; t++) bw[t] = *(const s4v*)(wt + (int64_t)(t * 16 + lr) * K + k);
+    };
+    s4v an[MT], bn[NT];
+    lda16(0, an);
+    ldb16(0, bn);
+    for (int tap = 0; tap < 27; tap++) {
+      s4v a[MT], bw[NT];
+#pragma unroll
+      for (int m = 0; m < MT; m++) a[m] = an[m];
+#pragma unroll
+      for (int t = 0; t < NT; t++) bw[t] = bn[t];
+      if (tap + 1 < 27) {
+        lda16(tap + 1, an);
+        ldb16(tap + 1, bn);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bw[t], acc[m][t], 0, 0, 0);
     }
   }
   conv_epilogue<COUT, MT, MODE, HN*(CS_MAX + LDS_PAD)>(acc, x0, y0, z0, n1, n2, n3, bias, relu, out, ov, lds, pout);
