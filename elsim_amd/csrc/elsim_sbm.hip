@@ -313,20 +313,6 @@ __global__ __launch_bounds__(256) void k_shift(Dims d, double* __restrict__ f, c
   each(d, ci, [&](int64_t q, int64_t, int) { f[q] = which == 0 ? f[q] - del : f[q] + del; });
 }
 
-// Kfw, Kbw (SBM:614-622)
-__global__ __launch_bounds__(256) void k_kfw(Dims d, const double* __restrict__ cnp, const double* __restrict__ tab,
-                                             int sz, elsim_sbm_params P, double* __restrict__ kfw,
-                                             double* __restrict__ kbw) {
-  const Col ci = colx(d);
-  each(d, ci, [&](int64_t q, int64_t n, int) {
-    const double cp = cnp[q];
-    const double ioc = table(cp, tab + 2 * sz, tab + 3 * sz, sz) * P.i0_scale;
-    const double ocv = table(cp, tab + 4 * sz, tab + 5 * sz, sz);
-    kfw[n] = ioc / (P.frd * P.kfw_cref) * exp(0.5 * P.cst1 * ocv);
-    kbw[n] = ioc / (P.frd * cp) * exp(-(0.5 * P.cst1 * ocv));
-  });
-}
-
 __global__ void k_table_n(int64_t n, const double* __restrict__ cord, const double* __restrict__ vals, int sz,
                           const double* __restrict__ c, double* __restrict__ f) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
@@ -334,19 +320,27 @@ __global__ void k_table_n(int64_t n, const double* __restrict__ cord, const doub
 }
 
 // Butler-Volmer (SBM:631-641) and both potential right-hand sides
-// (SBM:645, 727).
+// (SBM:645, 727).  The rate constants Kfw, Kbw (SBM:614-622) depend on
+// c_P alone, which the coupling loop does not change, so they are formed
+// here from c_P, where it is read anyway, instead of being stored once and
+// read back by every evaluation.
 __global__ __launch_bounds__(256) void k_bv(Dims d, const double* __restrict__ php, const double* __restrict__ phe,
                                             const double* __restrict__ cne, const double* __restrict__ cnp,
-                                            const double* __restrict__ kfw, const double* __restrict__ kbw,
-                                            const double* __restrict__ avp, const double* __restrict__ rhe,
-                                            elsim_sbm_params P, double* __restrict__ rxn, double* __restrict__ rxc,
+                                            const double* __restrict__ tab, int sz, const double* __restrict__ avp,
+                                            const double* __restrict__ rhe, elsim_sbm_params P,
+                                            double* __restrict__ rxn, double* __restrict__ rxc,
                                             double* __restrict__ rhsp, double* __restrict__ rhse) {
   const Col ci = colx(d);
   each(d, ci, [&](int64_t q, int64_t n, int) {
     const double a = avp[n];
     if (a > P.avp_min) {
+      const double cp = cnp[q];
+      const double ioc = table(cp, tab + 2 * sz, tab + 3 * sz, sz) * P.i0_scale;
+      const double ocv = table(cp, tab + 4 * sz, tab + 5 * sz, sz);
+      const double kfw = ioc / (P.frd * P.kfw_cref) * exp(0.5 * P.cst1 * ocv);
+      const double kbw = ioc / (P.frd * cp) * exp(-(0.5 * P.cst1 * ocv));
       const double dph = php[q] - phe[q];
-      const double rx = kfw[n] * cne[q] * exp(-(0.5 * P.cst1 * dph)) - kbw[n] * cnp[q] * exp(0.5 * P.cst1 * dph);
+      const double rx = kfw * cne[q] * exp(-(0.5 * P.cst1 * dph)) - kbw * cp * exp(0.5 * P.cst1 * dph);
       rxn[n] = rx;
       rxc[n] = rx * P.frd;
     }
@@ -472,7 +466,7 @@ struct SbmState {
   int sz = 0;
   int iter = 1;
   DevBuf tab, psp, pse, cnp, cne, php, phe, kapp, pce, pcl;  // ghosted (tab: 8 sz)
-  DevBuf avp, rxn, rxc, rhsp, rhse, rhsc, rhe, kfw, kbw, mu, pm;  // interior
+  DevBuf avp, rxn, rxc, rhsp, rhse, rhsc, rhe, mu, pm;  // interior
   DevBuf dab01, scal;
   CgWork cg;
   SbmScal* hs = nullptr;  // pinned mirror
@@ -493,7 +487,7 @@ struct SbmState {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // step timer, created once
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
-                    &rhse, &rhsc, &rhe, &kfw, &kbw, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0, &outer_dp,
+                    &rhse, &rhsc, &rhe, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0, &outer_dp,
                     &outer_de, &ws_dir2, &ws_dir3};
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
@@ -779,8 +773,6 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
   ELSIM_TRY(launch_matvec_f64(ELSIM_OP_PHP, d, s.g(s.cne), s.g(s.pce), nullptr, 1.0 / (P.dx * P.dx), s.g(s.rhe),
                               nullptr, false));
   ELSIM_TRY(coef(s, P.tc2 * P.d0, 1, s.g(s.pcl), nullptr));
-  k_kfw<<<grid, blk, 0, stv>>>(d, s.g(s.cnp), s.g(s.tab), s.sz, P, s.g(s.kfw), s.g(s.kbw));
-  ELSIM_CHECK_LAUNCH();
 
   // 4. warm start: both potentials extrapolated linearly in time before the
   //    first Butler-Volmer evaluation, so the coupling loop starts near the
@@ -812,7 +804,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
   int nlp = 0;
   auto butler_volmer = [&]() -> int {
     const Range range("butler_volmer");
-    k_bv<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.kfw), s.g(s.kbw),
+    k_bv<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.tab), s.sz,
                                 s.g(s.avp), s.g(s.rhe), P, s.g(s.rxn), s.g(s.rxc), s.g(s.rhsp), s.g(s.rhse));
     ELSIM_CHECK_LAUNCH();
     return ELSIM_OK;
@@ -1100,7 +1092,7 @@ int elsim_sbm_create(const int* n1, const int* n2, const int* n3, const elsim_sb
   for (DevBuf* b : {&s->psp, &s->pse, &s->cnp, &s->cne, &s->php, &s->phe, &s->kapp, &s->pce, &s->pcl, &s->php_prev,
                     &s->phe_prev})
     ELSIM_TRY(dev_alloc(*b, G));
-  for (DevBuf* b : {&s->avp, &s->rxn, &s->rxc, &s->rhsp, &s->rhse, &s->rhsc, &s->rhe, &s->kfw, &s->kbw, &s->mu, &s->pm})
+  for (DevBuf* b : {&s->avp, &s->rxn, &s->rxc, &s->rhsp, &s->rhse, &s->rhsc, &s->rhe, &s->mu, &s->pm})
     ELSIM_TRY(dev_alloc(*b, N));
   ELSIM_TRY(dev_alloc(s->dab01, 2 * (size_t)(d.n1 + 2) * (d.n3 + 2) * 8));
   ELSIM_TRY(dev_alloc(s->scal, sizeof(SbmScal)));
