@@ -154,44 +154,47 @@ __global__ __launch_bounds__(256) void k_psi(Dims d, const float* __restrict__ s
 // Ghost cells of a ghosted array.  Every ghost takes the value of its clamped
 // interior neighbour (the one-rank BcMpiYZ outer boundary plus the d2 mirror,
 // SBM:163-176), except the d2 planes whose rule is a value or the CnE flux.
-__global__ void k_bc(Dims d, double* __restrict__ f, BcArgs b) {
-  const int64_t A = 2 * d.g12();                           // d3 planes, all i, j
-  const int64_t B = 2 * (int64_t)(d.n1 + 2) * d.n3;        // d2 planes, all i, k in 1..n3
-  const int64_t C = 2 * (int64_t)d.n2 * d.n3;              // d1 lines, j, k interior
-  const int64_t tot = A + B + C;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    if (q < A) {
-      const int64_t r = q % d.g12();
-      k = q < d.g12() ? 0 : d.n3 + 1;
-      i = (int)(r % d.g1());
-      j = (int)(r / d.g1());
-    } else if (q < A + B) {
-      const int64_t r0 = q - A, half = B / 2, r = r0 % half;
-      j = r0 < half ? 0 : d.n2 + 1;
-      i = (int)(r % (d.n1 + 2));
-      k = (int)(r / (d.n1 + 2)) + 1;
-    } else {
-      const int64_t r0 = q - A - B, half = C / 2, r = r0 % half;
-      i = r0 < half ? 0 : d.n1 + 1;
-      j = (int)(r % d.n2) + 1;
-      k = (int)(r / d.n2) + 1;
-    }
-    const int ic = min(max(i, 1), d.n1), jc = min(max(j, 1), d.n2), kc = min(max(k, 1), d.n3);
-    double v = f[d.gi(ic, jc, kc)];
-    if (j == 0 && b.lo == 1) {
-      v = b.lo_vp ? *b.lo_vp : b.lo_v;
-    } else if (j == 0 && b.lo == 2) {
-      // SBM:558: infx*t_minus*dx/(0.5*(Dab(:,0,:)+Dab(:,1,:))) + CnE(:,1,:)
-      const int64_t pl = (int64_t)(d.n1 + 2) * (d.n3 + 2);
-      const int64_t o = i + (int64_t)(d.n1 + 2) * k;
-      v = *b.infx * b.tm * b.dx / (0.5 * (b.dab01[o] + b.dab01[pl + o])) + v;
-    } else if (j == d.n2 + 1 && b.hi == 1) {
-      v = b.hi_v;
-    }
-    if ((j == 0 && b.lo == 3) || (j == d.n2 + 1 && b.hi == 3)) v = f[d.gi(i, j, k)];
-    f[d.gi(i, j, k)] = v;
+// One thread per ghost cell (the face cells number below 2^31 for every box
+// the library accepts), 32-bit index arithmetic: the launch is latency-bound,
+// so no thread walks several cells.
+__global__ __launch_bounds__(256) void k_bc(Dims d, double* __restrict__ f, BcArgs b) {
+  const int g1 = d.n1 + 2, g12 = g1 * (d.n2 + 2);
+  const int A = 2 * g12;                 // d3 planes, all i, j
+  const int B = 2 * g1 * d.n3;           // d2 planes, all i, k in 1..n3
+  const int C = 2 * d.n2 * d.n3;         // d1 lines, j, k interior
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= A + B + C) return;
+  int i, j, k;
+  if (q < A) {
+    const int r = q < g12 ? q : q - g12;
+    k = q < g12 ? 0 : d.n3 + 1;
+    i = r % g1;
+    j = r / g1;
+  } else if (q < A + B) {
+    const int r0 = q - A, half = B / 2, r = r0 < half ? r0 : r0 - half;
+    j = r0 < half ? 0 : d.n2 + 1;
+    i = r % g1;
+    k = r / g1 + 1;
+  } else {
+    const int r0 = q - A - B, half = C / 2, r = r0 < half ? r0 : r0 - half;
+    i = r0 < half ? 0 : d.n1 + 1;
+    j = r % d.n2 + 1;
+    k = r / d.n2 + 1;
   }
+  const int ic = min(max(i, 1), d.n1), jc = min(max(j, 1), d.n2), kc = min(max(k, 1), d.n3);
+  double v = f[d.gi(ic, jc, kc)];
+  if (j == 0 && b.lo == 1) {
+    v = b.lo_vp ? *b.lo_vp : b.lo_v;
+  } else if (j == 0 && b.lo == 2) {
+    // SBM:558: infx*t_minus*dx/(0.5*(Dab(:,0,:)+Dab(:,1,:))) + CnE(:,1,:)
+    const int64_t pl = (int64_t)(d.n1 + 2) * (d.n3 + 2);
+    const int64_t o = i + (int64_t)(d.n1 + 2) * k;
+    v = *b.infx * b.tm * b.dx / (0.5 * (b.dab01[o] + b.dab01[pl + o])) + v;
+  } else if (j == d.n2 + 1 && b.hi == 1) {
+    v = b.hi_v;
+  }
+  if ((j == 0 && b.lo == 3) || (j == d.n2 + 1 && b.hi == 3)) v = f[d.gi(i, j, k)];
+  f[d.gi(i, j, k)] = v;
 }
 
 // AvP (SBM:274-289), interior
@@ -517,7 +520,11 @@ SbmState* lookup(const int64_t* h) {
 
 int launch_bc(const Dims& d, double* f, const BcArgs& b) {
   const int64_t tot = 2 * d.g12() + 2 * (int64_t)(d.n1 + 2) * d.n3 + 2 * (int64_t)d.n2 * d.n3;
-  k_bc<<<flat_blocks(tot), FLAT_T, 0, stream()>>>(d, f, b);
+  if (tot >= ((int64_t)1 << 31)) {
+    set_error("boundary faces of %d x %d x %d exceed 2^31 cells", d.n1, d.n2, d.n3);
+    return ELSIM_ERR_ARG;
+  }
+  k_bc<<<(unsigned)((tot + 255) / 256), 256, 0, stream()>>>(d, f, b);
   ELSIM_CHECK_LAUNCH();
   return ELSIM_OK;
 }
@@ -853,7 +860,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
       // with the search direction's homogeneous ghosts), so it never makes
       // the start worse in the norm CG minimises.
       double* r0 = s.g(s.ws_r0);
-      double rr = 0.0, r_ws = 0.0;
+      double rr = 0.0;
       ELSIM_TRY(launch_resid_store(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), r0, &rr));
       const double rms_r = std::sqrt(rr / (double)d.N());
       z.ws_res_plain = std::sqrt(rr);
@@ -880,8 +887,6 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
           k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), pd, alpha);
           ELSIM_CHECK_LAUNCH();
           ELSIM_TRY(bc(s, s.g(s.php), pbc0));
-          ELSIM_TRY(launch_resid_norm2(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), nullptr, sc, s.g(s.rhsp), &r_ws));
-          z.ws_res_nn = std::sqrt(r_ws);
           z.ws_used = 2;
           z.ws_alpha = alpha;
           z.ws_energy_gain = num * num / -den;  // decrease of the squared A-norm error
@@ -912,6 +917,9 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
     // CvgP / CvgL remain the coupling iteration's fixed-point residual.
     if (moved) ELSIM_TRY(butler_volmer());
     ELSIM_TRY(solve(s, ELSIM_OP_PHP, mode, &r));
+    // the residual the corrected start leaves, at the reaction re-evaluated
+    // there (the first phP solve's start)
+    if (nlp == 0 && z.ws_used == 2) z.ws_res_nn = r.res0;
     z.it_php += r.iters;
     z.relres_php = r.relres;
     z.n_reliable += r.n_reliable;

@@ -21,6 +21,7 @@ struct CgDev {
   double rmax2;    // mixed: ||r||^2 at the last reliable update
   double delta2;   // mixed: reliable-update trigger (relative, squared)
   double rr_true;  // last FP64 true residual norm^2
+  double rr0;      // FP64 residual norm^2 of the solve's start
   int iters;
   int maxit;
   int done;        // 0 running, 1 converged, 2 iteration cap, 3 not finite
@@ -79,6 +80,7 @@ struct SolveResult {
   int status = 0;
   int n_reliable = 0;
   double relres = 0.0;  // FP64 true relative residual of the correction system
+  double res0 = 0.0;    // ||b - A x|| at the solve's start (FP64)
 };
 
 // One CG solve of the correction system A_hom e = b - A x (x's ghosts as

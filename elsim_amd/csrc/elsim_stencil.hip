@@ -585,6 +585,7 @@ __global__ __launch_bounds__(FIN_T) void k_fin(int op, int nb, const double* __r
     case FIN_RESID:
       cg->rr = tot[0];
       cg->rmax2 = tot[0];
+      cg->rr0 = tot[0];
       break;
     case FIN_BEFF:
       cg->bn2 = cg->rel_r0 ? cg->rr : tot[0];
@@ -1149,6 +1150,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   out->iters = r.iters;
   out->n_reliable = r.n_reliable;
   out->relres = r.bn2 > 0 ? std::sqrt(r.rr_true / r.bn2) : 0.0;
+  out->res0 = std::sqrt(r.rr0);
   out->status = r.done == 1 ? ELSIM_OK : (r.done == 2 ? ELSIM_NOT_CONVERGED : ELSIM_ERR_NAN);
   if (out->status == ELSIM_ERR_NAN) set_error("CG produced a non-finite value (op %d)", OP);
   return ELSIM_OK;

@@ -146,7 +146,9 @@ typedef struct elsim_step_stats {
   double  seconds;     /* device time of the step (hipEvent) */
   int32_t ws_used;     /* 1: extrapolated start, 2: plus the learned correction */
   int32_t ws_pad_;
-  double  ws_res_plain, ws_res_nn; /* ||b - A x|| of the first phP solve before / after the correction */
+  double  ws_res_plain, ws_res_nn; /* ||b - A x|| of the first phP solve: at the extrapolated start (before
+                                      the correction), and at the corrected start with the reaction
+                                      re-evaluated there (the residual that solve starts from) */
   double  ws_alpha;     /* Galerkin step length along the prediction */
   double  ws_energy_gain; /* decrease of the squared A-norm error it gave */
 } elsim_step_stats;
