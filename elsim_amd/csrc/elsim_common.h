@@ -72,6 +72,34 @@ inline bool dims_ok(int n1, int n2, int n3) {
          (int64_t)n1 * n2 * n3 < ((int64_t)1 << 31);
 }
 
+// The ghost cells of a box, numbered 0 .. ghost_cells(d) - 1: the two d3
+// planes (all i, j), the two d2 planes (all i, k in 1..n3), the two d1 lines
+// (j, k interior).  Below 2^31 for every box the callers accept (checked on
+// the host), so 32-bit arithmetic.
+__host__ __device__ inline int64_t ghost_cells(const Dims& d) {
+  return 2 * d.g12() + 2 * (int64_t)(d.n1 + 2) * d.n3 + 2 * (int64_t)d.n2 * d.n3;
+}
+__device__ inline void ghost_cell(const Dims& d, int q, int& i, int& j, int& k) {
+  const int g1 = d.n1 + 2, g12 = g1 * (d.n2 + 2);
+  const int A = 2 * g12, B = 2 * g1 * d.n3;
+  if (q < A) {
+    const int r = q < g12 ? q : q - g12;
+    k = q < g12 ? 0 : d.n3 + 1;
+    i = r % g1;
+    j = r / g1;
+  } else if (q < A + B) {
+    const int r0 = q - A, half = B / 2, r = r0 < half ? r0 : r0 - half;
+    j = r0 < half ? 0 : d.n2 + 1;
+    i = r % g1;
+    k = r / g1 + 1;
+  } else {
+    const int r0 = q - A - B, half = d.n2 * d.n3, r = r0 < half ? r0 : r0 - half;
+    i = r0 < half ? 0 : d.n1 + 1;
+    j = r % d.n2 + 1;
+    k = r / d.n2 + 1;
+  }
+}
+
 // Column-marching launch: a block of TX x TY threads owns TX x TY columns of
 // (d1,d2) and marches KC planes of d3.  256 consecutive d1 voxels per block
 // row give fully coalesced 1 KiB (FP32) / 2 KiB (FP64) row loads.  The grid is

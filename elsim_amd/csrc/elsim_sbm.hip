@@ -154,33 +154,13 @@ __global__ __launch_bounds__(256) void k_psi(Dims d, const float* __restrict__ s
 // Ghost cells of a ghosted array.  Every ghost takes the value of its clamped
 // interior neighbour (the one-rank BcMpiYZ outer boundary plus the d2 mirror,
 // SBM:163-176), except the d2 planes whose rule is a value or the CnE flux.
-// One thread per ghost cell (the face cells number below 2^31 for every box
-// the library accepts), 32-bit index arithmetic: the launch is latency-bound,
-// so no thread walks several cells.
+// One thread per ghost cell (ghost_cell's numbering), 32-bit index
+// arithmetic: the launch is latency-bound, so no thread walks several cells.
 __global__ __launch_bounds__(256) void k_bc(Dims d, double* __restrict__ f, BcArgs b) {
-  const int g1 = d.n1 + 2, g12 = g1 * (d.n2 + 2);
-  const int A = 2 * g12;                 // d3 planes, all i, j
-  const int B = 2 * g1 * d.n3;           // d2 planes, all i, k in 1..n3
-  const int C = 2 * d.n2 * d.n3;         // d1 lines, j, k interior
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= A + B + C) return;
+  if (q >= ghost_cells(d)) return;
   int i, j, k;
-  if (q < A) {
-    const int r = q < g12 ? q : q - g12;
-    k = q < g12 ? 0 : d.n3 + 1;
-    i = r % g1;
-    j = r / g1;
-  } else if (q < A + B) {
-    const int r0 = q - A, half = B / 2, r = r0 < half ? r0 : r0 - half;
-    j = r0 < half ? 0 : d.n2 + 1;
-    i = r % g1;
-    k = r / g1 + 1;
-  } else {
-    const int r0 = q - A - B, half = C / 2, r = r0 < half ? r0 : r0 - half;
-    i = r0 < half ? 0 : d.n1 + 1;
-    j = r % d.n2 + 1;
-    k = r / d.n2 + 1;
-  }
+  ghost_cell(d, q, i, j, k);
   const int ic = min(max(i, 1), d.n1), jc = min(max(j, 1), d.n2), kc = min(max(k, 1), d.n3);
   double v = f[d.gi(ic, jc, kc)];
   if (j == 0 && b.lo == 1) {
@@ -307,6 +287,22 @@ __global__ void k_coef(Dims d, const double* __restrict__ cne, const double* __r
   }
 }
 
+// Both potential coefficients in one pass over c_E and psi_E (SBM:579-583,
+// 601-604): pce = psE (sd0 D), pcl = psE c (sd1 D), D = exp(...) of k_coef,
+// evaluated once (the same values as two k_coef passes).
+__global__ __launch_bounds__(256) void k_coef_pair(int64_t G, const double* __restrict__ cne,
+                                                   const double* __restrict__ pse, double sd0, double sd1,
+                                                   elsim_sbm_params P, double* __restrict__ pce,
+                                                   double* __restrict__ pcl) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= G) return;
+  const double c = cne[q], w = pse[q];
+  const double e = P.dab_c3 * c;
+  const double ex = exp(P.dab_c0 - P.dab_c1 * c + P.dab_c2 * (e * e));
+  pce[q] = w * (sd0 * ex);
+  pcl[q] = w * c * (sd1 * ex);
+}
+
 // interior += delta (device scalar expression)
 __global__ __launch_bounds__(256) void k_shift(Dims d, double* __restrict__ f, const SbmScal* s, int which,
                                                double target, double dt) {
@@ -326,23 +322,39 @@ __global__ void k_table_n(int64_t n, const double* __restrict__ cord, const doub
 // (SBM:645, 727).  The rate constants Kfw, Kbw (SBM:614-622) depend on
 // c_P alone, which the coupling loop does not change, so they are formed
 // here from c_P, where it is read anyway, instead of being stored once and
-// read back by every evaluation.
-__global__ __launch_bounds__(256) void k_bv(Dims d, const double* __restrict__ php, const double* __restrict__ phe,
+// read back by every evaluation.  WS: the step's first evaluation also
+// extrapolates both potentials first (phi_P += dir, phi_E += dir_e - shift,
+// interior), the values k_ws_axpy and k_ws_axpy_shift would store.
+template <bool WS>
+__global__ __launch_bounds__(256) void k_bv(Dims d, double* __restrict__ php, double* __restrict__ phe,
                                             const double* __restrict__ cne, const double* __restrict__ cnp,
                                             const double* __restrict__ tab, int sz, const double* __restrict__ avp,
                                             const double* __restrict__ rhe, elsim_sbm_params P,
                                             double* __restrict__ rxn, double* __restrict__ rxc,
-                                            double* __restrict__ rhsp, double* __restrict__ rhse) {
+                                            double* __restrict__ rhsp, double* __restrict__ rhse,
+                                            const double* __restrict__ dir, const double* __restrict__ dir_e,
+                                            double shift) {
   const Col ci = colx(d);
   each(d, ci, [&](int64_t q, int64_t n, int) {
+    double pp, pe;
+    if constexpr (WS) {
+      pp = php[q] + dir[q];
+      pe = phe[q] + (dir_e[q] - shift);
+      php[q] = pp;
+      phe[q] = pe;
+    }
     const double a = avp[n];
     if (a > P.avp_min) {
+      if constexpr (!WS) {
+        pp = php[q];
+        pe = phe[q];
+      }
       const double cp = cnp[q];
       const double ioc = table(cp, tab + 2 * sz, tab + 3 * sz, sz) * P.i0_scale;
       const double ocv = table(cp, tab + 4 * sz, tab + 5 * sz, sz);
       const double kfw = ioc / (P.frd * P.kfw_cref) * exp(0.5 * P.cst1 * ocv);
       const double kbw = ioc / (P.frd * cp) * exp(-(0.5 * P.cst1 * ocv));
-      const double dph = php[q] - phe[q];
+      const double dph = pp - pe;
       const double rx = kfw * cne[q] * exp(-(0.5 * P.cst1 * dph)) - kbw * cp * exp(0.5 * P.cst1 * dph);
       rxn[n] = rx;
       rxc[n] = rx * P.frd;
@@ -519,7 +531,7 @@ SbmState* lookup(const int64_t* h) {
 }
 
 int launch_bc(const Dims& d, double* f, const BcArgs& b) {
-  const int64_t tot = 2 * d.g12() + 2 * (int64_t)(d.n1 + 2) * d.n3 + 2 * (int64_t)d.n2 * d.n3;
+  const int64_t tot = ghost_cells(d);
   if (tot >= ((int64_t)1 << 31)) {
     set_error("boundary faces of %d x %d x %d exceed 2^31 cells", d.n1, d.n2, d.n3);
     return ELSIM_ERR_ARG;
@@ -776,10 +788,11 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
   }
 
   // 3. potential coefficients (SBM:579-622)
-  ELSIM_TRY(coef(s, P.tc1 * P.d0, 0, s.g(s.pce), nullptr));
+  k_coef_pair<<<(unsigned)((d.G() + 255) / 256), 256, 0, stv>>>(d.G(), s.g(s.cne), s.g(s.pse), P.tc1 * P.d0,
+                                                                  P.tc2 * P.d0, P, s.g(s.pce), s.g(s.pcl));
+  ELSIM_CHECK_LAUNCH();
   ELSIM_TRY(launch_matvec_f64(ELSIM_OP_PHP, d, s.g(s.cne), s.g(s.pce), nullptr, 1.0 / (P.dx * P.dx), s.g(s.rhe),
                               nullptr, false));
-  ELSIM_TRY(coef(s, P.tc2 * P.d0, 1, s.g(s.pcl), nullptr));
 
   // 4. warm start: both potentials extrapolated linearly in time before the
   //    first Butler-Volmer evaluation, so the coupling loop starts near the
@@ -788,17 +801,38 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
   BcArgs pbc0 = mirror_bc();
   pbc0.hi = 1;
   pbc0.hi_v = P.bvp;
+  auto butler_volmer = [&](bool extrapolate) -> int {
+    const Range range("butler_volmer");
+    if (extrapolate)
+      k_bv<true><<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.tab), s.sz,
+                                        s.g(s.avp), s.g(s.rhe), P, s.g(s.rxn), s.g(s.rxc), s.g(s.rhsp),
+                                        s.g(s.rhse), s.g(s.ws_dir), s.g(s.ws_dir_e), s.hs->shift);
+    else
+      k_bv<false><<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.tab), s.sz,
+                                         s.g(s.avp), s.g(s.rhe), P, s.g(s.rxn), s.g(s.rxc), s.g(s.rhsp),
+                                         s.g(s.rhse), nullptr, nullptr, 0.0);
+    ELSIM_CHECK_LAUNCH();
+    return ELSIM_OK;
+  };
+  // the first Butler-Volmer evaluation of the coupling loop, done here when
+  // it also applies the extrapolation
+  bool bv_done = false;
   if (ws_on) {
-    k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.ws_dir), 1.0);
-    ELSIM_CHECK_LAUNCH();
+    // phi_P += its last change; phi_E += only the solves' part of its last
+    // change.  The rest of that change is the current control's uniform
+    // shift (SBM:831), which the previous step applied at its end (its dv
+    // dt, kept in the scalars with that step's own dt) and which this step
+    // applies again with its own sign.  The reaction reads interior values
+    // only, so the ghosts follow.
+    if (P.max_outer > 0 && 1.0 > P.outer_tol) {  // the coupling loop runs at least once
+      ELSIM_TRY(butler_volmer(true));
+      bv_done = true;
+    } else {
+      k_ws_axpy<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.ws_dir), 1.0);
+      k_ws_axpy_shift<<<grid, blk, 0, stv>>>(d, s.g(s.phe), s.g(s.ws_dir_e), s.hs->shift);
+      ELSIM_CHECK_LAUNCH();
+    }
     ELSIM_TRY(bc(s, s.g(s.php), pbc0));
-    // phi_E: only the solves' part of its last change.  The rest of that
-    // change is the current control's uniform shift (SBM:831), which the
-    // previous step applied at its end (its dv dt, kept in the scalars with
-    // that step's own dt) and which this step applies again with its own
-    // sign.
-    k_ws_axpy_shift<<<grid, blk, 0, stv>>>(d, s.g(s.phe), s.g(s.ws_dir_e), s.hs->shift);
-    ELSIM_CHECK_LAUNCH();
     BcArgs ebc0 = mirror_bc();
     ebc0.lo = 1;
     ebc0.lo_vp = &s.S()->bve;
@@ -809,19 +843,13 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
   // 5. outer loop (SBM:624-812)
   double cvgp = 1.0, cvgl = 1.0;
   int nlp = 0;
-  auto butler_volmer = [&]() -> int {
-    const Range range("butler_volmer");
-    k_bv<<<grid, blk, 0, stv>>>(d, s.g(s.php), s.g(s.phe), s.g(s.cne), s.g(s.cnp), s.g(s.tab), s.sz,
-                                s.g(s.avp), s.g(s.rhe), P, s.g(s.rxn), s.g(s.rxc), s.g(s.rhsp), s.g(s.rhse));
-    ELSIM_CHECK_LAUNCH();
-    return ELSIM_OK;
-  };
   while (cvgp > P.outer_tol && cvgl > P.outer_tol) {
     if (nlp >= P.max_outer) {
       if (!status) status = ELSIM_NOT_CONVERGED;
       break;
     }
-    ELSIM_TRY(butler_volmer());
+    if (!bv_done) ELSIM_TRY(butler_volmer(false));
+    bv_done = false;
     bool moved = false;  // a start correction below moved phi_P or phi_E
     {
       // SBM:647: phP(:, nx+1, :) = BvP only
@@ -915,7 +943,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
     // after that evaluation, the reaction is evaluated again at the corrected
     // start, so that both solves start where the reaction was evaluated and
     // CvgP / CvgL remain the coupling iteration's fixed-point residual.
-    if (moved) ELSIM_TRY(butler_volmer());
+    if (moved) ELSIM_TRY(butler_volmer(false));
     ELSIM_TRY(solve(s, ELSIM_OP_PHP, mode, &r));
     // the residual the corrected start leaves, at the reaction re-evaluated
     // there (the first phP solve's start)

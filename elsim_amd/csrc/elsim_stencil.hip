@@ -340,10 +340,14 @@ __global__ __launch_bounds__(256) void k_pcg64_dir(int64_t G, double* __restrict
 
 // ------------------------------------------------ mixed-precision kernels --
 
-// FP32 copies of the coefficient (and psE) over the ghosted box.
-__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-  double v[FLAT_U];
-  FLAT_LOOP(n, { v[u] = a[q]; }, { b[q] = (float)v[u]; })
+// b = (float) a on the ghost cells of the box (the interior: k_dinv).
+__global__ __launch_bounds__(256) void k_ghost_f32(Dims d, const double* __restrict__ a, float* __restrict__ b) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ghost_cells(d)) return;
+  int i, j, k;
+  ghost_cell(d, q, i, j, k);
+  const int64_t c = d.gi(i, j, k);
+  b[c] = (float)a[c];
 }
 
 // The vector kernels of the mixed solver work on element pairs: a lane
@@ -1076,9 +1080,18 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   } else {
     uint16_t* dinv = (uint16_t*)w.dinv32.p;  // BF16
     float *r32 = (float*)w.r32.p, *w32 = (float*)w.w32.p, *y32 = (float*)w.y32.p, *p32 = (float*)w.p32.p;
-    k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.kap, (float*)w.kap32.p);
-    if (OP == ELSIM_OP_CNE) k_to_f32<<<fb, FLAT_T, 0, s>>>(G, a.pse, (float*)w.pse32.p);
-    k_dinv<OP, uint16_t><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv);
+    // FP32 copies of the coefficients: interior in the Jacobi set-up, which
+    // reads them anyway, ghosts by a face kernel
+    const int64_t ng = ghost_cells(d);
+    if (ng >= ((int64_t)1 << 31)) {
+      set_error("boundary faces of %d x %d x %d exceed 2^31 cells", d.n1, d.n2, d.n3);
+      return ELSIM_ERR_ARG;
+    }
+    const unsigned gb = (unsigned)((ng + 255) / 256);
+    k_ghost_f32<<<gb, 256, 0, s>>>(d, a.kap, (float*)w.kap32.p);
+    if (OP == ELSIM_OP_CNE) k_ghost_f32<<<gb, 256, 0, s>>>(d, a.pse, (float*)w.pse32.p);
+    k_dinv<OP, uint16_t><<<gc, blk, 0, s>>>(d, a.kap, a.pse, a.scale, dinv, (float*)w.kap32.p,
+                                            OP == ELSIM_OP_CNE ? (float*)w.pse32.p : nullptr);
     k_mx_start<<<fbu, FLAT_T, 0, s>>>(G2, (const double2*)r0, (const uint32_t*)dinv, (float2*)r32, (float2*)p32,
                                       (float2*)y32, (double2*)e64, P);
     fin2(FIN_MX_START, fbu);

@@ -711,14 +711,19 @@ __device__ __forceinline__ void tile64(const Dims& d, int tile, const double* __
 
 // dinv = 1 / diag(A_hom) (FP64), stored as T (uint16_t: BF16).  Mirror
 // faces do not contribute; zero-Dirichlet faces do.  Interior of a ghosted
-// array.
+// array.  kap32 / pse32 (optional): FP32 copies of the interior of kap / pse,
+// read here anyway (their ghosts: k_ghost_f32).
 template <int OP, typename T>
 __global__ __launch_bounds__(256) void k_dinv(Dims d, const double* __restrict__ kap, const double* __restrict__ pse,
-                                              double scale, T* __restrict__ dinv) {
+                                              double scale, T* __restrict__ dinv,
+                                              float* __restrict__ kap32 = nullptr,
+                                              float* __restrict__ pse32 = nullptr) {
   const ColIdx ci = col_idx(d);
   const int64_t sx = d.g1(), sz = d.g12();
   for_col(d, ci, [&](int64_t c, int64_t, int k) {
     const double kc = kap[c];
+    if (kap32) kap32[c] = (float)kc;
+    if (pse32) pse32[c] = (float)pse[c];
     double s = 0.0;
     if (ci.i > 1) s += (kap[c - 1] + kc) * 0.5;
     if (ci.i < d.n1) s += (kap[c + 1] + kc) * 0.5;
