@@ -771,7 +771,10 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
   ELSIM_CHECK_LAUNCH();
   ELSIM_TRY(fin_sbm(s, R_INFX));
   ELSIM_TRY(coef(s, P.d0, 0, s.g(s.pce), s.g(s.dab01)));
-  ELSIM_TRY(sync_scal(s));
+  // the host mirror of the scalars is current from the last step's end; the
+  // first step reads the totals of psi_P and psi_E (the reference stop
+  // rule's weights) here
+  if (s.iter <= 1) ELSIM_TRY(sync_scal(s));
   if (!(P.skip_cne_first && s.iter == 1)) {
     ELSIM_TRY(solve(s, ELSIM_OP_CNE, mode, &r));
     z.it_cne = r.iters;

@@ -120,9 +120,10 @@ __global__ __launch_bounds__(256) void k_resid64(Dims d, const double* __restric
 }
 
 // k_resid64 and ||b_eff||^2 in one kernel, b_eff = r0 + A_hom x (the
-// residual of a zero interior with the boundary data of x): per tile the
-// residual sweep, then the homogeneous sweep of the same x (which finds x and
-// the coefficient in cache) reading back the r0 the first one wrote.
+// residual of a zero interior with the boundary data of x): one sweep that
+// yields both A x (stored ghosts) and A_hom x, so each voxel's r and b_eff
+// are the values and summation order of a residual sweep followed by a
+// homogeneous sweep of the same tile.
 // Partials: slot 0 r.r, slot 1 ||b_eff||^2 (FIN_BEFF then sets tol2 and, for
 // the relative rule, done if the start already meets it).
 template <int OP>
@@ -136,16 +137,14 @@ __global__ __launch_bounds__(256) void k_resid64_beff(Dims d, const double* __re
   const int nt = stencil_ntiles<double>(d);
   for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
     const StIdx si = stencil_idx<Rows<double>::SJ>(d, tile);
-    sweep<double, OP, false>(d, si, x, kap, pse, scale, [&](int64_t c, int64_t n, double xc, double v) {
+    sweep_dual<double, OP>(d, si, x, kap, pse, scale, [&](int64_t c, int64_t n, double xc, double v, double vh) {
       const double r = rhs[n] - v;
       if (res) res[c] = r;
       r0[c] = r;
       if (prj) prj[c] = r;
       x0[c] = xc;
       acc[0] += r * r;
-    });
-    sweep<double, OP, true>(d, si, x, kap, pse, scale, [&](int64_t c, int64_t, double, double v) {
-      const double b = r0[c] + v;
+      const double b = r + vh;
       acc[1] += b * b;
     });
   }

@@ -40,9 +40,15 @@ struct Rule<ELSIM_OP_POT_DD> {
 // bytes at 256^3.  Per-voxel arithmetic does not depend on the direction.
 // The values of p come from ld(q) (q a ghosted index), so that a caller can
 // form p on the fly from other arrays (the fused CG direction update).
-template <typename T, int OP, bool HOM, class Ld, class Epi>
+// DUAL (with HOM false): epi(c, n, p_centre, value, value_hom) also gets
+// the homogeneous rule's value, computed from the same registers: equal to
+// value away from the faces, and at a face voxel the arithmetic of the HOM
+// sweep on the substituted neighbours -- the values of a second, HOM sweep
+// over the same tile, without its loads.
+template <typename T, int OP, bool HOM, class Ld, class Epi, bool DUAL = false>
 __device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld, const T* __restrict__ kap,
                                          const T* __restrict__ pse, T scale, Epi&& epi) {
+  static_assert(!(DUAL && HOM), "the dual sweep reads the stored ghosts");
   constexpr int SJ = Rows<T>::SJ;
   const int64_t sz = d.g12(), nz = (int64_t)d.n1 * d.n2;
   const int i = s.i;
@@ -133,7 +139,30 @@ __device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld,
         v = ps[t] * x - scale * sv;
       else
         v = scale * sv;
-      if (col_ok && j <= d.n2) epi(rc[t] + off, noff + (int64_t)t * d.n1, x, v);
+      if constexpr (DUAL) {
+        T vh = v;
+        if (k == 1 || k == d.n3 || i == 1 || i == d.n1 || j == 1 || j == d.n2) {
+          T hm = pm, hz = pz, hl = pl, hr = pr, hjm = pjm, hjp = pjp;
+          if (k == 1) hm = x;
+          if (k == d.n3) hz = x;
+          if (i == 1) hl = x;
+          if (i == d.n1) hr = x;
+          if (j == 1) hjm = Rule<OP>::lo_zero ? T(0) : x;
+          if (j == d.n2) hjp = Rule<OP>::hi_zero ? T(0) : x;
+          T sh = f2 * (hr - x) - f1 * (x - hl);
+          sh = sh + f4 * (hjp - x);
+          sh = sh - f3 * (x - hjm);
+          sh = sh + f6 * (hz - x);
+          sh = sh - f5 * (x - hm);
+          if (Rule<OP>::diag)
+            vh = ps[t] * x - scale * sh;
+          else
+            vh = scale * sh;
+        }
+        if (col_ok && j <= d.n2) epi(rc[t] + off, noff + (int64_t)t * d.n1, x, v, vh);
+      } else {
+        if (col_ok && j <= d.n2) epi(rc[t] + off, noff + (int64_t)t * d.n1, x, v);
+      }
     }
 #pragma unroll
     for (int t = 0; t < SJ; t++) {
@@ -150,6 +179,14 @@ __device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __
                                       const T* __restrict__ kap, const T* __restrict__ pse, T scale, Epi&& epi) {
   sweep_ld<T, OP, HOM>(
       d, s, [p](int64_t q) { return p[q]; }, kap, pse, scale, epi);
+}
+// sweep with the stored ghosts, epi(c, n, p_centre, value, value_hom).
+template <typename T, int OP, class Epi>
+__device__ __forceinline__ void sweep_dual(const Dims& d, const StIdx& s, const T* __restrict__ p,
+                                           const T* __restrict__ kap, const T* __restrict__ pse, T scale,
+                                           Epi&& epi) {
+  auto ld = [p](int64_t q) { return p[q]; };
+  sweep_ld<T, OP, false, decltype(ld)&, Epi&, true>(d, s, ld, kap, pse, scale, epi);
 }
 
 // Pointwise walk over a block's columns (interior only).
