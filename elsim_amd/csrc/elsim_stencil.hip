@@ -268,9 +268,9 @@ __global__ __launch_bounds__(256) void k_dot_resid(Dims d, const double* __restr
 
 // num = dv.(rhs - A x) (x's ghosts as stored) and den = dv.(A dv) (the
 // homogeneous rule): the two dot products of a Galerkin step along dv, in
-// one kernel -- per tile a sweep over x, then one over dv, which finds the
-// coefficient in cache -- instead of k_dot_resid and a k_matvec whose output
-// only fed its dot product.  Same per-tile sums as those two kernels.
+// one kernel -- one walk over x and dv together, sharing the coefficient's
+// loads -- instead of k_dot_resid and a k_matvec whose output only fed its
+// dot product.  Same per-tile sums as those two kernels.
 // Partials: slot 0 num, slot 1 den.
 template <int OP>
 __global__ __launch_bounds__(256) void k_galerkin_dots(Dims d, const double* __restrict__ x,
@@ -281,10 +281,11 @@ __global__ __launch_bounds__(256) void k_galerkin_dots(Dims d, const double* __r
   const int nt = stencil_ntiles<double>(d);
   for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
     const StIdx si = stencil_idx<Rows<double>::SJ>(d, tile);
-    sweep<double, OP, false>(d, si, x, kap, nullptr, scale,
-                             [&](int64_t c, int64_t n, double, double v) { acc[0] += dv[c] * (rhs[n] - v); });
-    sweep<double, OP, true>(d, si, dv, kap, nullptr, scale,
-                            [&](int64_t, int64_t, double pc, double v) { acc[1] += pc * v; });
+    sweep_pair<OP>(d, si, x, dv, kap, nullptr, scale,
+                   [&](int64_t, int64_t n, double, double v, double pc, double vh) {
+                     acc[0] += pc * (rhs[n] - v);
+                     acc[1] += pc * vh;
+                   });
   }
   block_partials<2>(acc, partials);
 }

@@ -45,7 +45,7 @@ struct Rule<ELSIM_OP_POT_DD> {
 // value away from the faces, and at a face voxel the arithmetic of the HOM
 // sweep on the substituted neighbours -- the values of a second, HOM sweep
 // over the same tile, without its loads.
-template <typename T, int OP, bool HOM, class Ld, class Epi, bool DUAL = false>
+template <typename T, int OP, bool HOM, class Ld, class Epi, bool DUAL = false, typename P = T>
 __device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld, const T* __restrict__ kap,
                                          const T* __restrict__ pse, T scale, Epi&& epi) {
   static_assert(!(DUAL && HOM), "the dual sweep reads the stored ghosts");
@@ -64,7 +64,8 @@ __device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld,
   const bool ld_left = s.lane == 0 && (i > 1 || !HOM);
   const bool ld_right = s.lane == 63 && (i < d.n1 || (!HOM && i == d.n1));
   const T half = T(0.5);
-  T pc[SJ], kc[SJ], pb[SJ], kb[SJ];  // plane k, the plane behind (k - direction)
+  P pc[SJ], pb[SJ];  // plane k, the plane behind (k - direction)
+  T kc[SJ], kb[SJ];
 #pragma unroll
   for (int t = 0; t < SJ; t++) {
     pc[t] = ld(rc[t]);
@@ -76,15 +77,18 @@ __device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld,
   const int nk = s.k1 - s.k0 + 1;
   for (int kk = 0; kk < nk; kk++, off += step, noff += nstep) {
     const int k = down ? s.k1 - kk : s.k0 + kk;
-    T pa[SJ], ka[SJ], ps[SJ];  // plane ahead (k + direction)
+    P pa[SJ];  // plane ahead (k + direction)
+    T ka[SJ], ps[SJ];
 #pragma unroll
     for (int t = 0; t < SJ; t++) {
       pa[t] = ld(rc[t] + off + step);
       ka[t] = kap[rc[t] + off + step];
       if (Rule<OP>::diag) ps[t] = pse[rc[t] + off];
     }
-    const T pjlo = ld(clo + off), kjlo = kap[clo + off], pjhi = ld(chi + off), kjhi = kap[chi + off];
-    T pim[SJ], pip[SJ], kim[SJ], kip[SJ];
+    const P pjlo = ld(clo + off), pjhi = ld(chi + off);
+    const T kjlo = kap[clo + off], kjhi = kap[chi + off];
+    P pim[SJ], pip[SJ];
+    T kim[SJ], kip[SJ];
 #pragma unroll
     for (int t = 0; t < SJ; t++) {
       pim[t] = lane_shift<true>(pc[t]);
@@ -109,12 +113,12 @@ __device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld,
 #pragma unroll
     for (int t = 0; t < SJ; t++) {
       const int j = s.j0 + t;
-      const T x = pc[t];
-      T pz = down ? pb[t] : pa[t], pm = down ? pa[t] : pb[t];
+      const P x = pc[t];
+      P pz = down ? pb[t] : pa[t], pm = down ? pa[t] : pb[t];
       const T kz = down ? kb[t] : ka[t], km = down ? ka[t] : kb[t];
-      T pl = pim[t], pr = pip[t];
-      T pjm = t == 0 ? pjlo : pc[t - 1];
-      T pjp = t == SJ - 1 ? pjhi : pc[t + 1];
+      P pl = pim[t], pr = pip[t];
+      P pjm = t == 0 ? pjlo : pc[t - 1];
+      P pjp = t == SJ - 1 ? pjhi : pc[t + 1];
       const T kjm = t == 0 ? kjlo : kc[t - 1];
       const T kjp = t == SJ - 1 ? kjhi : kc[t + 1];
       if (HOM) {
@@ -122,34 +126,34 @@ __device__ __forceinline__ void sweep_ld(const Dims& d, const StIdx& s, Ld&& ld,
         if (k == d.n3) pz = x;
         if (i == 1) pl = x;
         if (i == d.n1) pr = x;
-        if (j == 1) pjm = Rule<OP>::lo_zero ? T(0) : x;
-        if (j == d.n2) pjp = Rule<OP>::hi_zero ? T(0) : x;
+        if (j == 1) pjm = Rule<OP>::lo_zero ? P(T(0)) : x;
+        if (j == d.n2) pjp = Rule<OP>::hi_zero ? P(T(0)) : x;
       }
       const T k0v = kc[t];
       const T f1 = (kim[t] + k0v) * half, f2 = (kip[t] + k0v) * half;
       const T f3 = (kjm + k0v) * half, f4 = (kjp + k0v) * half;
       const T f5 = (km + k0v) * half, f6 = (kz + k0v) * half;
-      T sv = f2 * (pr - x) - f1 * (x - pl);
+      P sv = f2 * (pr - x) - f1 * (x - pl);
       sv = sv + f4 * (pjp - x);
       sv = sv - f3 * (x - pjm);
       sv = sv + f6 * (pz - x);
       sv = sv - f5 * (x - pm);
-      T v;
+      P v;
       if (Rule<OP>::diag)
         v = ps[t] * x - scale * sv;
       else
         v = scale * sv;
       if constexpr (DUAL) {
-        T vh = v;
+        P vh = v;
         if (k == 1 || k == d.n3 || i == 1 || i == d.n1 || j == 1 || j == d.n2) {
-          T hm = pm, hz = pz, hl = pl, hr = pr, hjm = pjm, hjp = pjp;
+          P hm = pm, hz = pz, hl = pl, hr = pr, hjm = pjm, hjp = pjp;
           if (k == 1) hm = x;
           if (k == d.n3) hz = x;
           if (i == 1) hl = x;
           if (i == d.n1) hr = x;
-          if (j == 1) hjm = Rule<OP>::lo_zero ? T(0) : x;
-          if (j == d.n2) hjp = Rule<OP>::hi_zero ? T(0) : x;
-          T sh = f2 * (hr - x) - f1 * (x - hl);
+          if (j == 1) hjm = Rule<OP>::lo_zero ? P(T(0)) : x;
+          if (j == d.n2) hjp = Rule<OP>::hi_zero ? P(T(0)) : x;
+          P sh = f2 * (hr - x) - f1 * (x - hl);
           sh = sh + f4 * (hjp - x);
           sh = sh - f3 * (x - hjm);
           sh = sh + f6 * (hz - x);
@@ -180,6 +184,23 @@ __device__ __forceinline__ void sweep(const Dims& d, const StIdx& s, const T* __
   sweep_ld<T, OP, HOM>(
       d, s, [p](int64_t q) { return p[q]; }, kap, pse, scale, epi);
 }
+// Two arrays walked together over one coefficient: the per-voxel arithmetic
+// of sweep_ld applied to each component (the same operations in the same
+// order as a walk over that array alone).
+struct D2 {
+  double a, b;
+  D2() = default;
+  __device__ explicit D2(double v) : a(v), b(v) {}
+  __device__ D2(double x, double y) : a(x), b(y) {}
+};
+__device__ __forceinline__ D2 operator+(D2 p, D2 q) { return D2(p.a + q.a, p.b + q.b); }
+__device__ __forceinline__ D2 operator-(D2 p, D2 q) { return D2(p.a - q.a, p.b - q.b); }
+__device__ __forceinline__ D2 operator*(double f, D2 p) { return D2(f * p.a, f * p.b); }
+template <bool FROM_LEFT>
+__device__ __forceinline__ D2 lane_shift(D2 v) {
+  return D2(lane_shift<FROM_LEFT>(v.a), lane_shift<FROM_LEFT>(v.b));
+}
+
 // sweep with the stored ghosts, epi(c, n, p_centre, value, value_hom).
 template <typename T, int OP, class Epi>
 __device__ __forceinline__ void sweep_dual(const Dims& d, const StIdx& s, const T* __restrict__ p,
@@ -187,6 +208,18 @@ __device__ __forceinline__ void sweep_dual(const Dims& d, const StIdx& s, const 
                                            Epi&& epi) {
   auto ld = [p](int64_t q) { return p[q]; };
   sweep_ld<T, OP, false, decltype(ld)&, Epi&, true>(d, s, ld, kap, pse, scale, epi);
+}
+
+// x with the stored ghosts and y with the homogeneous rule, in one walk:
+// epi(c, n, x_c, (A x)_c, y_c, (A_hom y)_c) -- the values of a sweep over x
+// and a HOM sweep over y.
+template <int OP, class Epi>
+__device__ __forceinline__ void sweep_pair(const Dims& d, const StIdx& s, const double* __restrict__ x,
+                                           const double* __restrict__ y, const double* __restrict__ kap,
+                                           const double* __restrict__ pse, double scale, Epi&& epi) {
+  auto ld = [x, y](int64_t q) { return D2(x[q], y[q]); };
+  auto e2 = [&](int64_t c, int64_t n, D2 pc, D2 v, D2 vh) { epi(c, n, pc.a, v.a, pc.b, vh.b); };
+  sweep_ld<double, OP, false, decltype(ld)&, decltype(e2)&, true, D2>(d, s, ld, kap, pse, scale, e2);
 }
 
 // Pointwise walk over a block's columns (interior only).
