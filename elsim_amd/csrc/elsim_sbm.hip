@@ -206,17 +206,23 @@ __global__ void k_scale_g(int64_t n, const double* __restrict__ a, double s, dou
 __global__ __launch_bounds__(256) void k_ch_mu(Dims d, const double* __restrict__ c, const double* __restrict__ psp,
                                                const double* __restrict__ tab, int sz, double eps, double dx,
                                                double* __restrict__ mu, double* __restrict__ pm) {
+  // the column's d3 neighbours roll through registers (planes k - 1, k, k + 1)
   const Col ci = colx(d);
-  const int64_t sx = d.g1(), sz3 = d.g12();
-  each(d, ci, [&](int64_t q, int64_t n, int) {
-    const double cc = c[q];
+  if (!ci.active) return;
+  const int64_t sx = d.g1(), sz3 = d.g12(), nz = (int64_t)d.n1 * d.n2;
+  int64_t q = d.gi(ci.i, ci.j, ci.k0), n = d.ni(ci.i, ci.j, ci.k0);
+  double cb = c[q - sz3], cc = c[q];
+  for (int k = ci.k0; k <= ci.k1; k++, q += sz3, n += nz) {
+    const double ca = c[q + sz3];
     double v = c[q - sx] - 2 * cc + c[q + sx];
     v = v + c[q - 1] - 2 * cc + c[q + 1];
-    v = v + c[q - sz3] - 2 * cc + c[q + sz3];
+    v = v + cb - 2 * cc + ca;
     const double lap = v / (dx * dx);
     mu[n] = table(cc, tab, tab + sz, sz) - eps * lap;
     pm[n] = psp[q] * table(cc, tab + 6 * sz, tab + 7 * sz, sz);
-  });
+    cb = cc;
+    cc = ca;
+  }
 }
 
 // CH part 2 (SBM:422-443): DvC with mirrored mu/pm, update where psP > min,
@@ -225,29 +231,40 @@ __global__ __launch_bounds__(256) void k_ch_update(Dims d, double* __restrict__ 
                                                    const double* __restrict__ pm, const double* __restrict__ psp,
                                                    const double* __restrict__ rxn, const double* __restrict__ avp,
                                                    elsim_sbm_params P, SbmScal* s, double* partials) {
+  // the column's d3 neighbours of mu and pm roll through registers (the
+  // mirrored value at the box's end planes, as the offsets zp, zm = 0 give)
   const Col ci = colx(d);
   const int64_t nx = d.n1, nz = (int64_t)d.n1 * d.n2;
   double acc[1] = {0.0};
-  each(d, ci, [&](int64_t q, int64_t n, int k) {
+  if (ci.active) {
     const int i = ci.i, j = ci.j;
     const int64_t yp = i < d.n1 ? 1 : 0, ym = i > 1 ? 1 : 0;
     const int64_t xp = j < d.n2 ? nx : 0, xm = j > 1 ? nx : 0;
-    const int64_t zp = k < d.n3 ? nz : 0, zm = k > 1 ? nz : 0;
-    const double m0 = mu[n], p0 = pm[n];
-    double v = 0.5 * (pm[n + yp] + p0) * (mu[n + yp] - m0) - 0.5 * (p0 + pm[n - ym]) * (m0 - mu[n - ym]);
-    v = v + 1.0 * 0.5 * (pm[n + xp] + p0) * (mu[n + xp] - m0);
-    v = v - 1.0 * 0.5 * (p0 + pm[n - xm]) * (m0 - mu[n - xm]);
-    v = v + 0.5 * (pm[n + zp] + p0) * (mu[n + zp] - m0);
-    v = v - 0.5 * (p0 + pm[n - zm]) * (m0 - mu[n - zm]);
-    const double dvc = v / (P.dx * P.dx);
-    double cn = c[q];
-    const double ps = psp[q];
-    if (ps > P.ch_psi_min) {
-      cn = cn + P.dt * (dvc + (rxn[n] / P.rho) * (avp[n] / P.dx)) / ps;
-      c[q] = cn;
+    const int64_t sz3 = d.g12();
+    int64_t q = d.gi(i, j, ci.k0), n = d.ni(i, j, ci.k0);
+    double m0 = mu[n], p0 = pm[n];
+    double mb = ci.k0 > 1 ? mu[n - nz] : m0, pb = ci.k0 > 1 ? pm[n - nz] : p0;
+    for (int k = ci.k0; k <= ci.k1; k++, q += sz3, n += nz) {
+      const double ma = k < d.n3 ? mu[n + nz] : m0, pa = k < d.n3 ? pm[n + nz] : p0;
+      double v = 0.5 * (pm[n + yp] + p0) * (mu[n + yp] - m0) - 0.5 * (p0 + pm[n - ym]) * (m0 - mu[n - ym]);
+      v = v + 1.0 * 0.5 * (pm[n + xp] + p0) * (mu[n + xp] - m0);
+      v = v - 1.0 * 0.5 * (p0 + pm[n - xm]) * (m0 - mu[n - xm]);
+      v = v + 0.5 * (pa + p0) * (ma - m0);
+      v = v - 0.5 * (p0 + pb) * (m0 - mb);
+      const double dvc = v / (P.dx * P.dx);
+      double cn = c[q];
+      const double ps = psp[q];
+      if (ps > P.ch_psi_min) {
+        cn = cn + P.dt * (dvc + (rxn[n] / P.rho) * (avp[n] / P.dx)) / ps;
+        c[q] = cn;
+      }
+      acc[0] += cn * ps;
+      mb = m0;
+      pb = p0;
+      m0 = ma;
+      p0 = pa;
     }
-    acc[0] += cn * ps;
-  });
+  }
   block_partials<1>(acc, partials);
 }
 

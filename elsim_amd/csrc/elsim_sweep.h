@@ -789,9 +789,13 @@ __global__ __launch_bounds__(256) void k_dinv(Dims d, const double* __restrict__
                                               float* __restrict__ kap32 = nullptr,
                                               float* __restrict__ pse32 = nullptr) {
   const ColIdx ci = col_idx(d);
+  if (!ci.active) return;
   const int64_t sx = d.g1(), sz = d.g12();
-  for_col(d, ci, [&](int64_t c, int64_t, int k) {
-    const double kc = kap[c];
+  // the column's d3 neighbours roll through registers
+  int64_t c = d.gi(ci.i, ci.j, ci.k0);
+  double kb = kap[c - sz], kc = kap[c];
+  for (int k = ci.k0; k <= ci.k1; k++, c += sz) {
+    const double ka = kap[c + sz];
     if (kap32) kap32[c] = (float)kc;
     if (pse32) pse32[c] = (float)pse[c];
     double s = 0.0;
@@ -799,14 +803,16 @@ __global__ __launch_bounds__(256) void k_dinv(Dims d, const double* __restrict__
     if (ci.i < d.n1) s += (kap[c + 1] + kc) * 0.5;
     if (ci.j > 1 || Rule<OP>::lo_zero) s += (kap[c - sx] + kc) * 0.5;
     if (ci.j < d.n2 || Rule<OP>::hi_zero) s += (kap[c + sx] + kc) * 0.5;
-    if (k > 1) s += (kap[c - sz] + kc) * 0.5;
-    if (k < d.n3) s += (kap[c + sz] + kc) * 0.5;
+    if (k > 1) s += (kb + kc) * 0.5;
+    if (k < d.n3) s += (ka + kc) * 0.5;
     const double dg = Rule<OP>::diag ? pse[c] + scale * s : -scale * s;
     if constexpr (std::is_same<T, uint16_t>::value)
       dinv[c] = f32_to_bf16((float)(1.0 / dg));
     else
       dinv[c] = (T)(1.0 / dg);
-  });
+    kb = kc;
+    kc = ka;
+  }
 }
 
 }  // namespace elsim
