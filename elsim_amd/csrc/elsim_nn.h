@@ -26,6 +26,9 @@ int64_t nn_param_count();
 // 16 or 8; with 8, the network's input channels 8..15 are zero); grid dims
 // multiples of 8.
 int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred, int in_used);
+// The same prediction written as FP64 into the interior of a ghosted
+// (n1 + 2) x (n2 + 2) x (n3 + 2) box (ghosts untouched).
+int nn_forward_g(NnNet* net, int n1, int n2, int n3, const uint16_t* x, double* pred_g, int in_used);
 // The network of a handle issued by elsim_nn_create (nullptr if unknown or
 // destroyed).
 NnNet* nn_lookup(int64_t handle);

@@ -570,7 +570,7 @@ __global__ __launch_bounds__(512) void k_dec0_out(int n1, int n2, int n3, const 
                                                   const uint16_t* __restrict__ lo, View lv,
                                                   const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wf,
                                                   const float* __restrict__ bias, const float* __restrict__ ow,
-                                                  float* __restrict__ pred) {
+                                                  float* __restrict__ pred, double* __restrict__ pred_g) {
   __shared__ __align__(16) uint16_t lds_c[D0_CHN * D0_CRS];
   __shared__ __align__(16) uint16_t lds_s[D0_SHN * D0_SRS];
   __shared__ __align__(16) uint16_t lds_o[D0_FN * D0_ORS];
@@ -724,7 +724,10 @@ __global__ __launch_bounds__(512) void k_dec0_out(int n1, int n2, int n3, const 
 #pragma unroll
         for (int c = 0; c < 16; c++) sum += ow[c] * bf2f(lds_o[tid * D0_ORS + c]);
         sum += ow[16];
-        pred[gx + (int64_t)n1 * (gy + (int64_t)n2 * gz)] = sum;
+        if (pred_g)  // FP64 into the interior of a ghosted box
+          pred_g[(gx + 1) + (int64_t)(n1 + 2) * ((gy + 1) + (int64_t)(n2 + 2) * (gz + 1))] = (double)sum;
+        else
+          pred[gx + (int64_t)n1 * (gy + (int64_t)n2 * gz)] = sum;
       }
     }
   }
@@ -1159,7 +1162,7 @@ struct NnNet {
   // channels [in_used, 16) being zero (8: the first layer runs on 8-channel
   // slabs).
   int forward(const uint16_t* x0, float* pred, const float* target, double* loss_host, bool infer = false,
-              int in_used = NN_IN_CH) {
+              int in_used = NN_IN_CH, double* pred_g = nullptr) {
     ELSIM_TRY(pack());
     hipStream_t s = stream();
     const int a1 = n1, a2 = n2, a3 = n3;
@@ -1201,9 +1204,13 @@ struct NnNet {
       const int nt = ((m1 + D0_TX - 1) / D0_TX) * ((m2 + D0_TY - 1) / D0_TY) * ((m3 + D0_TZ - 1) / D0_TZ);
       k_dec0_out<<<std::min(nt, cu_count()), 512, 0, s>>>(a1, a2, a3, C0, View{16, 0}, (uint16_t*)d1.p, View{32, 0},
                                                           WT(L_D0), (uint16_t*)wf.p, B + b_off[L_D0], B + out_off,
-                                                          pred);
+                                                          pred, pred_g);
       ELSIM_CHECK_LAUNCH();
       return ELSIM_OK;
+    }
+    if (pred_g) {
+      set_error("network forward: the FP64 ghosted output is written by the inference path only");
+      return ELSIM_ERR_ARG;
     }
     if (!infer) k_up<<<nblk_cap(N0 * 4), 256, 0, s>>>(a1, a2, a3, 32, (uint16_t*)d1.p, C0, View{48, 0});
     ELSIM_TRY(conv<0>(a1, a2, a3, C0, 48, i0, WT(L_D0), B + b_off[L_D0], 1, 16, d0.p, View{16, 0}, s,
@@ -1338,6 +1345,11 @@ struct NnNet {
 int nn_forward(NnNet* net, int n1, int n2, int n3, const uint16_t* x, float* pred, int in_used) {
   ELSIM_TRY(net->ensure(n1, n2, n3, false));
   return net->forward(x, pred, nullptr, nullptr, true, in_used);
+}
+
+int nn_forward_g(NnNet* net, int n1, int n2, int n3, const uint16_t* x, double* pred_g, int in_used) {
+  ELSIM_TRY(net->ensure(n1, n2, n3, false));
+  return net->forward(x, nullptr, nullptr, nullptr, true, in_used, pred_g);
 }
 
 }  // namespace elsim

@@ -432,13 +432,6 @@ __global__ __launch_bounds__(256) void k_interior(Dims d, const double* __restri
   each(d, ci, [&](int64_t c, int64_t n, int) { out[n] = a[c]; });
 }
 
-// out(interior) = base + scale * pred (ghosts untouched).
-__global__ __launch_bounds__(256) void k_ws_apply(Dims d, const double* base, const float* __restrict__ pred,
-                                                  double scale, double* out) {
-  const Col ci = colx(d);
-  each(d, ci, [&](int64_t c, int64_t n, int) { out[c] = base[c] + scale * (double)pred[n]; });
-}
-
 // out = a - prev, then prev = a, over the whole ghosted box (the last
 // step's change of a field and the copy kept for the next one, in one pass).
 __global__ __launch_bounds__(256) void k_ws_diff_keep(int64_t G, const double* __restrict__ a,
@@ -504,7 +497,7 @@ struct SbmState {
   SbmScal* hs = nullptr;  // pinned mirror
   // learned warm start: phi_P of the previous step, the network, its input,
   // output and the candidate field
-  DevBuf php_prev, phe_prev, nn_x, nn_pred, ws_cand, ws_dir, ws_dir_e, ws_r0, outer_dp, outer_de;
+  DevBuf php_prev, phe_prev, nn_x, ws_cand, ws_dir, ws_dir_e, ws_r0, outer_dp, outer_de;
   // phi_P's changes over the two steps before the last one (ws_p_hist of
   // them valid)
   DevBuf ws_dir2, ws_dir3;
@@ -519,7 +512,7 @@ struct SbmState {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // step timer, created once
   ~SbmState() {
     DevBuf* bs[] = {&tab, &psp, &pse, &cnp, &cne, &php, &phe, &kapp, &pce, &pcl, &avp, &rxn, &rxc, &rhsp,
-                    &rhse, &rhsc, &rhe, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &nn_pred, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0, &outer_dp,
+                    &rhse, &rhsc, &rhe, &mu, &pm, &dab01, &scal, &php_prev, &phe_prev, &nn_x, &ws_cand, &ws_dir, &ws_dir_e, &ws_r0, &outer_dp,
                     &outer_de, &ws_dir2, &ws_dir3};
     for (DevBuf* b : bs) dev_free(*b);
     cg.release();
@@ -732,7 +725,6 @@ int galerkin_step(SbmState& s, int op, double* x, const double* kap, const doubl
 int ensure_ws_buffers(SbmState& s) {
   const size_t N = (size_t)s.d.N(), G = (size_t)s.d.G();
   ELSIM_TRY(dev_alloc(s.nn_x, N * NN_IN_CH * sizeof(uint16_t)));
-  ELSIM_TRY(dev_alloc(s.nn_pred, N * sizeof(float)));
   ELSIM_TRY(dev_alloc(s.ws_cand, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir, G * sizeof(double)));
   ELSIM_TRY(dev_alloc(s.ws_dir2, G * sizeof(double)));
@@ -922,12 +914,13 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
       if (s.ws_net && rms_r > 0.0) {
         {
           const Range range("ws_network");
-          ELSIM_TRY(nn_forward(s.ws_net, d.n1, d.n2, d.n3, xin, (float*)s.nn_pred.p, s.rec_x ? 16 : 8));
+          // the correction direction, written by the network's last kernel
+          // as FP64 into the interior; its ghosts are never read (the
+          // Galerkin step's operator on it uses the homogeneous rule, and
+          // the update is interior)
+          ELSIM_TRY(nn_forward_g(s.ws_net, d.n1, d.n2, d.n3, xin, s.g(s.ws_cand), s.rec_x ? 16 : 8));
         }
-        double* pd = s.g(s.ws_cand);  // correction direction, zero ghosts
-        ELSIM_HIP(hipMemsetAsync(pd, 0, (size_t)d.G() * sizeof(double), stv));
-        k_ws_apply<<<grid, blk, 0, stv>>>(d, pd, (const float*)s.nn_pred.p, 1.0, pd);
-        ELSIM_CHECK_LAUNCH();
+        double* pd = s.g(s.ws_cand);
         double num = 0.0, den = 0.0;
         ELSIM_TRY(launch_galerkin_dots(ELSIM_OP_PHP, d, s.g(s.php), s.g(s.kapp), sc, s.g(s.rhsp), pd, &num, &den));
         const double alpha = (den < 0.0 && std::isfinite(num / den)) ? num / den : 0.0;  // den = p.L p < 0
