@@ -115,14 +115,24 @@ __global__ __launch_bounds__(256) void k_red_gw(Dims d, const double* __restrict
 
 // sum over the interior of (x - x0)^2 w (all ghosted); dx (optional): the
 // interior of x - x0 as well (the iteration's correction, for the next
-// coupling iteration's Galerkin step)
-__global__ __launch_bounds__(256) void k_red_cvg(Dims d, const double* __restrict__ x, const double* __restrict__ x0,
+// coupling iteration's Galerkin step).  e (optional): x still holds the
+// solve's start x0 and the solve's correction e is applied here first
+// (x <- x + e on the interior; the same values as a separate pass).
+__global__ __launch_bounds__(256) void k_red_cvg(Dims d, double* __restrict__ x, const double* __restrict__ x0,
                                                  const double* __restrict__ w, int op, SbmScal* s, elsim_sbm_params P,
-                                                 double* partials, double* __restrict__ dx) {
+                                                 double* partials, double* __restrict__ dx,
+                                                 const double* __restrict__ e) {
   const Col ci = colx(d);
   double acc[1] = {0.0};
   each(d, ci, [&](int64_t c, int64_t, int) {
-    const double t = x[c] - x0[c];
+    double t;
+    if (e) {  // x still holds the solve's start (x0): apply its correction here
+      const double xo = x[c], xn = xo + e[c];
+      x[c] = xn;
+      t = xn - xo;
+    } else {
+      t = x[c] - x0[c];
+    }
     if (dx) dx[c] = t;
     acc[0] += t * t * w[c];
   });
@@ -607,6 +617,7 @@ int solve(SbmState& s, int op, int mode, SolveResult* r) {
     a.tw = s.hs->tpe;
     a.rel_r0 = P.cne_rule == 1;
   } else if (op == ELSIM_OP_PHP) {
+    a.defer_add = true;  // applied with the convergence measure
     a.x = s.g(s.php);
     a.kap = s.g(s.kapp);
     a.pse = s.g(s.pse);
@@ -618,6 +629,7 @@ int solve(SbmState& s, int op, int mode, SolveResult* r) {
     a.w8 = s.g(s.psp);
     a.tw = s.hs->tpp;
   } else {
+    a.defer_add = true;
     a.x = s.g(s.phe);
     a.kap = s.g(s.pcl);
     a.pse = s.g(s.pse);
@@ -965,26 +977,29 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
     z.relres_php = r.relres;
     z.n_reliable += r.n_reliable;
     if (r.status) status = r.status;
+    const bool keep_dx = P.ws_mode != ELSIM_WS_PLAIN;  // the corrections, for the next iteration's steps
+    if (keep_dx) ELSIM_TRY(ensure_ws_buffers(s));
+    // CvgP (SBM:723) and the change, in the pass that applies the solve's
+    // correction when the solve left it; the ghosts follow from the interior
+    const double* e64 = (const double*)s.cg.e64.p;
+    k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.psp), R_CVGP, s.S(), P,
+                                     ws.partials, keep_dx ? s.g(s.outer_dp) : nullptr, r.e_pending ? e64 : nullptr);
+    ELSIM_CHECK_LAUNCH();
+    ELSIM_TRY(fin_sbm(s, R_CVGP));
     BcArgs pbc = mirror_bc();
     pbc.hi = 1;
     pbc.hi_v = P.bvp;
     ELSIM_TRY(bc(s, s.g(s.php), pbc));
-    const bool keep_dx = P.ws_mode != ELSIM_WS_PLAIN;  // the corrections, for the next iteration's steps
-    if (keep_dx) ELSIM_TRY(ensure_ws_buffers(s));
-    k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.psp), R_CVGP, s.S(), P,
-                                     ws.partials, keep_dx ? s.g(s.outer_dp) : nullptr);
-    ELSIM_CHECK_LAUNCH();
-    ELSIM_TRY(fin_sbm(s, R_CVGP));
     ELSIM_TRY(solve(s, ELSIM_OP_PHE, mode, &r));
     z.it_phe += r.iters;
     z.relres_phe = r.relres;
     z.n_reliable += r.n_reliable;
     if (r.status) status = r.status;
-    ELSIM_TRY(bc(s, s.g(s.phe), ebc));
     k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.phe), (const double*)s.cg.x0.p, s.g(s.pse), R_CVGL, s.S(), P,
-                                     ws.partials, keep_dx ? s.g(s.outer_de) : nullptr);
+                                     ws.partials, keep_dx ? s.g(s.outer_de) : nullptr, r.e_pending ? e64 : nullptr);
     ELSIM_CHECK_LAUNCH();
     ELSIM_TRY(fin_sbm(s, R_CVGL));
+    ELSIM_TRY(bc(s, s.g(s.phe), ebc));
     ELSIM_TRY(sync_scal(s));
     cvgp = s.hs->cvgp;
     cvgl = s.hs->cvgl;

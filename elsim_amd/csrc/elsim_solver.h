@@ -73,6 +73,7 @@ struct SolveArgs {
   double delta;         // mixed: reliable-update trigger
   int rel_r0 = 0;       // relative rule against ||r0|| (the start's residual) instead of ||b_eff||
   uint64_t coef_id = 0;  // nonzero: names kap (and pse); the multigrid set-up is reused while it is unchanged
+  bool defer_add = false;  // mixed modes: x stays as it came in, the correction is left in work.e64 (e_pending)
 };
 
 struct SolveResult {
@@ -81,6 +82,7 @@ struct SolveResult {
   int n_reliable = 0;
   double relres = 0.0;  // FP64 true relative residual of the correction system
   double res0 = 0.0;    // ||b - A x|| at the solve's start (FP64)
+  bool e_pending = false;  // defer_add: x += work.e64 (interior) is the caller's
 };
 
 // One CG solve of the correction system A_hom e = b - A x (x's ghosts as

@@ -898,6 +898,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   auto fin1 = [&](int op, int nb) { k_fin<1><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
   auto fin2 = [&](int op, int nb) { k_fin<2><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
   const bool fp32_inner = mode == ELSIM_MODE_MIXED || mode == ELSIM_MODE_MIXED_MG;
+  bool e_pending = false;  // defer_add: the correction is left in e64 for the caller
   // b_eff's norm is not used when the rule is relative to ||r0|| (CnE's
   // increment rule): the finaliser then only sets the tolerance
   if (a.rel_r0) {
@@ -1076,7 +1077,10 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
       }
       nrun = plan.next(*w.host, restart, 2);
     }
-    k_add_e<<<fb, FLAT_T, 0, s>>>(G, a.x, e64);
+    if (a.defer_add)
+      e_pending = true;
+    else
+      k_add_e<<<fb, FLAT_T, 0, s>>>(G, a.x, e64);
   } else {
     uint16_t* dinv = (uint16_t*)w.dinv32.p;  // BF16
     float *r32 = (float*)w.r32.p, *w32 = (float*)w.w32.p, *y32 = (float*)w.y32.p, *p32 = (float*)w.p32.p;
@@ -1155,11 +1159,15 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
       }
       nrun = plan.next(*w.host, restart, batch);
     }
-    k_add_e<<<fb, FLAT_T, 0, s>>>(G, a.x, e64);
+    if (a.defer_add)
+      e_pending = true;
+    else
+      k_add_e<<<fb, FLAT_T, 0, s>>>(G, a.x, e64);
   }
   ELSIM_CHECK_LAUNCH();
   ELSIM_TRY(poll(w, s));
   const CgDev& r = *w.host;
+  out->e_pending = e_pending;
   out->iters = r.iters;
   out->n_reliable = r.n_reliable;
   out->relres = r.bn2 > 0 ? std::sqrt(r.rr_true / r.bn2) : 0.0;
