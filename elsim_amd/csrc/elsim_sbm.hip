@@ -605,6 +605,9 @@ int solve(SbmState& s, int op, int mode, SolveResult* r) {
   a.stop_rule = P.stop_rule;
   a.rtol = P.rtol;
   a.delta = P.inner_rtol;
+  // the mixed solvers keep x fixed until their correction is applied, so
+  // the convergence measures need no copy of the start (k_red_cvg)
+  a.keep_x0 = false;
   if (op == ELSIM_OP_CNE) {
     a.x = s.g(s.cne);
     a.kap = s.g(s.pce);
@@ -981,8 +984,13 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
     if (keep_dx) ELSIM_TRY(ensure_ws_buffers(s));
     // CvgP (SBM:723) and the change, in the pass that applies the solve's
     // correction when the solve left it; the ghosts follow from the interior
+    // without a stored start and without a pending correction the solve
+    // left x as it came in (its start met the tolerance): x is its own start
     const double* e64 = (const double*)s.cg.e64.p;
-    k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.php), (const double*)s.cg.x0.p, s.g(s.psp), R_CVGP, s.S(), P,
+    auto start_of = [&](const SolveResult& q, const double* x) {
+      return q.x0_stored ? (const double*)s.cg.x0.p : x;
+    };
+    k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.php), start_of(r, s.g(s.php)), s.g(s.psp), R_CVGP, s.S(), P,
                                      ws.partials, keep_dx ? s.g(s.outer_dp) : nullptr, r.e_pending ? e64 : nullptr);
     ELSIM_CHECK_LAUNCH();
     ELSIM_TRY(fin_sbm(s, R_CVGP));
@@ -995,7 +1003,7 @@ int step_impl(SbmState& s, int mode, elsim_step_stats* st) {
     z.relres_phe = r.relres;
     z.n_reliable += r.n_reliable;
     if (r.status) status = r.status;
-    k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.phe), (const double*)s.cg.x0.p, s.g(s.pse), R_CVGL, s.S(), P,
+    k_red_cvg<<<grid, blk, 0, stv>>>(d, s.g(s.phe), start_of(r, s.g(s.phe)), s.g(s.pse), R_CVGL, s.S(), P,
                                      ws.partials, keep_dx ? s.g(s.outer_de) : nullptr, r.e_pending ? e64 : nullptr);
     ELSIM_CHECK_LAUNCH();
     ELSIM_TRY(fin_sbm(s, R_CVGL));

@@ -74,6 +74,7 @@ struct SolveArgs {
   int rel_r0 = 0;       // relative rule against ||r0|| (the start's residual) instead of ||b_eff||
   uint64_t coef_id = 0;  // nonzero: names kap (and pse); the multigrid set-up is reused while it is unchanged
   bool defer_add = false;  // mixed modes: x stays as it came in, the correction is left in work.e64 (e_pending)
+  bool keep_x0 = true;     // mixed modes: store the start's copy in work.x0 (the FP64 modes always do)
 };
 
 struct SolveResult {
@@ -83,11 +84,12 @@ struct SolveResult {
   double relres = 0.0;  // FP64 true relative residual of the correction system
   double res0 = 0.0;    // ||b - A x|| at the solve's start (FP64)
   bool e_pending = false;  // defer_add: x += work.e64 (interior) is the caller's
+  bool x0_stored = false;  // work.x0 holds the start's copy
 };
 
 // One CG solve of the correction system A_hom e = b - A x (x's ghosts as
 // stored), x <- x + e.  On return x0 (interior copy of x at entry) is in
-// work.x0.  mode = ELSIM_MODE_FP64 or ELSIM_MODE_MIXED.
+// work.x0 when SolveResult.x0_stored.  mode = ELSIM_MODE_FP64 or ELSIM_MODE_MIXED.
 int cg_solve(CgWork& w, const SolveArgs& a, int mode, SolveResult* res);
 
 // Single operator application (parity tests / roofline).  hom: in-kernel

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void k_matvec4(Dims d, const float* __restrict
 
 // Initial residual with x's ghosts as stored (SBM:492-499, 649-655, 729-735):
 // res = r0 = rhs - A x; prj(interior) = res; x0 = x(interior); cg->rr = sum res^2.
-// res and prj may be null (the mixed solvers start from r0 alone).
+// res and prj may be null (the mixed solvers start from r0 alone), and x0
+// (no copy of the start wanted).
 template <int OP>
 __global__ __launch_bounds__(256) void k_resid64(Dims d, const double* __restrict__ x, const double* __restrict__ kap,
                                                  const double* __restrict__ pse, double scale,
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(256) void k_resid64(Dims d, const double* __restric
       if (res) res[c] = r;
       r0[c] = r;
       if (prj) prj[c] = r;
-      x0[c] = xc;
+      if (x0) x0[c] = xc;
       acc[0] += r * r;
     });
   block_partials<1>(acc, partials);
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void k_resid64_beff(Dims d, const double* __re
       if (res) res[c] = r;
       r0[c] = r;
       if (prj) prj[c] = r;
-      x0[c] = xc;
+      if (x0) x0[c] = xc;
       acc[0] += r * r;
       const double b = r + vh;
       acc[1] += b * b;
@@ -899,16 +900,21 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   auto fin2 = [&](int op, int nb) { k_fin<2><<<1, FIN_T, 0, s>>>(op, nb, P, cg, nullptr, a.rtol); };
   const bool fp32_inner = mode == ELSIM_MODE_MIXED || mode == ELSIM_MODE_MIXED_MG;
   bool e_pending = false;  // defer_add: the correction is left in e64 for the caller
+  // the start's copy: the FP64 paths form their correction from it; the
+  // mixed paths, which keep x fixed and accumulate the correction, store it
+  // only on request
+  const bool store_x0 = !fp32_inner || a.keep_x0;
+  double* x0s = store_x0 ? x0 : nullptr;
   // b_eff's norm is not used when the rule is relative to ||r0|| (CnE's
   // increment rule): the finaliser then only sets the tolerance
   if (a.rel_r0) {
     k_resid64<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, fp32_inner ? nullptr : res, r0,
-                                       fp32_inner ? nullptr : prj, x0, cg, P);
+                                       fp32_inner ? nullptr : prj, x0s, cg, P);
     fin1(FIN_RESID, nb64);
     fin1(FIN_BEFF, 0);
   } else {
     k_resid64_beff<OP><<<nb64, blk, 0, s>>>(d, a.x, a.kap, a.pse, a.scale, a.rhs, fp32_inner ? nullptr : res, r0,
-                                            fp32_inner ? nullptr : prj, x0, P);
+                                            fp32_inner ? nullptr : prj, x0s, P);
     fin1(FIN_RESID, nb64);
     k_fin<1><<<1, FIN_T, 0, s>>>(FIN_BEFF, nb64, P + kMaxRedBlocks, cg, nullptr, a.rtol);  // slot 1
   }
@@ -1168,6 +1174,7 @@ int solve_op(CgWork& w, const SolveArgs& a, int mode, SolveResult* out) {
   ELSIM_TRY(poll(w, s));
   const CgDev& r = *w.host;
   out->e_pending = e_pending;
+  out->x0_stored = store_x0;
   out->iters = r.iters;
   out->n_reliable = r.n_reliable;
   out->relres = r.bn2 > 0 ? std::sqrt(r.rr_true / r.bn2) : 0.0;
